@@ -1,0 +1,385 @@
+"""Bolt.jl's hot-path API on top of libbolt_hip.so.
+
+Names, argument meaning and defaults follow Bolt.jl (citations into src/):
+CosmoParams (Bolt.jl:56-66), Background (background.jl:104-128),
+IonizationHistory(RECFAST(bg), par, bg) (recfast.jl:674-726), Hierarchy
+(perturbations.jl:7-21), boltsolve / boltsolve_rsa (perturbations.jl:25-111),
+source_grid (spectra.jl:6-23), quadratic_k / log10_k (spectra.jl:60-68),
+cltt (spectra.jl:84-96, 132-150) and plin (spectra.jl:163-198).
+
+The background and ionization history are computed on the host (C++, FP64)
+and uploaded once per cosmology; everything downstream runs on the GPU.
+
+Python identifiers cannot hold subscript digits, primes or superscripts, so
+Bolt's bg.ℋ′ / bg.η₀ / ih.csb² ... are spelled bg.H_p / bg.eta0 / ih.csb2 here
+(bg.H, bg.H_p, bg.H_pp, bg.eta, bg.eta_p, bg.eta_pp, bg.rho0M, bg.H0, bg.eta0,
+bg.rho_crit, bg.Omega_L; ih.Xe, ih.tau, ih.tau_p, ih.tau_pp, ih.g, ih.g_p,
+ih.g_pp, ih.Tmat, ih.csb2, ih.tau0).
+"""
+import ctypes
+import math
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib, pd, pi
+
+# natural units (c = ħ = k_B = 1, Mpc): 1 eV in Mpc^-1 (Bolt.jl:51)
+MASS_NATURAL = 1.602176634e-19 / (1.054571817e-34 * 299792458.0) * 3.0856775814913673e22
+
+
+@dataclass
+class CosmoParams:
+    h: float = 0.7
+    Ω_r: float = 5.0469e-5
+    Ω_b: float = 0.046
+    Ω_c: float = 0.224
+    A: float = 2.097e-9
+    n: float = 1.0
+    Y_p: float = 0.24
+    N_ν: float = 3.046
+    Σm_ν: float = 0.06 * MASS_NATURAL
+
+    def as_tuple(self):
+        return (self.h, self.Ω_r, self.Ω_b, self.Ω_c, self.A, self.n, self.Y_p, self.N_ν, self.Σm_ν)
+
+    def c_struct(self):
+        return _abi.CosmoParamsC(*[float(v) for v in self.as_tuple()])
+
+
+class _Spline:
+    """A table of the background / ionization history, callable at x."""
+
+    def __init__(self, owner, tab, deriv=0):
+        self._o, self._t, self._d = owner, tab, deriv
+
+    def __call__(self, x):
+        xs = np.atleast_1d(np.asarray(x, dtype=np.float64)).copy()
+        out = np.empty_like(xs)
+        check(lib().bolt_host_spline_eval(ctypes.byref(self._o.c_tables()), self._t, self._d, pd(xs), len(xs),
+                                          pd(out)))
+        return out if np.ndim(x) else float(out[0])
+
+
+class Background:
+    """Background(par; x_grid=-20.0:0.01:0.0, nq=15) with the RECFAST
+    ionization history tables computed alongside (both host-side)."""
+
+    def __init__(self, par: CosmoParams, x_grid=(-20.0, 0.0, 2001), nq=15):
+        x0, x1, nx = x_grid
+        self.par = par
+        self.nx, self.nq, self.x0, self.x1 = int(nx), int(nq), float(x0), float(x1)
+        self.coef = np.zeros((_abi.NTAB, self.nx + 2))
+        self.scal = np.zeros(_abi.NSCAL)
+        self.quad_pts = np.zeros(self.nq)
+        self.quad_wts = np.zeros(self.nq)
+        check(lib().bolt_host_tables(ctypes.byref(par.c_struct()), self.x0, self.x1, self.nx, self.nq,
+                                     pd(self.coef), pd(self.scal), pd(self.quad_pts), pd(self.quad_wts)))
+        self.H0 = self.scal[_abi.S_H0]
+        self.eta0 = self.scal[_abi.S_ETA0]
+        self.rho_crit = self.scal[_abi.S_RHO_CRIT]
+        self.Omega_L = self.scal[_abi.S_OMEGA_L]
+        self.x_grid = np.array([(self.x0 * (self.nx - 1 - j) + self.x1 * j) / (self.nx - 1)
+                                for j in range(self.nx)])
+        self.H = _Spline(self, _abi.T_HC)
+        self.H_p = _Spline(self, _abi.T_HC_P)
+        self.H_pp = _Spline(self, _abi.T_HC_PP)
+        self.eta = _Spline(self, _abi.T_ETA)
+        self.eta_p = _Spline(self, _abi.T_ETA_P)
+        self.eta_pp = _Spline(self, _abi.T_ETA_PP)
+        self.rho0M = _Spline(self, _abi.T_RHO0M)
+        self._ct = None
+        self._uploads = {}
+
+    def c_tables(self):
+        if self._ct is None:
+            self._ct = _abi.TablesC(self.nx, self.nq, self.x0, self.x1, (ctypes.c_double * _abi.NSCAL)(*self.scal),
+                                    pd(self.quad_pts), pd(self.quad_wts), pd(self.coef))
+        return self._ct
+
+    def tables_dict(self):
+        return dict(nx=self.nx, nq=self.nq, x0=self.x0, x1=self.x1, H0=self.H0, eta0=self.eta0,
+                    rho_crit=self.rho_crit, xq=self.quad_pts, wq=self.quad_wts, coef=self.coef)
+
+    def cosmo_id(self, ctx):
+        key = id(ctx)
+        if key not in self._uploads:
+            cid = ctypes.c_int(-1)
+            check(lib().bolt_cosmo_upload(ctx.handle, ctypes.byref(self.par.c_struct()),
+                                          ctypes.byref(self.c_tables()), ctypes.byref(cid)))
+            self._uploads[key] = cid.value
+        return self._uploads[key]
+
+
+class RECFAST:
+    """RECFAST(bg; kws...) marker (recfast.jl:22-124); the default constants are
+    the only configuration the hot path consumes."""
+
+    def __init__(self, bg: Background):
+        self.bg = bg
+
+
+class IonizationHistory:
+    """IonizationHistory(RECFAST(bg), par, bg): the ionization tables."""
+
+    def __init__(self, rec: RECFAST, par: CosmoParams, bg: Background):
+        self.bg = bg
+        self.tau0 = bg.scal[_abi.S_TAU0]
+        self.Xe = _Spline(bg, _abi.T_XE)
+        self.tau = _Spline(bg, _abi.T_TAU)
+        self.tau_p = _Spline(bg, _abi.T_TAU_P)
+        self.tau_pp = _Spline(bg, _abi.T_TAU_PP)
+        self.g = _Spline(bg, _abi.T_G)
+        self.g_p = _Spline(bg, _abi.T_G_P)
+        self.g_pp = _Spline(bg, _abi.T_G_PP)
+        self.Tmat = _Spline(bg, _abi.T_TMAT)
+        self.csb2 = _Spline(bg, _abi.T_CSB2)
+
+
+def recfast_xe(par: CosmoParams, z, Tnow=-1.0):
+    """Xe_RECFAST on redshifts z (recfast.jl:413-433)."""
+    z = np.ascontiguousarray(np.atleast_1d(z), dtype=np.float64)
+    out = np.zeros_like(z)
+    check(lib().bolt_host_recfast_xe(ctypes.byref(par.c_struct()), Tnow, pd(z), len(z), pd(out)))
+    return out
+
+
+# ------------------------------------------------------------------- device
+class Context:
+    """One HIP context (device + stream) of libbolt_hip."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().bolt_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.lock = threading.Lock()
+
+    def close(self):
+        if self.handle:
+            lib().bolt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class PerturbationIntegrator:
+    pass
+
+
+class BasicNewtonian(PerturbationIntegrator):
+    """Bolt's Newtonian-gauge hierarchy; here it runs on the GPU."""
+
+
+HIPNewtonian = BasicNewtonian
+
+
+@dataclass
+class Hierarchy:
+    integrator: PerturbationIntegrator
+    par: CosmoParams
+    bg: Background
+    ih: IonizationHistory
+    k: float
+    ℓᵧ: int = 8
+    ℓ_ν: int = 8
+    ℓ_mν: int = 10
+    nq: int = 15
+
+    def cfg(self, reltol=1e-6, abstol=1e-6, max_steps=0):
+        return hier_cfg(self.ℓᵧ, self.ℓ_ν, self.ℓ_mν, self.nq, reltol, abstol, max_steps)
+
+    @property
+    def n(self):
+        return 2 * (self.ℓᵧ + 1) + (self.ℓ_ν + 1) + (self.ℓ_mν + 1) * self.nq + 5
+
+
+def hier_cfg(ℓᵧ=8, ℓ_ν=8, ℓ_mν=10, nq=15, reltol=1e-6, abstol=1e-6, max_steps=0):
+    return _abi.HierCfgC(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol, max_steps, 0)
+
+
+def state_length(cfg):
+    return 2 * (cfg.ell_g + 1) + (cfg.ell_nu + 1) + (cfg.ell_mnu + 1) * cfg.nq + 5
+
+
+class Solution:
+    """Saved states of boltsolve; sol(x) for x in the saved set."""
+
+    def __init__(self, xs, states, stats):
+        self.t = np.asarray(xs)
+        self.u = states
+        self.stats = stats
+
+    def __call__(self, x):
+        i = np.nonzero(self.t == x)[0]
+        if len(i) == 0:
+            raise KeyError(f"x = {x} was not in saveat")
+        return self.u[i[0]]
+
+
+def _stats_dict(st):
+    return dict(accepted=st.steps_accepted, rejected=st.steps_rejected, rhs_evals=st.rhs_evals,
+                linear_solves=st.linear_solves, kernel_ms=st.kernel_ms)
+
+
+def rhs_batch(par, bg, cfg, ks, x, U, mutate=False, ctx=None):
+    """hierarchy! for modes ks at x; U: [n][B] (mode fastest). Returns (dU, U')."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    U = np.array(U, dtype=np.float64, order="C")
+    dU = np.zeros_like(U)
+    with ctx.lock:
+        check(lib().bolt_rhs_batch(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), x, pd(U), pd(dU),
+                                   1 if mutate else 0))
+    return dU, U
+
+
+def initial_conditions_batch(par, bg, cfg, ks, ctx=None):
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    U = np.zeros((state_length(cfg), len(ks)))
+    with ctx.lock:
+        check(lib().bolt_initial_conditions(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(U)))
+    return U
+
+
+def boltsolve_batch(par, bg, cfg, ks, x_save, ctx=None, raise_on_fail=True):
+    """States [B][nsave][n] at x_save for modes ks, plus per-mode status and stats."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    xs = np.ascontiguousarray(x_save, dtype=np.float64)
+    n = state_length(cfg)
+    out = np.zeros((len(ks), len(xs), n))
+    status = np.zeros(len(ks), dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        rc = lib().bolt_boltsolve_batch(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(xs),
+                                        len(xs), pd(out), pi(status), ctypes.byref(st))
+    if raise_on_fail or rc != _abi.BOLT_E_NUMERIC:
+        check(rc)
+    return out, status, _stats_dict(st)
+
+
+def boltsolve(h: Hierarchy, ode_alg=None, reltol=1e-6, abstol=1e-6, saveat=None, ctx=None):
+    """boltsolve(hierarchy; reltol, abstol) (perturbations.jl:25-33); the
+    solution is kept at `saveat` (default: the background's x grid)."""
+    xs = h.bg.x_grid if saveat is None else np.asarray(saveat, dtype=np.float64)
+    out, status, st = boltsolve_batch(h.par, h.bg, h.cfg(reltol, abstol), [h.k], xs, ctx)
+    return Solution(xs, out[0], st)
+
+
+def boltsolve_rsa(h: Hierarchy, ode_alg=None, reltol=1e-6, abstol=1e-6, ctx=None):
+    """results[pertlen, nx] with the RSA overwrite (perturbations.jl:86-111)."""
+    ctx = ctx or default_context()
+    cfg = h.cfg(reltol, abstol)
+    ks = np.array([h.k], dtype=np.float64)
+    out = np.zeros((1, h.bg.nx, h.n))
+    status = np.zeros(1, dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        check(lib().bolt_boltsolve_rsa_batch(ctx.handle, h.bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), 1, pd(out),
+                                             pi(status), ctypes.byref(st)))
+    return out[0].T.copy()
+
+
+class SourceGrid:
+    """The interpolated source S(x, k): Gridded linear in x and k with Line
+    extrapolation, like LinearInterpolation((x_grid, k_grid), grid,
+    extrapolation_bc=Line()) (spectra.jl:21)."""
+
+    def __init__(self, x_grid, k_grid, grid, stats=None):
+        self.x_grid = np.asarray(x_grid)
+        self.k_grid = np.asarray(k_grid)
+        self.grid = grid          # [nk][nx]
+        self.stats = stats
+
+    @staticmethod
+    def _locate(g, v):
+        i = int(np.clip(np.searchsorted(g, v, side="right") - 1, 0, len(g) - 2))
+        return i, (v - g[i]) / (g[i + 1] - g[i])
+
+    def __call__(self, x, k):
+        ix, tx = self._locate(self.x_grid, x)
+        ik, tk = self._locate(self.k_grid, k)
+        g = self.grid
+        a = (1 - tx) * g[ik][ix] + tx * g[ik][ix + 1]
+        b = (1 - tx) * g[ik + 1][ix] + tx * g[ik + 1][ix + 1]
+        return (1 - tk) * a + tk * b
+
+
+def source_grid(par, bg, ih, k_grid, integrator=None, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν=10, nq=15, abstol=1e-6,
+                ctx=None):
+    """source_grid(par, bg, ih, k_grid, BasicNewtonian(); ℓᵧ=8, reltol=1e-11)
+    (spectra.jl:6-23).  As in Bolt, ℓ_ν, ℓ_mν and nq keep the Hierarchy
+    defaults unless given."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(k_grid, dtype=np.float64)
+    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol)
+    S = np.zeros((len(ks), bg.nx))
+    status = np.zeros(len(ks), dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        check(lib().bolt_source_grid(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(S),
+                                     pi(status), ctypes.byref(st)))
+    return SourceGrid(bg.x_grid, ks, S, _stats_dict(st))
+
+
+def quadratic_k(kmin, kmax, nk):
+    """quadratic_k (spectra.jl:60-63): k_i = kmin + (kmax-kmin)(i/nk)², i = 1..nk."""
+    return np.array([kmin + (kmax - kmin) * (i / nk) ** 2 for i in range(1, nk + 1)])
+
+
+def log10_k(kmin, kmax, nk):
+    """log10_k (spectra.jl:65-68)."""
+    return np.array([10 ** (math.log10(kmin) + math.log10(kmax / kmin) * (i - 1) / (nk - 1))
+                     for i in range(1, nk + 1)])
+
+
+def cltt_on_grid(ells, sf: SourceGrid, kgrid, par, bg, bessel_mode=0, ctx=None):
+    """cltt(ℓ, s_itp, kgrid, par, bg) (spectra.jl:84-96) for a list of ℓ."""
+    ctx = ctx or default_context()
+    ells_a = np.ascontiguousarray(np.atleast_1d(ells), dtype=np.int32)
+    S = np.ascontiguousarray(sf.grid, dtype=np.float64)
+    ksrc = np.ascontiguousarray(sf.k_grid, dtype=np.float64)
+    kg = np.ascontiguousarray(kgrid, dtype=np.float64)
+    cl = np.zeros(len(ells_a))
+    with ctx.lock:
+        check(lib().bolt_cltt(ctx.handle, bg.cosmo_id(ctx), pd(S), pd(ksrc), len(ksrc), pd(kg), len(kg), pi(ells_a),
+                              len(ells_a), bessel_mode, pd(cl)))
+    return cl if np.ndim(ells) else float(cl[0])
+
+
+def cltt(ells, par, bg, ih, sf, ctx=None):
+    """cltt(ℓ, par, bg, ih, sf) and cltt(ℓ⃗, ...) (spectra.jl:132-150): the dense
+    grid is quadratic_k(0.01H₀, 1000H₀, 5000)."""
+    dense = quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
+    return cltt_on_grid(ells, sf, dense, par, bg, 0, ctx)
+
+
+def plin(k, par, bg, ih, n_q=15, ℓᵧ=50, ℓ_ν=50, ℓ_mν=20, x=0.0, reltol=1e-5, abstol=1e-6, ctx=None):
+    """plin (spectra.jl:163-198); k may be a scalar or an array (one batched launch)."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(np.atleast_1d(k), dtype=np.float64)
+    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, n_q, reltol, abstol)
+    pk = np.zeros(len(ks))
+    status = np.zeros(len(ks), dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        check(lib().bolt_plin(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), float(x), pd(pk),
+                              pi(status), ctypes.byref(st)))
+    return pk if np.ndim(k) else float(pk[0])

@@ -1,0 +1,1688 @@
+// libbolt_hip: MI355X (gfx950) kernels for Bolt.jl's perturbation hot path and
+// the C ABI declared in include/bolt_hip.h.
+//
+// Integrator design.  hierarchy! (src/perturbations.jl:161-271) is linear in
+// the state: du = A(x) u.  Writing A = L + c_S sᵀ + c_P pᵀ + c_Π πᵀ, where L
+// couples each hierarchy only to its ℓ±1 neighbours (plus the baryon/photon
+// tight-coupling pair) and s, p, π are the Ψ, Φ′ and Π functionals, the
+// KenCarp4 stage equation (I - hγA) z = r is solved exactly by
+//   * a banded LU (bandwidth <= 2, no pivoting: every chain is a positive
+//     diagonal plus a skew-symmetrisable coupling) of I - hγL, one lane per chain,
+//   * a 3x3 capacitance system for the Ψ/Φ′/Π border (Woodbury).
+// One wavefront integrates one k-mode; state vectors live in LDS in "chain
+// order" (each hierarchy contiguous); a persistent grid pulls modes from an
+// atomic work counter, costliest (largest k) first.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bolt_hip.h"
+
+namespace bolt {
+int set_error(int code, const std::string& msg);
+}
+using bolt::set_error;
+
+#define WAVE 64
+
+// ---------------------------------------------------------------- device data
+struct CosmoDev {
+    double h, Or, Ob, Oc, A, ns, Yp, Nnu, mnu;
+    double H0, eta0, rho_crit, Onu;  // Onu = 7*(2/3)*N_ν/8*(4/11)^(4/3)*Ω_r
+    double x0, x1, dx;
+    int nx, nq;
+    const double* coef;    // [16][nx+2]
+    const double* qv;      // [nq] q_i
+    const double* df0;     // [nq] dlnf0dlnq(q_i)
+    const double* base;    // [nq] 4π q_i² f0(q_i)/dxdq(q_i) w_i
+    const double* thbase;  // [nq] 4π q_i³ f0(q_i)/dxdq(q_i) w_i
+    double Pprim_A, Pprim_n;
+    double Om_nu_nr;       // massive-ν Ω used by plin (spectra.jl:182-187)
+};
+
+struct CfgDev {
+    int ng, nn, nm, nq, n;
+    double reltol, abstol;
+    int max_steps;
+};
+
+struct ModeDesc {
+    double k;
+    int cosmo, cfg, out;
+    int pad;
+};
+
+// geometry of the chain-ordered state
+struct Geo {
+    int ng, nn, nm, nq, n;
+    int cB, cC, cM, cD, cE, nch;
+};
+
+__device__ __host__ inline Geo make_geo(const CfgDev& c) {
+    Geo g;
+    g.ng = c.ng; g.nn = c.nn; g.nm = c.nm; g.nq = c.nq; g.n = c.n;
+    g.cB = g.ng + 3;                 // chain A: δb, vb, Θ0..Θℓᵧ
+    g.cC = g.cB + g.ng + 1;          // chain B: Θᵖ0..Θᵖℓᵧ
+    g.cM = g.cC + g.nn + 1;          // chain C: 𝒩0..𝒩ℓ_ν
+    g.cD = g.cM + g.nq * (g.nm + 1); // chains M_i: ℳ0,i..ℳℓ_mν,i
+    g.cE = g.cD + 2;                 // chain D: δ, v ; chain E: Φ
+    g.nch = 3 + g.nq + 2;
+    return g;
+}
+
+// chain-order index -> Bolt's state index (perturbations.jl:114-125)
+__device__ __host__ inline int chain_to_state(const Geo& g, int c) {
+    int iN = 2 * (g.ng + 1), iM = iN + g.nn + 1, iPhi = iM + (g.nm + 1) * g.nq;
+    if (c < g.cB) {
+        if (c == 0) return iPhi + 3;  // δ_b
+        if (c == 1) return iPhi + 4;  // v_b
+        return c - 2;                 // Θℓ
+    }
+    if (c < g.cC) return g.ng + 1 + (c - g.cB);
+    if (c < g.cM) return iN + (c - g.cC);
+    if (c < g.cD) {
+        int t = c - g.cM, qi = t / (g.nm + 1), l = t % (g.nm + 1);
+        return iM + l * g.nq + qi;
+    }
+    if (c == g.cD) return iPhi + 1;  // δ
+    if (c == g.cD + 1) return iPhi + 2;  // v
+    return iPhi;                          // Φ
+}
+
+__device__ inline void chain_range(const Geo& g, int ch, int& c0, int& m) {
+    if (ch == 0) { c0 = 0; m = g.ng + 3; }
+    else if (ch == 1) { c0 = g.cB; m = g.ng + 1; }
+    else if (ch == 2) { c0 = g.cC; m = g.nn + 1; }
+    else if (ch < 3 + g.nq) { c0 = g.cM + (ch - 3) * (g.nm + 1); m = g.nm + 1; }
+    else if (ch == 3 + g.nq) { c0 = g.cD; m = 2; }
+    else { c0 = g.cE; m = 1; }
+}
+
+// uniform cubic B-spline (src/util.jl:11-13)
+__device__ inline double spl(const CosmoDev& C, int tab, double x, int deriv = 0) {
+    double t = (x - C.x0) / C.dx;
+    int i = (int)floor(t);
+    i = i < 0 ? 0 : (i > C.nx - 2 ? C.nx - 2 : i);
+    double s = t - i, r = 1.0 - s;
+    const double* q = C.coef + (size_t)tab * (C.nx + 2) + i;
+    if (deriv == 0)
+        return r * r * r / 6.0 * q[0] + (2.0 / 3.0 - s * s + 0.5 * s * s * s) * q[1] +
+               (2.0 / 3.0 - r * r + 0.5 * r * r * r) * q[2] + s * s * s / 6.0 * q[3];
+    if (deriv == 1)
+        return (-0.5 * r * r * q[0] + (-2.0 * s + 1.5 * s * s) * q[1] + (2.0 * r - 1.5 * r * r) * q[2] +
+                0.5 * s * s * q[3]) / C.dx;
+    return (r * q[0] + (-2.0 + 3.0 * s) * q[1] + (-2.0 + 3.0 * r) * q[2] + s * q[3]) / (C.dx * C.dx);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < WAVE; m <<= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int m = 1; m < WAVE; m <<= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
+    return v;
+}
+
+// x-dependent coefficients of hierarchy! (perturbations.jl:163-172)
+struct XC {
+    double x, a, Hc, eta, tp, tpp, csb2, kH, R, c12, gg, k23;
+    bool rsa;
+};
+
+__device__ inline XC make_xc(const CosmoDev& C, double k, double x) {
+    XC X;
+    X.x = x;
+    X.a = exp(x);
+    X.Hc = spl(C, BOLT_T_HC, x);
+    X.eta = spl(C, BOLT_T_ETA, x);
+    X.tp = spl(C, BOLT_T_TAU_P, x);
+    X.tpp = spl(C, BOLT_T_TAU_PP, x);
+    X.csb2 = spl(C, BOLT_T_CSB2, x);
+    X.kH = k / X.Hc;
+    X.R = 4.0 * C.Or / (3.0 * C.Ob * X.a);
+    double H02 = C.H0 * C.H0;
+    X.c12 = 12.0 * H02 / (k * k) / (X.a * X.a);
+    X.gg = H02 / (2.0 * X.Hc * X.Hc);
+    X.k23 = k * k / (3.0 * X.Hc * X.Hc);
+    X.rsa = (k * X.eta > 240.0) && (-X.tp * X.Hc / X.eta > 100.0);  // perturbations.jl:216
+    return X;
+}
+
+// LDS workspace of one wavefront
+struct WS {
+    double *u, *f, *un, *fn, *arg, *w, *Z[6];
+    double *l2, *l1, *d, *u1, *u2, *cS, *cP, *cPi;
+    double *q, *df0, *base, *qe, *eqdf, *sw, *rw;
+};
+
+__device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
+    double* p = lds;
+    W.u = p; p += n; W.f = p; p += n; W.un = p; p += n; W.fn = p; p += n;
+    W.arg = p; p += n; W.w = p; p += n;
+    for (int i = 0; i < 6; ++i) { W.Z[i] = p; p += n; }
+    W.l2 = p; p += n; W.l1 = p; p += n; W.d = p; p += n; W.u1 = p; p += n; W.u2 = p; p += n;
+    W.cS = p; p += n; W.cP = p; p += n; W.cPi = p; p += n;
+    W.q = p; p += nq; W.df0 = p; p += nq; W.base = p; p += nq;
+    W.qe = p; p += nq; W.eqdf = p; p += nq; W.sw = p; p += nq; W.rw = p; p += nq;
+}
+__host__ inline size_t ws_bytes(int n, int nq) { return sizeof(double) * ((size_t)20 * n + 7 * nq); }
+
+// per-x q-quantities (perturbations.jl:203-213, 127-145); lanes < nq; caller syncs
+__device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, WS& W, int lane) {
+    if (lane < G.nq) {
+        double q = W.q[lane];
+        double am = X.a * C.mnu;
+        double eps = sqrt(q * q + am * am);
+        W.qe[lane] = q / eps;
+        W.eqdf[lane] = eps / q * W.df0[lane];
+        W.sw[lane] = W.base[lane] * (q * q / eps);
+        W.rw[lane] = W.base[lane] * eps;
+    }
+}
+
+// L bands and border columns c_S, c_P, c_Π at X, chain order
+__device__ void assemble(const CosmoDev& C, const Geo& G, const XC& X, WS& W, int lane) {
+    const double kH = X.kH, tp = X.tp;
+    const bool rsa = X.rsa;
+    const double Heta = X.Hc * X.eta;
+    for (int c = lane; c < G.n; c += WAVE) {
+        double l2 = 0, l1 = 0, d = 0, u1 = 0, u2 = 0, cs = 0, cp = 0, cpi = 0;
+        if (c < G.cB) {  // chain A: δb, vb, Θ
+            if (c == 0) { u1 = kH; cp = -3.0; }
+            else if (c == 1) { l1 = -kH * X.csb2; d = -1.0 + tp * X.R; u2 = 3.0 * tp * X.R; cs = -kH; }
+            else if (!rsa) {
+                int l = c - 2;
+                if (l == 0) { u1 = -kH; cp = -1.0; }
+                else if (l == 1) { l2 = tp / 3.0; l1 = kH / 3.0; d = tp; u1 = -2.0 * kH / 3.0; cs = kH / 3.0; }
+                else if (l < G.ng) {
+                    l1 = l * kH / (2 * l + 1); d = tp; u1 = -(l + 1) * kH / (2 * l + 1);
+                    if (l == 2) cpi = -tp / 10.0;
+                } else { l1 = kH; d = -((G.ng + 1) / Heta - tp); }
+            }
+        } else if (c < G.cC) {  // chain B: Θᵖ
+            if (!rsa) {
+                int l = c - G.cB;
+                if (l == 0) { d = tp; u1 = -kH; cpi = -tp / 2.0; }
+                else if (l < G.ng) {
+                    l1 = l * kH / (2 * l + 1); d = tp; u1 = -(l + 1) * kH / (2 * l + 1);
+                    if (l == 2) cpi = -tp / 10.0;
+                } else { l1 = kH; d = -((G.ng + 1) / Heta - tp); }
+            }
+        } else if (c < G.cM) {  // chain C: 𝒩
+            if (!rsa) {
+                int l = c - G.cC;
+                if (l == 0) { u1 = -kH; cp = -1.0; }
+                else if (l == 1) { l1 = kH / 3.0; u1 = -2.0 * kH / 3.0; cs = kH / 3.0; }
+                else if (l < G.nn) { l1 = l * kH / (2 * l + 1); u1 = -(l + 1) * kH / (2 * l + 1); }
+                else { l1 = kH; d = -(G.nn + 1) / Heta; }
+            }
+        } else if (c < G.cD) {  // chains M_i
+            int t = c - G.cM, qi = t / (G.nm + 1), l = t % (G.nm + 1);
+            double qe = W.qe[qi];
+            if (l == 0) { u1 = -kH * qe; cp = W.df0[qi]; }
+            else if (l == 1) { l1 = kH * qe / 3.0; u1 = -2.0 * kH * qe / 3.0; cs = -kH / 3.0 * W.eqdf[qi]; }
+            else if (l < G.nm) { l1 = kH * qe * l / (2 * l + 1); u1 = -kH * qe * (l + 1) / (2 * l + 1); }
+            else { l1 = qe * kH; d = -(G.nm + 1) / Heta; }
+        } else if (c == G.cD) { u1 = kH; cp = -3.0; }      // δ
+        else if (c == G.cD + 1) { d = -1.0; cs = -kH; }   // v
+        else { cp = 1.0; }                                  // Φ
+        W.l2[c] = l2; W.l1[c] = l1; W.d[c] = d; W.u1[c] = u1; W.u2[c] = u2;
+        W.cS[c] = cs; W.cP[c] = cp; W.cPi[c] = cpi;
+    }
+}
+
+// Ψ, Φ′, Π of a chain-ordered vector (perturbations.jl:184-194, 247)
+__device__ inline void functionals(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
+                                   int lane, double& S, double& P, double& Pi) {
+    double s2 = 0.0, s0 = 0.0;
+    if (lane < G.nq) {
+        int b = G.cM + lane * (G.nm + 1);
+        s2 = W.sw[lane] * v[b + 2];
+        s0 = W.rw[lane] * v[b];
+    }
+    s2 = wave_sum(s2);
+    s0 = wave_sum(s0);
+    double a = X.a;
+    S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2] + s2 / C.rho_crit / 4.0);
+    P = S - X.k23 * v[G.cE] +
+        X.gg * (C.Oc / a * v[G.cD] + C.Ob / a * v[0] + 4.0 * C.Or / (a * a) * v[2] +
+                4.0 * C.Onu / (a * a) * v[G.cC] + s0 / C.rho_crit / (a * a));
+    Pi = v[4] + v[G.cB + 2] + v[G.cB];
+}
+
+// out = scale * (L v + c_S S + c_P P + c_Π Π)
+__device__ inline void apply_rhs(const Geo& G, const WS& W, const double* v, double S, double P, double Pi,
+                                 double scale, double* out, int lane) {
+    for (int c = lane; c < G.n; c += WAVE) {
+        double acc = W.d[c] * v[c];
+        if (c >= 1) acc += W.l1[c] * v[c - 1];
+        if (c >= 2) acc += W.l2[c] * v[c - 2];
+        if (c + 1 < G.n) acc += W.u1[c] * v[c + 1];
+        if (c + 2 < G.n) acc += W.u2[c] * v[c + 2];
+        acc += W.cS[c] * S + W.cP[c] * P + W.cPi[c] * Pi;
+        out[c] = scale * acc;
+    }
+}
+
+// RSA overwrite of u (perturbations.jl:217-227); one lane; caller syncs around
+__device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v, double Phip) {
+    double Phi = v[G.cE], vb = v[1], db = v[0];
+    double Hc = X.Hc, tp = X.tp;
+    v[2] = Phi - Hc / k * tp * vb;
+    v[3] = Hc / k * (-2.0 * Phip + tp * (Phi - X.csb2 * db) + Hc / k * (X.tpp - tp) * vb);
+    v[4] = 0.0;
+    v[G.cC] = Phi;
+    v[G.cC + 1] = -2.0 * Hc / k * Phip;
+    v[G.cC + 2] = 0.0;
+}
+
+// I - hγL in place
+__device__ inline void to_w(const Geo& G, WS& W, double hg, int lane) {
+    for (int c = lane; c < G.n; c += WAVE) {
+        W.l2[c] *= -hg; W.l1[c] *= -hg; W.u1[c] *= -hg; W.u2[c] *= -hg;
+        W.d[c] = 1.0 - hg * W.d[c];
+    }
+}
+
+__device__ inline double rcp(double d) { return 1.0 / d; }
+
+// factor each chain (no pivoting, bandwidth 2) and solve for w, cS, cP, cPi
+__device__ void chain_factor_solve4(const Geo& G, WS& W, int lane) {
+    if (lane >= G.nch) return;
+    int c0, m;
+    chain_range(G, lane, c0, m);
+    double* l2 = W.l2; double* l1 = W.l1; double* d = W.d; double* u1 = W.u1; double* u2 = W.u2;
+    double* b0 = W.w; double* b1 = W.cS; double* b2 = W.cP; double* b3 = W.cPi;
+    for (int j = 0; j < m; ++j) {
+        int c = c0 + j;
+        double inv = rcp(d[c]);
+        d[c] = inv;
+        double x0 = b0[c], x1 = b1[c], x2 = b2[c], x3 = b3[c];
+        double uu1 = u1[c], uu2 = u2[c];
+        if (j + 1 < m) {
+            double m1 = l1[c + 1] * inv;
+            l1[c + 1] = m1;
+            d[c + 1] -= m1 * uu1;
+            u1[c + 1] -= m1 * uu2;
+            b0[c + 1] -= m1 * x0; b1[c + 1] -= m1 * x1; b2[c + 1] -= m1 * x2; b3[c + 1] -= m1 * x3;
+        }
+        if (j + 2 < m) {
+            double m2 = l2[c + 2] * inv;
+            l2[c + 2] = m2;
+            l1[c + 2] -= m2 * uu1;
+            d[c + 2] -= m2 * uu2;
+            b0[c + 2] -= m2 * x0; b1[c + 2] -= m2 * x1; b2[c + 2] -= m2 * x2; b3[c + 2] -= m2 * x3;
+        }
+    }
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // x[c+1], x[c+2]
+    for (int j = m - 1; j >= 0; --j) {
+        int c = c0 + j;
+        double inv = d[c], a1 = (j + 1 < m) ? u1[c] : 0.0, a2 = (j + 2 < m) ? u2[c] : 0.0;
+        double y0 = (b0[c] - a1 * p0 - a2 * q0) * inv;
+        double y1 = (b1[c] - a1 * p1 - a2 * q1) * inv;
+        double y2 = (b2[c] - a1 * p2 - a2 * q2) * inv;
+        double y3 = (b3[c] - a1 * p3 - a2 * q3) * inv;
+        b0[c] = y0; b1[c] = y1; b2[c] = y2; b3[c] = y3;
+        q0 = p0; q1 = p1; q2 = p2; q3 = p3;
+        p0 = y0; p1 = y1; p2 = y2; p3 = y3;
+    }
+}
+
+// solve with stored factors for one right-hand side
+__device__ void chain_solve1(const Geo& G, const WS& W, double* b, int lane) {
+    if (lane >= G.nch) return;
+    int c0, m;
+    chain_range(G, lane, c0, m);
+    for (int j = 0; j < m; ++j) {
+        int c = c0 + j;
+        double x = b[c];
+        if (j + 1 < m) b[c + 1] -= W.l1[c + 1] * x;
+        if (j + 2 < m) b[c + 2] -= W.l2[c + 2] * x;
+    }
+    double p = 0, q = 0;
+    for (int j = m - 1; j >= 0; --j) {
+        int c = c0 + j;
+        double a1 = (j + 1 < m) ? W.u1[c] : 0.0, a2 = (j + 2 < m) ? W.u2[c] : 0.0;
+        double y = (b[c] - a1 * p - a2 * q) * W.d[c];
+        b[c] = y;
+        q = p;
+        p = y;
+    }
+}
+
+// 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
+struct Cap {
+    double m[3][3];
+};
+
+__device__ inline Cap build_cap(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double hg, int lane) {
+    Cap K;
+    double s, p, pi;
+    functionals(C, G, X, W, W.cS, lane, s, p, pi);
+    K.m[0][0] = 1.0 - hg * s; K.m[1][0] = -hg * p; K.m[2][0] = -hg * pi;
+    functionals(C, G, X, W, W.cP, lane, s, p, pi);
+    K.m[0][1] = -hg * s; K.m[1][1] = 1.0 - hg * p; K.m[2][1] = -hg * pi;
+    functionals(C, G, X, W, W.cPi, lane, s, p, pi);
+    K.m[0][2] = -hg * s; K.m[1][2] = -hg * p; K.m[2][2] = 1.0 - hg * pi;
+    return K;
+}
+
+__device__ inline void solve3(const Cap& K, const double* r, double* y) {
+    // Gaussian elimination with partial pivoting on a 3x3 copy
+    double a[3][4];
+    for (int i = 0; i < 3; ++i) { for (int j = 0; j < 3; ++j) a[i][j] = K.m[i][j]; a[i][3] = r[i]; }
+    for (int col = 0; col < 3; ++col) {
+        int p = col;
+        for (int i = col + 1; i < 3; ++i) if (fabs(a[i][col]) > fabs(a[p][col])) p = i;
+        if (p != col) for (int j = 0; j < 4; ++j) { double t = a[col][j]; a[col][j] = a[p][j]; a[p][j] = t; }
+        for (int i = col + 1; i < 3; ++i) {
+            double f = a[i][col] / a[col][col];
+            for (int j = col; j < 4; ++j) a[i][j] -= f * a[col][j];
+        }
+    }
+    for (int i = 2; i >= 0; --i) {
+        double s = a[i][3];
+        for (int j = i + 1; j < 3; ++j) s -= a[i][j] * y[j];
+        y[i] = s / a[i][i];
+    }
+}
+
+// solve (I - hγA) z = b given chain factors and capacitance; b in place -> z
+__device__ inline void border_finish(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const Cap& K,
+                                     double hg, double* b, int lane) {
+    double r[3], y[3];
+    functionals(C, G, X, W, b, lane, r[0], r[1], r[2]);
+    solve3(K, r, y);
+    __syncthreads();
+    for (int c = lane; c < G.n; c += WAVE)
+        b[c] += hg * (W.cS[c] * y[0] + W.cP[c] * y[1] + W.cPi[c] * y[2]);
+}
+
+// source_function (perturbations.jl:343-383) on chain-ordered u (mutated) and du
+__device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double k, const double* u,
+                            const double* du, int lane) {
+    double x = X.x;
+    double Hc = X.Hc, Hcp = spl(C, BOLT_T_HC_P, x), Hcpp = spl(C, BOLT_T_HC_PP, x);
+    double tau = spl(C, BOLT_T_TAU, x), tp = X.tp, tpp = X.tpp;
+    double g = spl(C, BOLT_T_G, x), gp = spl(C, BOLT_T_G_P, x), gpp = spl(C, BOLT_T_G_PP, x);
+    double a = X.a, H02 = C.H0 * C.H0;
+    double s2 = 0.0, s2p = 0.0;
+    if (lane < G.nq) {
+        int b = G.cM + lane * (G.nm + 1) + 2;
+        s2 = W.sw[lane] * u[b];
+        s2p = W.sw[lane] * du[b];
+    }
+    double sig = wave_sum(s2), sigp = wave_sum(s2p);
+    const double* T = u + 2;
+    const double* P = u + G.cB;
+    const double* dT = du + 2;
+    const double* dP = du + G.cB;
+    double N2 = u[G.cC + 2], dN2 = du[G.cC + 2];
+    double Phip = du[G.cE], vb = u[1], vbp = du[1];
+    double pre = 12.0 * H02 / (k * k) / (a * a);
+    double Psi = -u[G.cE] - pre * (C.Or * T[2] + C.Onu * N2 + sig / C.rho_crit / 4.0);
+    double Psip = -Phip - pre * (C.Or * (dT[2] - 2.0 * T[2]) + C.Onu * (dN2 - 2.0 * N2) +
+                                 (sigp - 2.0 * sig) / C.rho_crit / 4.0);
+    double Pi = T[2] + P[2] + P[0];
+    double Pip = dT[2] + dP[2] + dP[0];
+    double term1 = g * (T[0] + Psi + Pi / 4.0) + exp(-tau) * (Psip - Phip);
+    double term2 = (-1.0 / k) * (Hcp * g * vb + Hc * gp * vb + Hc * g * vbp);
+    double Pipp = 2.0 * k / (5.0 * Hc) * (-Hcp / Hc * T[1] + dT[1]) + (3.0 / 10.0) * (tpp * Pi + tp * Pip) -
+                  3.0 * k / (5.0 * Hc) * (-Hcp / Hc * (T[3] + P[1] + P[3]) + (dT[3] + dP[1] + dP[3]));
+    double term3 = (3.0 / (4.0 * k * k)) * ((Hcp * Hcp + Hc * Hcpp) * g * Pi + 3.0 * Hc * Hcp * (gp * Pi + g * Pip) +
+                                            Hc * Hc * (gpp * Pi + 2.0 * gp * Pip + g * Pipp));
+    return term1 + term2 + term3;
+}
+
+// initial_conditions (perturbations.jl:274-338), chain order; lane 0 does the
+// photon/ν recursions, lanes 1..nq the ℳ chains
+__device__ void initial_conditions(const CosmoDev& C, const Geo& G, const WS& W, double k, double* u, int lane) {
+    double xi = C.x0;
+    double Hc = spl(C, BOLT_T_HC, xi), eta = spl(C, BOLT_T_ETA, xi), tp = spl(C, BOLT_T_TAU_P, xi);
+    double ai = sqrt(exp(xi) * exp(xi));
+    double fnu = 1.0 / (1.0 + 1.0 / (7.0 * (3.0 / 3.0) * C.Nnu / 8.0 * pow(4.0 / 11.0, 4.0 / 3.0)));
+    double Phi = (4.0 * fnu + 10.0) / (4.0 * fnu + 15.0) * 1.0;
+    double Cc = -((15.0 + 4.0 * fnu) / (20.0 + 8.0 * fnu)) * Phi;
+    double T0 = -40.0 * Cc / (15.0 + 4.0 * fnu) / 4.0;
+    double T1 = 10.0 * Cc / (15.0 + 4.0 * fnu) * (k * k * eta) / (3.0 * k);
+    double N2 = -(k * k * eta * eta) / 15.0 * 1.0 / (1.0 + 2.0 / 5.0 * fnu) * Phi / 2.0;
+    if (lane == 0) {
+        double* T = u + 2;
+        double* P = u + G.cB;
+        double* N = u + G.cC;
+        T[0] = T0;
+        T[1] = T1;
+        T[2] = -8.0 * k / (15.0 * Hc * tp) * T[1];
+        P[0] = (5.0 / 4.0) * T[2];
+        P[1] = -k / (4.0 * Hc * tp) * T[2];
+        P[2] = (1.0 / 4.0) * T[2];
+        for (int l = 3; l <= G.ng; ++l) {
+            T[l] = -(double)l / (2 * l + 1) * k / (Hc * tp) * T[l - 1];
+            P[l] = -(double)l / (2 * l + 1) * k / (Hc * tp) * P[l - 1];
+        }
+        double del = 3.0 / 4.0 * (4.0 * T[0]);
+        double v = -3.0 * k * T[1];
+        N[0] = T[0];
+        N[1] = T[1];
+        N[2] = N2;
+        for (int l = 3; l <= G.nn; ++l) N[l] = k / ((2 * l + 1) * Hc) * N[l - 1];
+        u[0] = del;   // δ_b
+        u[1] = v;     // v_b
+        u[G.cD] = del;
+        u[G.cD + 1] = v;
+        u[G.cE] = Phi;
+    } else if (lane <= G.nq) {
+        int i = lane - 1;
+        double q = W.q[i];
+        double eps = sqrt(q * q + (ai * C.mnu) * (ai * C.mnu));
+        double df0 = W.df0[i];
+        double* M = u + G.cM + i * (G.nm + 1);
+        M[0] = -T0 * df0;
+        M[1] = -eps / q * T1 * df0;
+        M[2] = -N2 * df0;
+        for (int l = 3; l <= G.nm; ++l) M[l] = q / eps * k / ((2 * l + 1) * Hc) * M[l - 1];
+    }
+}
+
+// ------------------------------------------------------------- KenCarp4 data
+__constant__ double KC_C[6] = {0.0, 0.5, 83.0 / 250.0, 31.0 / 50.0, 17.0 / 20.0, 1.0};
+__constant__ double KC_A[6][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.25, 0, 0, 0, 0, 0},
+    {8611.0 / 62500.0, -1743.0 / 31250.0, 0, 0, 0, 0},
+    {5012029.0 / 34652500.0, -654441.0 / 2922500.0, 174375.0 / 388108.0, 0, 0, 0},
+    {15267082809.0 / 155376265600.0, -71443401.0 / 120774400.0, 730878875.0 / 902184768.0,
+     2285395.0 / 8070912.0, 0, 0},
+    {82889.0 / 524892.0, 0.0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0}};
+__constant__ double KC_BH[6] = {4586570599.0 / 29645900160.0, 0.0, 178811875.0 / 945068544.0,
+                                814220225.0 / 1159782912.0, -3700637.0 / 11593932.0, 61727.0 / 225920.0};
+#define KC_G 0.25
+
+// ---------------------------------------------------------------- kernels
+struct SolveArgs {
+    const CosmoDev* cosmos;
+    const CfgDev* cfgs;
+    const ModeDesc* modes;
+    int nmodes;
+    int* work_counter;
+    int out_mode;          // 0: save states at x_save, 1: source at every x node
+    const double* x_save;  // [nsave] sorted
+    int nsave;
+    double* out;           // mode 0: [out][nsave][n_state]; mode 1: [out][nx]
+    int* status;           // [out]
+    long long* counters;   // [out][4]: accepted, rejected, rhs evals, solves
+    int n_state_stride;    // n of the output layout (mode 0)
+};
+
+// evaluate f at X on v (chain order) into out; returns Φ′
+__device__ inline double eval_rhs(const CosmoDev& C, const Geo& G, const XC& X, WS& W, const double* v,
+                                  double* out, int lane) {
+    double S, P, Pi;
+    functionals(C, G, X, W, v, lane, S, P, Pi);
+    apply_rhs(G, W, v, S, P, Pi, 1.0, out, lane);
+    return P;
+}
+
+__device__ inline double wrms(const Geo& G, const double* e, const double* ua, const double* ub, double rt,
+                              double at, int lane) {
+    double s = 0.0;
+    for (int c = lane; c < G.n; c += WAVE) {
+        double sc = at + rt * fmax(fabs(ua[c]), fabs(ub[c]));
+        double r = e[c] / sc;
+        s += r * r;
+    }
+    s = wave_sum(s);
+    return sqrt(s / G.n);
+}
+
+// output at the node/save point xv from the interval (xa, ua, fa) .. (xb, ub, fb)
+__device__ void emit(const CosmoDev& C, const Geo& G, WS& W, const SolveArgs& A, int out, double k, double xa,
+                     const double* ua, const double* fa, double xb, const double* ub, const double* fb, double xv,
+                     int idx, int lane) {
+    double h = xb - xa;
+    if (h == 0.0) {
+        for (int c = lane; c < G.n; c += WAVE) W.arg[c] = ua[c];
+    } else {
+        double th = (xv - xa) / h;
+        for (int c = lane; c < G.n; c += WAVE) {
+            double v;
+            if (th == 1.0) v = ub[c];
+            else {
+                double dd = ub[c] - ua[c];
+                v = (1.0 - th) * ua[c] + th * ub[c] +
+                    th * (th - 1.0) * ((1.0 - 2.0 * th) * dd + (th - 1.0) * h * fa[c] + th * h * fb[c]);
+            }
+            W.arg[c] = v;
+        }
+    }
+    if (A.out_mode == 0) {
+        double* dst = A.out + ((size_t)out * A.nsave + idx) * A.n_state_stride;
+        for (int c = lane; c < G.n; c += WAVE) dst[chain_to_state(G, c)] = W.arg[c];
+        return;
+    }
+    // source at a grid node: du = hierarchy!(u) (mutating u), then source_function
+    XC X = make_xc(C, k, xv);
+    qfill(C, G, X, W, lane);
+    __syncthreads();
+    assemble(C, G, X, W, lane);
+    __syncthreads();
+    double P = eval_rhs(C, G, X, W, W.arg, W.w, lane);
+    __syncthreads();
+    if (X.rsa && lane == 0) rsa_mutate(G, X, k, W.arg, P);
+    __syncthreads();
+    double s = source_fn(C, G, X, W, k, W.arg, W.w, lane);
+    if (lane == 0) A.out[(size_t)out * C.nx + idx] = s;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    __shared__ int s_mode;
+    for (;;) {
+        if (lane == 0) s_mode = atomicAdd(A.work_counter, 1);
+        __syncthreads();
+        int mi = s_mode;
+        __syncthreads();
+        if (mi >= A.nmodes) return;
+        ModeDesc md = A.modes[mi];
+        const CosmoDev C = A.cosmos[md.cosmo];
+        const CfgDev cf = A.cfgs[md.cfg];
+        Geo G = make_geo(cf);
+        WS W;
+        ws_bind(W, lds, G.n, G.nq);
+        const double k = md.k;
+        if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
+        for (int c = lane; c < G.n; c += WAVE) W.u[c] = 0.0;
+        __syncthreads();
+        initial_conditions(C, G, W, k, W.u, lane);
+        __syncthreads();
+        long long nacc = 0, nrej = 0, nrhs = 0, nsol = 0;
+        double x = C.x0, xend = C.x1;
+        const double rt = cf.reltol, at = cf.abstol;
+        int status = BOLT_MODE_OK;
+        int next = 0;  // next output index
+        const int nout = (A.out_mode == 0) ? A.nsave : C.nx;
+        // outputs at the starting point take the initial condition as saved
+        {
+            while (next < nout) {
+                double xv = (A.out_mode == 0) ? A.x_save[next]
+                                              : (C.x0 * (C.nx - 1 - next) + C.x1 * next) / (C.nx - 1);
+                if (!(xv <= x)) break;
+                emit(C, G, W, A, md.out, k, x, W.u, W.u, x, W.u, W.u, xv, next, lane);
+                ++next;
+            }
+        }
+        // FSAL start: f(u0) (the RSA overwrite, if on, persists in u)
+        XC X = make_xc(C, k, x);
+        qfill(C, G, X, W, lane);
+        __syncthreads();
+        assemble(C, G, X, W, lane);
+        __syncthreads();
+        double P0 = eval_rhs(C, G, X, W, W.u, W.f, lane);
+        ++nrhs;
+        __syncthreads();
+        if (X.rsa && lane == 0) rsa_mutate(G, X, k, W.u, P0);
+        __syncthreads();
+        // Hairer's starting step (order 4)
+        double h;
+        {
+            double d0 = wrms(G, W.u, W.u, W.u, rt, at, lane);
+            double d1 = wrms(G, W.f, W.u, W.u, rt, at, lane);
+            double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+            if (h0 > xend - x) h0 = xend - x;
+            for (int c = lane; c < G.n; c += WAVE) W.arg[c] = W.u[c] + h0 * W.f[c];
+            XC X1 = make_xc(C, k, x + h0);
+            qfill(C, G, X1, W, lane);
+            __syncthreads();
+            assemble(C, G, X1, W, lane);
+            __syncthreads();
+            eval_rhs(C, G, X1, W, W.arg, W.w, lane);
+            ++nrhs;
+            __syncthreads();
+            for (int c = lane; c < G.n; c += WAVE) W.w[c] -= W.f[c];
+            double d2 = wrms(G, W.w, W.u, W.u, rt, at, lane) / h0;
+            double dm = fmax(d1, d2);
+            double h1 = (dm <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / dm, 1.0 / 5.0);
+            h = fmin(100.0 * h0, h1);
+            if (h > xend - x) h = xend - x;
+        }
+        const int maxsteps = cf.max_steps > 0 ? cf.max_steps : 100000;
+        long long nsteps = 0;
+        while (x < xend) {
+            if (++nsteps > maxsteps) { status = BOLT_MODE_MAXSTEPS; break; }
+            bool last = false;
+            if (h >= xend - x) { h = xend - x; last = true; }
+            const double hg = h * KC_G;
+            for (int c = lane; c < G.n; c += WAVE) W.Z[0][c] = h * W.f[c];
+            Cap K;
+            XC Xs;
+            for (int s = 1; s < 6; ++s) {
+                for (int c = lane; c < G.n; c += WAVE) {
+                    double acc = W.u[c];
+                    for (int j = 0; j < s; ++j) acc += KC_A[s][j] * W.Z[j][c];
+                    W.arg[c] = acc;
+                }
+                Xs = make_xc(C, k, x + KC_C[s] * h);
+                qfill(C, G, Xs, W, lane);
+                __syncthreads();
+                assemble(C, G, Xs, W, lane);
+                __syncthreads();
+                {
+                    double S, P, Pi;
+                    functionals(C, G, Xs, W, W.arg, lane, S, P, Pi);
+                    apply_rhs(G, W, W.arg, S, P, Pi, h, W.w, lane);
+                }
+                ++nrhs;
+                to_w(G, W, hg, lane);
+                __syncthreads();
+                chain_factor_solve4(G, W, lane);
+                ++nsol;
+                __syncthreads();
+                K = build_cap(C, G, Xs, W, hg, lane);
+                border_finish(C, G, Xs, W, K, hg, W.w, lane);
+                for (int c = lane; c < G.n; c += WAVE) W.Z[s][c] = W.w[c];
+                __syncthreads();
+            }
+            // u_{n+1} (stiffly accurate: last stage) and the embedded error
+            for (int c = lane; c < G.n; c += WAVE) {
+                double acc = W.u[c], e = 0.0;
+                for (int j = 0; j < 5; ++j) {
+                    acc += KC_A[5][j] * W.Z[j][c];
+                    e += (KC_A[5][j] - KC_BH[j]) * W.Z[j][c];
+                }
+                acc += KC_G * W.Z[5][c];
+                e += (KC_G - KC_BH[5]) * W.Z[5][c];
+                W.un[c] = acc;
+                W.w[c] = e;
+            }
+            __syncthreads();
+            // Shampine's smoothed estimate: W⁻¹ e with the last stage's W
+            chain_solve1(G, W, W.w, lane);
+            ++nsol;
+            __syncthreads();
+            border_finish(C, G, Xs, W, K, hg, W.w, lane);
+            double en = wrms(G, W.w, W.u, W.un, rt, at, lane);
+            __syncthreads();
+            if (!isfinite(en)) {
+                ++nrej;
+                h *= 0.2;
+                if (h < 1e-14) { status = BOLT_MODE_NONFINITE; break; }
+                continue;
+            }
+            if (en <= 1.0) {
+                double xn = last ? xend : x + h;
+                // FSAL at x_{n+1}
+                XC Xn = make_xc(C, k, xn);
+                qfill(C, G, Xn, W, lane);
+                __syncthreads();
+                assemble(C, G, Xn, W, lane);
+                __syncthreads();
+                double Pn = eval_rhs(C, G, Xn, W, W.un, W.fn, lane);
+                ++nrhs;
+                __syncthreads();
+                if (Xn.rsa && lane == 0) rsa_mutate(G, Xn, k, W.un, Pn);
+                __syncthreads();
+                ++nacc;
+                while (next < nout) {
+                    double xv = (A.out_mode == 0) ? A.x_save[next]
+                                                  : (C.x0 * (C.nx - 1 - next) + C.x1 * next) / (C.nx - 1);
+                    if (!(xv <= xn)) break;
+                    emit(C, G, W, A, md.out, k, x, W.u, W.f, xn, W.un, W.fn, xv, next, lane);
+                    ++next;
+                }
+                for (int c = lane; c < G.n; c += WAVE) { W.u[c] = W.un[c]; W.f[c] = W.fn[c]; }
+                __syncthreads();
+                x = xn;
+                double fac = (en == 0.0) ? 10.0 : 0.9 * pow(en, -0.25);
+                fac = fmin(10.0, fmax(0.2, fac));
+                h *= fac;
+            } else {
+                ++nrej;
+                double fac = fmin(1.0, fmax(0.2, 0.9 * pow(en, -0.25)));
+                h *= fac;
+                if (h < 1e-14 * fmax(1.0, fabs(x))) { status = BOLT_MODE_DTMIN; break; }
+            }
+        }
+        if (lane == 0) {
+            A.status[md.out] = status;
+            if (A.counters) {
+                long long* cc = A.counters + (size_t)md.out * 4;
+                cc[0] = nacc; cc[1] = nrej; cc[2] = nrhs; cc[3] = nsol;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// hierarchy! parity hook: one wavefront per mode, u/du [n][B]
+__global__ void __launch_bounds__(WAVE) k_rhs(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes,
+                                               int B, double x, double* u, double* du, int mutate) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int mi = blockIdx.x;
+    if (mi >= B) return;
+    ModeDesc md = modes[mi];
+    const CosmoDev C = cosmos[md.cosmo];
+    Geo G = make_geo(cfgs[md.cfg]);
+    WS W;
+    ws_bind(W, lds, G.n, G.nq);
+    if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
+    for (int c = lane; c < G.n; c += WAVE) W.arg[c] = u[(size_t)chain_to_state(G, c) * B + mi];
+    XC X = make_xc(C, md.k, x);
+    qfill(C, G, X, W, lane);
+    __syncthreads();
+    assemble(C, G, X, W, lane);
+    __syncthreads();
+    double P = eval_rhs(C, G, X, W, W.arg, W.w, lane);
+    __syncthreads();
+    if (mutate && X.rsa && lane == 0) rsa_mutate(G, X, md.k, W.arg, P);
+    __syncthreads();
+    for (int c = lane; c < G.n; c += WAVE) {
+        size_t o = (size_t)chain_to_state(G, c) * B + mi;
+        du[o] = W.w[c];
+        if (mutate) u[o] = W.arg[c];
+    }
+}
+
+__global__ void __launch_bounds__(WAVE) k_ics(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes,
+                                               int B, double* u0) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int mi = blockIdx.x;
+    if (mi >= B) return;
+    ModeDesc md = modes[mi];
+    const CosmoDev C = cosmos[md.cosmo];
+    Geo G = make_geo(cfgs[md.cfg]);
+    WS W;
+    ws_bind(W, lds, G.n, G.nq);
+    if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
+    for (int c = lane; c < G.n; c += WAVE) W.u[c] = 0.0;
+    __syncthreads();
+    initial_conditions(C, G, W, md.k, W.u, lane);
+    __syncthreads();
+    for (int c = lane; c < G.n; c += WAVE) u0[(size_t)chain_to_state(G, c) * B + mi] = W.u[c];
+}
+
+// rsa_perts! after the switch node (perturbations.jl:35-111); one thread per (mode, node)
+__global__ void k_rsa_post(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes, int B, double* out) {
+    int mi = blockIdx.y;
+    if (mi >= B) return;
+    ModeDesc md = modes[mi];
+    const CosmoDev C = cosmos[md.cosmo];
+    const CfgDev cf = cfgs[md.cfg];
+    int nx = C.nx, n = cf.n;
+    double k = md.k;
+    // switch node (identical scan in every thread of the mode)
+    int ihor = -1, iod = -1;
+    for (int j = 0; j < nx && (ihor < 0 || iod < 0); ++j) {
+        double xj = (C.x0 * (nx - 1 - j) + C.x1 * j) / (nx - 1);
+        double eta = spl(C, BOLT_T_ETA, xj), Hc = spl(C, BOLT_T_HC, xj), tp = spl(C, BOLT_T_TAU_P, xj);
+        if (ihor < 0 && k * eta > 240.0) ihor = j;
+        if (iod < 0 && -tp * Hc / eta > 100.0) iod = j;
+    }
+    if (ihor < 0) ihor = nx - 1;
+    if (iod < 0) iod = nx - 1;
+    int isw = ihor > iod ? ihor : iod;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nx) return;
+    double xsw = (C.x0 * (nx - 1 - isw) + C.x1 * isw) / (nx - 1);
+    double xj = (C.x0 * (nx - 1 - j) + C.x1 * j) / (nx - 1);
+    if (!(xj > xsw)) return;
+    double* u = out + ((size_t)md.out * nx + j) * n;
+    int ng = cf.ng, nn = cf.nn, nm = cf.nm, nq = cf.nq;
+    int iN = 2 * (ng + 1), iM = iN + nn + 1, iPhi = iM + (nm + 1) * nq;
+    double a = exp(xj), Hc = spl(C, BOLT_T_HC, xj), tp = spl(C, BOLT_T_TAU_P, xj), tpp = spl(C, BOLT_T_TAU_PP, xj);
+    double csb2 = spl(C, BOLT_T_CSB2, xj);
+    double sr = 0, ss = 0;
+    for (int i = 0; i < nq; ++i) {
+        double q = C.qv[i], eps = sqrt(q * q + (a * C.mnu) * (a * C.mnu));
+        sr += C.base[i] * eps * u[iM + i];
+        ss += C.base[i] * (q * q / eps) * u[iM + 2 * nq + i];
+    }
+    double H02 = C.H0 * C.H0, Phi = u[iPhi];
+    double Psi = -Phi - 12.0 * H02 / (k * k) / (a * a) * (C.Or * u[2] + C.Onu * u[iN + 2] + ss / C.rho_crit / 4.0);
+    double Phip = Psi - k * k / (3.0 * Hc * Hc) * Phi +
+                  H02 / (2.0 * Hc * Hc) * (C.Oc / a * u[iPhi + 1] + C.Ob / a * u[iPhi + 3] + 4.0 * C.Or / (a * a) * u[0] +
+                                           4.0 * C.Onu / (a * a) * u[iN] + sr / C.rho_crit / (a * a));
+    double db = u[iPhi + 3], vb = u[iPhi + 4];
+    double T0 = Phi - Hc / k * tp * vb;
+    double T1 = Hc / k * (-2.0 * Phip + tp * (Phi - csb2 * db) + Hc / k * (tpp - tp) * vb);
+    u[0] = T0; u[1] = T1; u[2] = 0.0;
+    u[iN] = Phi; u[iN + 1] = -2.0 * Hc / k * Phip; u[iN + 2] = 0.0;
+    for (int l = 3; l <= ng; ++l) { u[l - 1] = 0.0; u[(ng + 1) + l - 1] = 0.0; }
+    for (int l = 3; l <= nn; ++l) u[iN + l - 1] = 0.0;
+}
+
+// plin's gauge transform and normalisation (spectra.jl:169-197), one thread per mode
+__global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes, int B, double x,
+                            const double* states, double* pk) {
+    int mi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mi >= B) return;
+    ModeDesc md = modes[mi];
+    const CosmoDev C = cosmos[md.cosmo];
+    const CfgDev cf = cfgs[md.cfg];
+    const double* u = states + (size_t)md.out * cf.n;
+    double k = md.k, a = exp(x);
+    int ng = cf.ng, nn = cf.nn, nm = cf.nm, nq = cf.nq;
+    int iM = 2 * (ng + 1) + nn + 1, iPhi = iM + (nm + 1) * nq;
+    double rho0 = spl(C, BOLT_T_RHO0M, x);
+    double sr = 0, th = 0;
+    for (int i = 0; i < nq; ++i) {
+        double q = C.qv[i], eps = sqrt(q * q + (a * C.mnu) * (a * C.mnu));
+        sr += C.base[i] * eps * u[iM + i];
+        th += C.thbase[i] * u[iM + nq + i];
+    }
+    double Mrho = sr / rho0;
+    double Mtheta = k * th / rho0;
+    double dcN = u[iPhi + 1], vcN = u[iPhi + 2], dbN = u[iPhi + 3], vbN = u[iPhi + 4];
+    double vmN = -Mtheta / k;
+    double Om = C.Oc + C.Ob + C.Om_nu_nr;
+    double Hc = spl(C, BOLT_T_HC, x);
+    double dc = dcN - 3.0 * Hc * vcN / k;
+    double db = dbN - 3.0 * Hc * vbN / k;
+    double dmn = Mrho - 3.0 * Hc * vmN / k;
+    double dm = (C.Oc * dc + C.Ob * db + C.Om_nu_nr * dmn) / Om;
+    double Pprim = C.Pprim_A * pow(k / 0.05, C.Pprim_n - 1.0);
+    pk[md.out] = (2.0 * M_PI * M_PI / (k * k * k)) * dm * dm * Pprim;
+}
+
+// ---------------------------------------------------------------- LOS (cltt)
+// T_ℓ(k) = Σ_x j_ℓ(k(η₀-η(x))) S(x,k) Δx  (spectra.jl:70-82), all ℓ <= lmax at
+// once by the upward recurrence j_{ℓ+1} = (2ℓ+1)/z j_ℓ - j_{ℓ-1}.  Past the
+// turning point ((2ℓ+1)/z > 2) the slot ends once |j_ℓ| < 1e-7 max|j| or once
+// |j_ℓ| grows (there the true j_ℓ decays monotonically, so growth means the
+// recurrence's growing solution has taken over).
+// One block of four wavefronts per dense-k midpoint; lanes own x nodes (slots
+// of 64 consecutive nodes); ℓ runs in chunks of 32 reduced across lanes by a
+// butterfly reduce-scatter and across the four wavefronts through LDS.
+#define LOS_SLOTS 16   // x slots of 64 nodes (up to 1024 nodes after x = -8)
+#define LOS_WAVES 4
+#define LOS_SPW (LOS_SLOTS / LOS_WAVES)
+#define LOS_CHUNK 32
+#define LOS_TAU 1e-7
+
+struct LosArgs {
+    const CosmoDev* cosmo;
+    const double* S;      // [nk_src][nx]
+    const double* k_src;  // [nk_src]
+    int nk_src;
+    const double* kgrid;  // [nkd]
+    int nkd;
+    const int* lidx;      // [lmax+1] output slot of ℓ or -1
+    int lmax;
+    int nell;
+    const double* etaj;   // [nxi] η at the integration nodes
+    const double* dxj;    // [nxi] x_{j+1}-x_j
+    int i0, nxi;
+    double* contrib;      // [nell][nkd-1]
+    long long* steps;     // executed recurrence steps (per k), may be NULL
+};
+
+__global__ void k_los_prep(const CosmoDev* cosmo, int i0, int nxi, double* etaj, double* dxj) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nxi) return;
+    const CosmoDev C = *cosmo;
+    double xa = (C.x0 * (C.nx - 1 - (i0 + j)) + C.x1 * (i0 + j)) / (C.nx - 1);
+    double xb = (C.x0 * (C.nx - 1 - (i0 + j + 1)) + C.x1 * (i0 + j + 1)) / (C.nx - 1);
+    etaj[j] = spl(C, BOLT_T_ETA, xa);
+    dxj[j] = xb - xa;
+}
+
+// advance one x slot through ℓ = L0+I0 .. L0+LOS_CHUNK-1, accumulating j_ℓ s
+template <int I0>
+__device__ inline void los_slot(double& jp, double& jc, double iz, double s, double& M, bool& al, int L0,
+                                double* acc, long long& nstep) {
+    double a_jp = jp, a_jc = jc, MM = M;
+    bool live = al;
+    const double base = (double)(2 * L0 - 1);
+#pragma unroll
+    for (int i = I0; i < LOS_CHUNK; ++i) {
+        double t = (base + 2.0 * i) * iz;  // (2ℓ-1)/z produces j_ℓ, ℓ = L0+i
+        double jn = t * a_jc - a_jp;
+        a_jp = a_jc;
+        a_jc = jn;
+        // past the turning point j_ℓ decays monotonically: a term below
+        // LOS_TAU·max or a growing one (the recurrence's error has taken over)
+        // ends the slot
+        if (t > 2.0 && (fabs(jn) < LOS_TAU * MM || fabs(jn) > fabs(a_jp))) live = false;
+        MM = fmax(MM, fabs(jn));
+        double v = live ? jn : 0.0;
+        nstep += live ? 1 : 0;
+        acc[i] += v * s;
+    }
+    jp = a_jp; jc = a_jc; M = MM; al = live;
+}
+
+__global__ void __launch_bounds__(256) k_los(LosArgs A) {
+    // one block (4 wavefronts) per dense-k midpoint; wave w owns x slots w, w+4, w+8, w+12
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const int ik = blockIdx.x;
+    if (ik >= A.nkd - 1) return;
+    __shared__ double part[LOS_WAVES][LOS_CHUNK];
+    const double eta0 = A.cosmo->eta0;
+    const int nx = A.cosmo->nx;
+    double k = (A.kgrid[ik] + A.kgrid[ik + 1]) / 2.0;
+    double dk = A.kgrid[ik + 1] - A.kgrid[ik];
+    // Gridded-linear interpolation in k with Line extrapolation
+    int lo = 0, hi = A.nk_src - 2;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (A.k_src[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    int ia = lo;
+    double tt = (k - A.k_src[ia]) / (A.k_src[ia + 1] - A.k_src[ia]);
+    const double* Sa = A.S + (size_t)ia * nx + A.i0;
+    const double* Sb = A.S + (size_t)(ia + 1) * nx + A.i0;
+
+    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], mx[LOS_SPW];
+    bool alive[LOS_SPW];
+#pragma unroll
+    for (int sl = 0; sl < LOS_SPW; ++sl) {
+        int j = (wv + LOS_WAVES * sl) * WAVE + lane;
+        if (j < A.nxi) {
+            double z = k * (eta0 - A.etaj[j]);
+            double s = ((1.0 - tt) * Sa[j] + tt * Sb[j]) * A.dxj[j];
+            double sn, cs;
+            sincos(z, &sn, &cs);
+            double j0 = sn / z, j1;
+            if (z < 0.1) {
+                double z2 = z * z;
+                j1 = z / 3.0 * (1.0 - z2 / 10.0 * (1.0 - z2 / 28.0 * (1.0 - z2 / 54.0 * (1.0 - z2 / 88.0))));
+            } else {
+                j1 = (j0 - cs) / z;
+            }
+            jp[sl] = j0; jc[sl] = j1;
+            iz[sl] = 1.0 / z;
+            sv[sl] = s;
+            mx[sl] = fmax(fabs(j0), fabs(j1));
+            alive[sl] = true;
+        } else {
+            jp[sl] = jc[sl] = iz[sl] = sv[sl] = mx[sl] = 0.0;
+            alive[sl] = false;
+        }
+    }
+    long long nstep = 0;
+    const double Pw = A.cosmo->Pprim_A * pow(k / 0.05, A.cosmo->Pprim_n - 1.0) * dk / k;
+    for (int L0 = 0; L0 <= A.lmax; L0 += LOS_CHUNK) {
+        double acc[LOS_CHUNK];
+#pragma unroll
+        for (int i = 0; i < LOS_CHUNK; ++i) acc[i] = 0.0;
+        bool any_live = false;
+        if (L0 == 0) {
+#pragma unroll
+            for (int sl = 0; sl < LOS_SPW; ++sl) {
+                acc[0] += jp[sl] * sv[sl];
+                acc[1] += jc[sl] * sv[sl];
+                los_slot<2>(jp[sl], jc[sl], iz[sl], sv[sl], mx[sl], alive[sl], 0, acc, nstep);
+            }
+            any_live = true;
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < LOS_SPW; ++sl) {
+                if (!__any(alive[sl])) continue;
+                any_live = true;
+                los_slot<0>(jp[sl], jc[sl], iz[sl], sv[sl], mx[sl], alive[sl], L0, acc, nstep);
+            }
+        }
+        double T = 0.0;
+        if (any_live) {  // wave-uniform
+            // butterfly reduce-scatter: lane l ends with Σ_lanes acc[l & 31]
+#pragma unroll
+            for (int w = LOS_CHUNK / 2; w >= 1; w >>= 1) {
+                bool up = (lane & w) != 0;
+#pragma unroll
+                for (int i = 0; i < w; ++i) {
+                    double send = up ? acc[i] : acc[i + w];
+                    double keep = up ? acc[i + w] : acc[i];
+                    acc[i] = keep + __shfl_xor(send, w, WAVE);
+                }
+            }
+            T = acc[0] + __shfl_xor(acc[0], 32, WAVE);
+        }
+        if (lane < LOS_CHUNK) part[wv][lane] = T;
+        int live_block = __syncthreads_or(any_live ? 1 : 0);
+        if (wv == 0 && lane < LOS_CHUNK) {
+            double Tt = part[0][lane];
+#pragma unroll
+            for (int w = 1; w < LOS_WAVES; ++w) Tt += part[w][lane];
+            int l = L0 + lane;
+            if (l <= A.lmax) {
+                int e = A.lidx[l];
+                if (e >= 0) A.contrib[(size_t)e * (A.nkd - 1) + ik] = Tt * Tt * Pw;
+            }
+        }
+        __syncthreads();
+        if (!live_block) {  // every term is past its cut: the remaining T_ℓ are zero
+            for (int l = L0 + LOS_CHUNK + threadIdx.x; l <= A.lmax; l += blockDim.x) {
+                int e = A.lidx[l];
+                if (e >= 0) A.contrib[(size_t)e * (A.nkd - 1) + ik] = 0.0;
+            }
+            break;
+        }
+    }
+    if (A.steps) {
+        double ns = wave_sum((double)nstep);
+        if (lane == 0) part[wv][0] = ns;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int w = 0; w < LOS_WAVES; ++w) t += part[w][0];
+            A.steps[ik] = (long long)t;
+        }
+    }
+}
+
+// C_ℓ = 4π Σ_k contrib (fixed order: per-thread strided sums, then a tree)
+__global__ void __launch_bounds__(256) k_los_reduce(const double* contrib, int nkm, int nell, double* cl) {
+    __shared__ double red[256];
+    int e = blockIdx.x;
+    if (e >= nell) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nkm; i += blockDim.x) s += contrib[(size_t)e * nkm + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cl[e] = 4.0 * M_PI * red[0];
+}
+
+// ------------------------------------------------------------------ host side
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return set_error(BOLT_E_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_));    \
+    } while (0)
+
+struct CosmoSlot {
+    bool used = false;
+    CosmoDev host;        // with device pointers
+    double* d_coef = nullptr;
+    double* d_q = nullptr;  // qv, df0, base, thbase contiguous
+    CosmoDev* d_self = nullptr;
+};
+
+struct bolt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    int lds_max = 64 * 1024;
+    std::vector<CosmoSlot> cosmos;
+    std::mutex mu;
+};
+
+static int check_cfg(const bolt_hier_cfg* c) {
+    if (!c) return set_error(BOLT_E_ARG, "cfg is NULL");
+    if (c->ell_g < 3 || c->ell_nu < 2 || c->ell_mnu < 2 || c->nq < 1 || c->nq > 64)
+        return set_error(BOLT_E_ARG, "hierarchy sizes need ell_g >= 3, ell_nu >= 2, ell_mnu >= 2, 1 <= nq <= 64");
+    if (!(c->reltol > 0) || !(c->abstol > 0)) return set_error(BOLT_E_ARG, "reltol and abstol must be positive");
+    return BOLT_OK;
+}
+
+static CfgDev to_cfgdev(const bolt_hier_cfg* c) {
+    CfgDev d;
+    d.ng = c->ell_g; d.nn = c->ell_nu; d.nm = c->ell_mnu; d.nq = c->nq;
+    d.n = 2 * (d.ng + 1) + (d.nn + 1) + (d.nm + 1) * d.nq + 5;
+    d.reltol = c->reltol; d.abstol = c->abstol; d.max_steps = c->max_steps;
+    return d;
+}
+
+static int get_cosmo(bolt_ctx* ctx, int id, CosmoSlot** out) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    if (id < 0 || id >= (int)ctx->cosmos.size() || !ctx->cosmos[id].used)
+        return set_error(BOLT_E_STATE, "unknown cosmology id");
+    *out = &ctx->cosmos[id];
+    return BOLT_OK;
+}
+
+// device buffer RAII
+struct DBuf {
+    void* p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+    template <class T> T* as() { return (T*)p; }
+};
+
+extern "C" {
+
+int bolt_ctx_create(int device, bolt_ctx** out) {
+    if (!out) return set_error(BOLT_E_ARG, "out is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(BOLT_E_DEVICE, "no HIP device present");
+    if (device < 0 || device >= ndev) return set_error(BOLT_E_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    bolt_ctx* c = new bolt_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return set_error(BOLT_E_DEVICE, "hipGetDeviceProperties failed"); }
+    c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_error(BOLT_E_DEVICE, "hipStreamCreate failed");
+    }
+    {
+        // opt in to the device's full LDS for the one-wavefront-per-mode kernels
+        int lds_max = (int)prop.sharedMemPerBlockOptin > 0 ? (int)prop.sharedMemPerBlockOptin
+                                                            : (int)prop.sharedMemPerBlock;
+        c->lds_max = lds_max - 1024;  // headroom for static __shared__
+        (void)hipFuncSetAttribute((const void*)k_boltsolve, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+        (void)hipFuncSetAttribute((const void*)k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+        (void)hipFuncSetAttribute((const void*)k_ics, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+        (void)hipGetLastError();  // an unsupported opt-in must not surface at the next launch check
+    }
+    *out = c;
+    return BOLT_OK;
+}
+
+int bolt_ctx_destroy(bolt_ctx* ctx) {
+    if (!ctx) return BOLT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& s : ctx->cosmos) {
+        if (s.d_coef) (void)hipFree(s.d_coef);
+        if (s.d_q) (void)hipFree(s.d_q);
+        if (s.d_self) (void)hipFree(s.d_self);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BOLT_OK;
+}
+
+int bolt_ctx_sync(bolt_ctx* ctx) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BOLT_OK;
+}
+
+int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_tables* t, int* cosmo_id) {
+    if (!ctx || !par || !t || !cosmo_id || !t->coef || !t->xq || !t->wq)
+        return set_error(BOLT_E_ARG, "bolt_cosmo_upload: NULL argument");
+    if (t->nx < 4 || t->nq < 1 || t->nq > 64 || !(t->x1 > t->x0))
+        return set_error(BOLT_E_ARG, "bolt_cosmo_upload: bad table geometry");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    CosmoSlot s;
+    s.used = true;
+    CosmoDev& C = s.host;
+    C.h = par->h; C.Or = par->Omega_r; C.Ob = par->Omega_b; C.Oc = par->Omega_c; C.A = par->A; C.ns = par->n;
+    C.Yp = par->Y_p; C.Nnu = par->N_nu; C.mnu = par->Sum_m_nu;
+    C.H0 = t->scal[BOLT_S_H0]; C.eta0 = t->scal[BOLT_S_ETA0]; C.rho_crit = t->scal[BOLT_S_RHO_CRIT];
+    C.Onu = 7.0 * (2.0 / 3.0) * par->N_nu / 8.0 * std::pow(4.0 / 11.0, 4.0 / 3.0) * par->Omega_r;
+    C.x0 = t->x0; C.x1 = t->x1; C.dx = (t->x1 - t->x0) / (t->nx - 1); C.nx = t->nx; C.nq = t->nq;
+    C.Pprim_A = par->A; C.Pprim_n = par->n;
+    {   // plin's Ω_ν (spectra.jl:182-186)
+        double Tg = std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par->Omega_r, 0.25);
+        double nufac = (90.0 * 1.2020569 / (11.0 * std::pow(M_PI, 4))) * (par->Omega_r * par->h * par->h / Tg) *
+                       std::pow(par->N_nu / 3.0, 0.75);
+        C.Om_nu_nr = par->Sum_m_nu * nufac / (par->h * par->h);
+    }
+    // q nodes and weights (perturbations.jl:127-158 with util.jl:24-27)
+    int nq = t->nq;
+    std::vector<double> qbuf(4 * nq);
+    double Tnu = std::pow(par->N_nu / 3.0, 0.25) * std::pow(4.0 / 11.0, 1.0 / 3.0) *
+                 std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par->Omega_r, 0.25);
+    double lo = std::log10(Tnu / 30.0), hi = std::log10(Tnu * 30.0);
+    for (int i = 0; i < nq; ++i) {
+        double q = std::pow(10.0, lo + (hi - lo) / 2.0 * (t->xq[i] + 1.0));
+        double tu = -1.0 + 2.0 / (hi - lo) * (1.0 + lo - lo);
+        double dxdq = (1.0 + tu) / (q * std::log(10.0));
+        double f0 = 2.0 / std::pow(2.0 * M_PI, 3) / (std::exp(q / Tnu) + 1.0);
+        qbuf[i] = q;
+        qbuf[nq + i] = -q / Tnu / (1.0 + std::exp(-q / Tnu));
+        qbuf[2 * nq + i] = 4.0 * M_PI * q * q * f0 / dxdq * t->wq[i];
+        qbuf[3 * nq + i] = 4.0 * M_PI * q * q * q * f0 / dxdq * t->wq[i];
+    }
+    size_t ncoef = (size_t)BOLT_NTAB * (t->nx + 2);
+    if (hipMalloc(&s.d_coef, ncoef * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s.d_q, qbuf.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s.d_self, sizeof(CosmoDev)) != hipSuccess) {
+        if (s.d_coef) (void)hipFree(s.d_coef);
+        if (s.d_q) (void)hipFree(s.d_q);
+        return set_error(BOLT_E_NOMEM, "bolt_cosmo_upload: hipMalloc failed");
+    }
+    HIPCHK(hipMemcpy(s.d_coef, t->coef, ncoef * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.d_q, qbuf.data(), qbuf.size() * sizeof(double), hipMemcpyHostToDevice));
+    C.coef = s.d_coef; C.qv = s.d_q; C.df0 = s.d_q + nq; C.base = s.d_q + 2 * nq; C.thbase = s.d_q + 3 * nq;
+    HIPCHK(hipMemcpy(s.d_self, &C, sizeof(CosmoDev), hipMemcpyHostToDevice));
+    int id = -1;
+    for (size_t i = 0; i < ctx->cosmos.size(); ++i) if (!ctx->cosmos[i].used) { id = (int)i; break; }
+    if (id < 0) { id = (int)ctx->cosmos.size(); ctx->cosmos.push_back(s); } else ctx->cosmos[id] = s;
+    *cosmo_id = id;
+    return BOLT_OK;
+}
+
+int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id) {
+    CosmoSlot* s;
+    int r = get_cosmo(ctx, cosmo_id, &s);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(s->d_coef); (void)hipFree(s->d_q); (void)hipFree(s->d_self);
+    *s = CosmoSlot();
+    return BOLT_OK;
+}
+
+}  // extern "C"
+
+// shared launch helper for the integrator
+static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, const double* d_k_or_host, int B,
+                        bool k_on_device, int out_mode, const double* d_xsave, int nsave, double* d_out,
+                        int* d_status, long long* d_counters, int n_stride, double* kernel_ms) {
+    CfgDev cd = to_cfgdev(cfg);
+    std::vector<double> kh(B);
+    if (k_on_device) HIPCHK(hipMemcpyAsync(kh.data(), d_k_or_host, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+    else std::memcpy(kh.data(), d_k_or_host, sizeof(double) * B);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < B; ++i)
+        if (!(kh[i] > 0.0) || !std::isfinite(kh[i])) return set_error(BOLT_E_ARG, "k must be positive and finite");
+    // costliest (largest k) first
+    std::vector<int> order(B);
+    for (int i = 0; i < B; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return kh[a] > kh[b]; });
+    std::vector<ModeDesc> md(B);
+    for (int i = 0; i < B; ++i) { md[i].k = kh[order[i]]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = order[i]; md[i].pad = 0; }
+    DBuf dmodes, dcfg, dctr;
+    HIPCHK(dmodes.alloc(sizeof(ModeDesc) * B));
+    HIPCHK(dcfg.alloc(sizeof(CfgDev)));
+    HIPCHK(dctr.alloc(sizeof(int)));
+    HIPCHK(hipMemcpyAsync(dmodes.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dcfg.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(dctr.p, 0, sizeof(int), ctx->stream));
+    SolveArgs A;
+    A.cosmos = cs->d_self;
+    A.cfgs = dcfg.as<CfgDev>();
+    A.modes = dmodes.as<ModeDesc>();
+    A.nmodes = B;
+    A.work_counter = dctr.as<int>();
+    A.out_mode = out_mode;
+    A.x_save = d_xsave;
+    A.nsave = nsave;
+    A.out = d_out;
+    A.status = d_status;
+    A.counters = d_counters;
+    A.n_state_stride = n_stride;
+    size_t lds = ws_bytes(cd.n, cd.nq);
+    if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, WAVE, lds));
+    if (per_cu < 1) per_cu = 1;
+    int grid = std::min(B, per_cu * ctx->n_cu);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(WAVE), lds, ctx->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    return BOLT_OK;
+}
+
+static void fill_stats(bolt_stats* st, const std::vector<long long>& ctr, int B, double ms) {
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    for (int i = 0; i < B; ++i) {
+        st->steps_accepted += ctr[4 * i];
+        st->steps_rejected += ctr[4 * i + 1];
+        st->rhs_evals += ctr[4 * i + 2];
+        st->linear_solves += ctr[4 * i + 3];
+    }
+    st->kernel_ms = ms;
+}
+
+extern "C" {
+
+int bolt_rhs_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B, double x,
+                   double* u, double* du, int mutate) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !u || !du || B <= 0) return set_error(BOLT_E_ARG, "bolt_rhs_batch: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    if (!(x >= cs->host.x0 && x <= cs->host.x1)) return set_error(BOLT_E_ARG, "x outside the table grid");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    CfgDev cd = to_cfgdev(cfg);
+    std::vector<ModeDesc> md(B);
+    for (int i = 0; i < B; ++i) { md[i].k = k[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; }
+    DBuf dm, dc, du_, dd;
+    size_t nb = sizeof(double) * (size_t)cd.n * B;
+    HIPCHK(dm.alloc(sizeof(ModeDesc) * B));
+    HIPCHK(dc.alloc(sizeof(CfgDev)));
+    HIPCHK(du_.alloc(nb));
+    HIPCHK(dd.alloc(nb));
+    HIPCHK(hipMemcpyAsync(dm.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dc.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(du_.p, u, nb, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_rhs, dim3(B), dim3(WAVE), ws_bytes(cd.n, cd.nq), ctx->stream, cs->d_self, dc.as<CfgDev>(),
+                       dm.as<ModeDesc>(), B, x, du_.as<double>(), dd.as<double>(), mutate);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(du, dd.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (mutate) HIPCHK(hipMemcpyAsync(u, du_.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BOLT_OK;
+}
+
+int bolt_initial_conditions(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                            double* u0) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !u0 || B <= 0) return set_error(BOLT_E_ARG, "bolt_initial_conditions: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    CfgDev cd = to_cfgdev(cfg);
+    std::vector<ModeDesc> md(B);
+    for (int i = 0; i < B; ++i) { md[i].k = k[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; }
+    DBuf dm, dc, dout;
+    size_t nb = sizeof(double) * (size_t)cd.n * B;
+    HIPCHK(dm.alloc(sizeof(ModeDesc) * B));
+    HIPCHK(dc.alloc(sizeof(CfgDev)));
+    HIPCHK(dout.alloc(nb));
+    HIPCHK(hipMemcpyAsync(dm.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dc.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_ics, dim3(B), dim3(WAVE), ws_bytes(cd.n, cd.nq), ctx->stream, cs->d_self, dc.as<CfgDev>(),
+                       dm.as<ModeDesc>(), B, dout.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(u0, dout.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BOLT_OK;
+}
+
+int bolt_boltsolve_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                         const double* x_save, int nsave, double* u_out, int* status, bolt_stats* stats) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !x_save || !u_out || !status || B <= 0 || nsave <= 0)
+        return set_error(BOLT_E_ARG, "bolt_boltsolve_batch: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    for (int i = 0; i < nsave; ++i) {
+        if (!(x_save[i] >= cs->host.x0 && x_save[i] <= cs->host.x1))
+            return set_error(BOLT_E_ARG, "x_save outside [x0, x1]");
+        if (i && !(x_save[i] >= x_save[i - 1])) return set_error(BOLT_E_ARG, "x_save must be sorted ascending");
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = bolt_state_length(cfg);
+    DBuf dk, dx, dout, dst, dctr;
+    size_t nb = sizeof(double) * (size_t)B * nsave * n;
+    HIPCHK(dx.alloc(sizeof(double) * nsave));
+    HIPCHK(dout.alloc(nb));
+    HIPCHK(dst.alloc(sizeof(int) * B));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * B));
+    HIPCHK(hipMemcpyAsync(dx.p, x_save, sizeof(double) * nsave, hipMemcpyHostToDevice, ctx->stream));
+    double ms = 0;
+    if ((r = launch_solve(ctx, cs, cfg, k, B, false, 0, dx.as<double>(), nsave, dout.as<double>(), dst.as<int>(),
+                          dctr.as<long long>(), n, &ms)))
+        return r;
+    std::vector<long long> ctr(4 * (size_t)B);
+    HIPCHK(hipMemcpyAsync(u_out, dout.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    fill_stats(stats, ctr, B, ms);
+    for (int i = 0; i < B; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
+    return BOLT_OK;
+}
+
+int bolt_boltsolve_rsa_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                             double* out, int* status, bolt_stats* stats) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !out || !status || B <= 0) return set_error(BOLT_E_ARG, "bolt_boltsolve_rsa_batch: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = bolt_state_length(cfg), nx = cs->host.nx;
+    std::vector<double> xs(nx);
+    for (int j = 0; j < nx; ++j) xs[j] = (cs->host.x0 * (nx - 1 - j) + cs->host.x1 * j) / (nx - 1);
+    DBuf dx, dout, dst, dctr, dm, dc;
+    size_t nb = sizeof(double) * (size_t)B * nx * n;
+    HIPCHK(dx.alloc(sizeof(double) * nx));
+    HIPCHK(dout.alloc(nb));
+    HIPCHK(dst.alloc(sizeof(int) * B));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * B));
+    HIPCHK(hipMemcpyAsync(dx.p, xs.data(), sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream));
+    double ms = 0;
+    if ((r = launch_solve(ctx, cs, cfg, k, B, false, 0, dx.as<double>(), nx, dout.as<double>(), dst.as<int>(),
+                          dctr.as<long long>(), n, &ms)))
+        return r;
+    CfgDev cd = to_cfgdev(cfg);
+    std::vector<ModeDesc> md(B);
+    for (int i = 0; i < B; ++i) { md[i].k = k[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; }
+    HIPCHK(dm.alloc(sizeof(ModeDesc) * B));
+    HIPCHK(dc.alloc(sizeof(CfgDev)));
+    HIPCHK(hipMemcpyAsync(dm.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dc.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_rsa_post, dim3((nx + 255) / 256, B), dim3(256), 0, ctx->stream, cs->d_self, dc.as<CfgDev>(),
+                       dm.as<ModeDesc>(), B, dout.as<double>());
+    HIPCHK(hipGetLastError());
+    std::vector<long long> ctr(4 * (size_t)B);
+    HIPCHK(hipMemcpyAsync(out, dout.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    fill_stats(stats, ctr, B, ms);
+    for (int i = 0; i < B; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
+    return BOLT_OK;
+}
+
+int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                         double* d_S, int* d_status, int64_t* d_counters, float* ms) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!d_k || !d_S || !d_status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid_dev: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    double dms = 0;
+    r = launch_solve(ctx, cs, cfg, d_k, nk, true, 1, nullptr, 0, d_S, d_status, (long long*)d_counters, 0, &dms);
+    if (ms) *ms = (float)dms;
+    return r;
+}
+
+int bolt_source_grid(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S,
+                     int* status, bolt_stats* stats) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !S || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int nx = cs->host.nx;
+    DBuf dS, dst, dctr;
+    size_t nb = sizeof(double) * (size_t)nk * nx;
+    HIPCHK(dS.alloc(nb));
+    HIPCHK(dst.alloc(sizeof(int) * nk));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * nk));
+    double ms = 0;
+    if ((r = launch_solve(ctx, cs, cfg, k, nk, false, 1, nullptr, 0, dS.as<double>(), dst.as<int>(),
+                          dctr.as<long long>(), 0, &ms)))
+        return r;
+    std::vector<long long> ctr(4 * (size_t)nk);
+    HIPCHK(hipMemcpyAsync(S, dS.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * nk, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * nk, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    fill_stats(stats, ctr, nk, ms);
+    for (int i = 0; i < nk; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
+    return BOLT_OK;
+}
+
+int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double x, double* pk,
+              int* status, bolt_stats* stats) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || !pk || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_plin: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    if (!(x >= cs->host.x0 && x <= cs->host.x1)) return set_error(BOLT_E_ARG, "x outside the table grid");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int n = bolt_state_length(cfg);
+    DBuf dx, dout, dst, dctr, dm, dc, dpk;
+    HIPCHK(dx.alloc(sizeof(double)));
+    HIPCHK(dout.alloc(sizeof(double) * (size_t)nk * n));
+    HIPCHK(dst.alloc(sizeof(int) * nk));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * nk));
+    HIPCHK(dpk.alloc(sizeof(double) * nk));
+    HIPCHK(hipMemcpyAsync(dx.p, &x, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double ms = 0;
+    if ((r = launch_solve(ctx, cs, cfg, k, nk, false, 0, dx.as<double>(), 1, dout.as<double>(), dst.as<int>(),
+                          dctr.as<long long>(), n, &ms)))
+        return r;
+    CfgDev cd = to_cfgdev(cfg);
+    std::vector<ModeDesc> md(nk);
+    for (int i = 0; i < nk; ++i) { md[i].k = k[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; }
+    HIPCHK(dm.alloc(sizeof(ModeDesc) * nk));
+    HIPCHK(dc.alloc(sizeof(CfgDev)));
+    HIPCHK(hipMemcpyAsync(dm.p, md.data(), sizeof(ModeDesc) * nk, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dc.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_plin_post, dim3((nk + 127) / 128), dim3(128), 0, ctx->stream, cs->d_self, dc.as<CfgDev>(),
+                       dm.as<ModeDesc>(), nk, x, dout.as<double>(), dpk.as<double>());
+    HIPCHK(hipGetLastError());
+    std::vector<long long> ctr(4 * (size_t)nk);
+    HIPCHK(hipMemcpyAsync(pk, dpk.p, sizeof(double) * nk, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * nk, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * nk, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    fill_stats(stats, ctr, nk, ms);
+    for (int i = 0; i < nk; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
+    return BOLT_OK;
+}
+
+}  // extern "C"
+
+// cltt core on device buffers; ells given on the host (validated there)
+static int cltt_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double* d_k_src, int nk_src,
+                     const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode, double* d_cl,
+                     long long* d_steps, float* ms_out) {
+    if (bessel_mode != 0) return set_error(BOLT_E_ARG, "bessel_mode 1 (spline table) is not implemented on device");
+    int nell = (int)ells.size();
+    int lmax = 0;
+    for (int l : ells) {
+        if (l < 0 || l > 20000) return set_error(BOLT_E_ARG, "ℓ out of range [0, 20000]");
+        lmax = std::max(lmax, l);
+    }
+    int nx = cs->host.nx;
+    int i0 = -1;
+    for (int j = 0; j < nx; ++j) {
+        double xj = (cs->host.x0 * (nx - 1 - j) + cs->host.x1 * j) / (nx - 1);
+        if (xj > -8.0) { i0 = j; break; }
+    }
+    if (i0 < 0 || i0 >= nx - 1) return set_error(BOLT_E_ARG, "x grid has no node after x = -8");
+    int nxi = nx - 1 - i0;
+    if (nxi > LOS_SLOTS * WAVE) return set_error(BOLT_E_ARG, "too many integration nodes for the LOS kernel");
+    std::vector<int> lidx(lmax + 1, -1);
+    for (int e = 0; e < nell; ++e) lidx[ells[e]] = e;
+    DBuf dl, deta, ddx, dcon;
+    HIPCHK(dl.alloc(sizeof(int) * (lmax + 1)));
+    HIPCHK(deta.alloc(sizeof(double) * nxi));
+    HIPCHK(ddx.alloc(sizeof(double) * nxi));
+    HIPCHK(dcon.alloc(sizeof(double) * (size_t)nell * (nkd - 1)));
+    HIPCHK(hipMemcpyAsync(dl.p, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, ctx->stream, cs->d_self, i0, nxi,
+                       deta.as<double>(), ddx.as<double>());
+    HIPCHK(hipGetLastError());
+    LosArgs A;
+    A.cosmo = cs->d_self; A.S = d_S; A.k_src = d_k_src; A.nk_src = nk_src; A.kgrid = d_kgrid; A.nkd = nkd;
+    A.lidx = dl.as<int>(); A.lmax = lmax; A.nell = nell; A.etaj = deta.as<double>(); A.dxj = ddx.as<double>();
+    A.i0 = i0; A.nxi = nxi; A.contrib = dcon.as<double>(); A.steps = d_steps;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int nkm = nkd - 1;
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    hipLaunchKernelGGL(k_los, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell, d_cl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_out) *ms_out = ms;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BOLT_OK;
+}
+
+extern "C" {
+
+int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src, int nk_src, const double* kgrid,
+              int nkd, const int* ells, int nell, int bessel_mode, double* cl) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if (!S || !k_src || !kgrid || !ells || !cl || nk_src < 2 || nkd < 2 || nell < 1)
+        return set_error(BOLT_E_ARG, "bolt_cltt: bad argument");
+    for (int i = 1; i < nk_src; ++i) if (!(k_src[i] > k_src[i - 1])) return set_error(BOLT_E_ARG, "k_src must increase");
+    for (int i = 1; i < nkd; ++i) if (!(kgrid[i] > kgrid[i - 1])) return set_error(BOLT_E_ARG, "kgrid must increase");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int nx = cs->host.nx;
+    DBuf dS, dks, dkg, dcl;
+    HIPCHK(dS.alloc(sizeof(double) * (size_t)nk_src * nx));
+    HIPCHK(dks.alloc(sizeof(double) * nk_src));
+    HIPCHK(dkg.alloc(sizeof(double) * nkd));
+    HIPCHK(dcl.alloc(sizeof(double) * nell));
+    HIPCHK(hipMemcpyAsync(dS.p, S, sizeof(double) * (size_t)nk_src * nx, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dks.p, k_src, sizeof(double) * nk_src, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<int> ev(ells, ells + nell);
+    if ((r = cltt_core(ctx, cs, dS.as<double>(), dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev, bessel_mode,
+                       dcl.as<double>(), nullptr, nullptr)))
+        return r;
+    HIPCHK(hipMemcpy(cl, dcl.p, sizeof(double) * nell, hipMemcpyDeviceToHost));
+    return BOLT_OK;
+}
+
+// device-resident variant used by the serving loop and the benchmark: ℓ list on
+// the host, everything else on the device; returns the k_los time in *ms.
+int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
+                       const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl, int64_t* d_steps,
+                       float* ms) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if (!d_S || !d_k_src || !d_kgrid || !ells || !d_cl || nk_src < 2 || nkd < 2 || nell < 1)
+        return set_error(BOLT_E_ARG, "bolt_cltt_dev: bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<int> ev(ells, ells + nell);
+    return cltt_core(ctx, cs, d_S, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_cl, (long long*)d_steps, ms);
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// C ABI for the host-side producers and the shared error/version plumbing.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "host_tables.h"
+
+namespace bolt {
+thread_local std::string g_last_error;
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+}  // namespace bolt
+
+using bolt::set_error;
+
+extern "C" {
+
+int bolt_abi_version(void) { return BOLT_ABI_VERSION; }
+
+const char* bolt_last_error(void) { return bolt::g_last_error.c_str(); }
+
+int bolt_state_length(const bolt_hier_cfg* c) {
+    if (!c) return set_error(BOLT_E_ARG, "cfg is NULL");
+    return 2 * (c->ell_g + 1) + (c->ell_nu + 1) + (c->ell_mnu + 1) * c->nq + 5;
+}
+
+int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq,
+                     double* coef_out, double* scal_out, double* xq_out, double* wq_out) {
+    if (!par || !coef_out || !scal_out || !xq_out || !wq_out)
+        return set_error(BOLT_E_ARG, "bolt_host_tables: NULL argument");
+    if (nx < 4 || nq < 1 || !(x1 > x0)) return set_error(BOLT_E_ARG, "bolt_host_tables: bad grid");
+    if (!(par->Omega_r > 0) || !(par->N_nu > 0) || !(par->h > 0))
+        return set_error(BOLT_E_ARG, "bolt_host_tables: Omega_r, N_nu and h must be positive");
+    try {
+        bolt::Background bg(*par, x0, x1, nx, nq);
+        bg.ionization(-1.0);
+        std::memcpy(coef_out, bg.coef.data(), sizeof(double) * bg.coef.size());
+        scal_out[BOLT_S_H0] = bg.H0;
+        scal_out[BOLT_S_ETA0] = bg.eta0;
+        scal_out[BOLT_S_RHO_CRIT] = bg.rho_crit;
+        scal_out[BOLT_S_OMEGA_L] = bg.Omega_L;
+        scal_out[BOLT_S_TAU0] = bg.tau0;
+        for (int i = 0; i < nq; ++i) {
+            xq_out[i] = bg.cosmo.xq[i];
+            wq_out[i] = bg.cosmo.wq[i];
+        }
+        for (double v : bg.coef)
+            if (!std::isfinite(v)) return set_error(BOLT_E_NUMERIC, "bolt_host_tables: non-finite table entry");
+    } catch (const std::exception& e) {
+        return set_error(BOLT_E_NUMERIC, std::string("bolt_host_tables: ") + e.what());
+    }
+    return BOLT_OK;
+}
+
+int bolt_host_recfast_xe(const bolt_cosmo_params* par, double Tnow, const double* z, int nz,
+                         double* xe_out) {
+    if (!par || !z || !xe_out || nz < 0) return set_error(BOLT_E_ARG, "bolt_host_recfast_xe: bad argument");
+    for (int i = 0; i < nz; ++i)
+        if (!(z[i] >= 0.0 && z[i] <= 10000.0))
+            return set_error(BOLT_E_ARG, "bolt_host_recfast_xe: z outside [0, 10000]");
+    return bolt::recfast_xe(*par, Tnow, z, nz, xe_out);
+}
+
+int bolt_host_spline_eval(const bolt_tables* t, int tab, int deriv, const double* xs, int m, double* out) {
+    if (!t || !t->coef || !xs || !out || tab < 0 || tab >= BOLT_NTAB || deriv < 0 || deriv > 2)
+        return set_error(BOLT_E_ARG, "bolt_host_spline_eval: bad argument");
+    double dx = (t->x1 - t->x0) / (t->nx - 1);
+    const double* c = t->coef + (size_t)tab * (t->nx + 2);
+    for (int i = 0; i < m; ++i) out[i] = bolt::spline_eval(c, t->nx, t->x0, dx, xs[i], deriv);
+    return BOLT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+/*
+ * bolt_hip.h -- C ABI of libbolt_hip.so, the MI355X (gfx950) implementation of
+ * Bolt.jl's perturbation hot path:
+ *
+ *   hierarchy! RHS + boltsolve (KenCarp4 ESDIRK)  ->  source_grid  ->  cltt / plin
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (0 = OK, negative = error class, see BOLT_E_*) and leaves a
+ * human readable message retrievable with bolt_last_error() (thread local).
+ * Functions without the _dev suffix take host arrays (what a Julia ccall with
+ * GC.@preserve hands over).  The _dev variants take device pointers and leave
+ * their results in device memory.  All entry points return after their
+ * kernels have finished; a context serialises its callers.
+ *
+ * Citations are into the Bolt.jl source tree (src/...).
+ */
+#ifndef BOLT_HIP_H
+#define BOLT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BOLT_ABI_VERSION 1
+
+/* status codes */
+#define BOLT_OK 0
+#define BOLT_E_ARG (-1)      /* invalid argument (sizes, NULL pointers, config) */
+#define BOLT_E_DEVICE (-2)   /* HIP runtime / device error, or no GPU present */
+#define BOLT_E_NOMEM (-3)    /* device allocation failed */
+#define BOLT_E_NUMERIC (-4)  /* a mode failed (see per-mode status) */
+#define BOLT_E_STATE (-5)    /* unknown cosmology id / context misuse */
+
+/* per-mode status written by the solvers */
+#define BOLT_MODE_OK 0
+#define BOLT_MODE_MAXSTEPS 1 /* step budget exhausted before x = 0 */
+#define BOLT_MODE_DTMIN 2    /* step size underflow */
+#define BOLT_MODE_NONFINITE 3/* NaN/Inf in the state or error norm */
+
+/* CosmoParams in Bolt's field order (src/Bolt.jl:56-66).  Sum_m_nu is in
+ * natural units (Mpc^-1), as Bolt stores it (0.06*mass_natural by default). */
+typedef struct bolt_cosmo_params {
+    double h, Omega_r, Omega_b, Omega_c, A, n, Y_p, N_nu, Sum_m_nu;
+} bolt_cosmo_params;
+
+/* Spline tables.  Each table is a cubic B-spline with natural ("Line(OnGrid())")
+ * end conditions on the uniform grid x_j = x0 + j*dx, j = 0..nx-1, stored as its
+ * nx+2 padded coefficients (src/util.jl:11-13).  Table order: */
+enum {
+    BOLT_T_HC = 0,   /* bg.ℋ      (src/background.jl:121) */
+    BOLT_T_HC_P,     /* bg.ℋ′     (spline_∂ₓ(ℋ_))  */
+    BOLT_T_HC_PP,    /* bg.ℋ′′    (spline_∂ₓ²(ℋ_)) */
+    BOLT_T_ETA,      /* bg.η */
+    BOLT_T_ETA_P,    /* bg.η′ */
+    BOLT_T_ETA_PP,   /* bg.η′′ */
+    BOLT_T_RHO0M,    /* bg.ρ₀ℳ (massive-ν background density) */
+    BOLT_T_XE,       /* ih.Xₑ      (src/ionization/recfast.jl:705) */
+    BOLT_T_TAU,      /* ih.τ */
+    BOLT_T_TAU_P,    /* ih.τ′ */
+    BOLT_T_TAU_PP,   /* ih.τ′′ */
+    BOLT_T_G,        /* ih.g̃ */
+    BOLT_T_G_P,      /* ih.g̃′ */
+    BOLT_T_G_PP,     /* ih.g̃′′ */
+    BOLT_T_TMAT,     /* ih.Tmat */
+    BOLT_T_CSB2,     /* ih.csb² */
+    BOLT_NTAB
+};
+
+/* scalar entries of the background / ionization history */
+enum { BOLT_S_H0 = 0, BOLT_S_ETA0, BOLT_S_RHO_CRIT, BOLT_S_OMEGA_L, BOLT_S_TAU0, BOLT_NSCAL };
+
+typedef struct bolt_tables {
+    int nx;             /* number of x nodes (2001 for x_grid = -20:0.01:0) */
+    int nq;             /* Gauss-Legendre order of the q quadrature */
+    double x0, x1;      /* first and last node; x_j = (x0*(nx-1-j) + x1*j)/(nx-1) */
+    double scal[BOLT_NSCAL];
+    const double* xq;   /* [nq] Gauss-Legendre nodes on [-1,1], ascending */
+    const double* wq;   /* [nq] weights */
+    const double* coef; /* [BOLT_NTAB][nx+2] */
+} bolt_tables;
+
+/* Hierarchy sizes and solver settings (src/perturbations.jl:7-21, 25-33).
+ * n = 2(ell_g+1) + (ell_nu+1) + (ell_mnu+1)*nq + 5. */
+typedef struct bolt_hier_cfg {
+    int ell_g, ell_nu, ell_mnu, nq;
+    double reltol, abstol;
+    int max_steps;      /* per mode; 0 -> 2,000,000 */
+    int flags;          /* reserved, 0 */
+} bolt_hier_cfg;
+
+typedef struct bolt_stats {
+    int64_t steps_accepted, steps_rejected, rhs_evals, linear_solves;
+    double kernel_ms;   /* device time of the solver launch (hipEvent) */
+} bolt_stats;
+
+/* ---------------------------------------------------------------- host --- */
+int bolt_abi_version(void);
+const char* bolt_last_error(void);
+int bolt_state_length(const bolt_hier_cfg* cfg);
+
+/* Background(par; x_grid, nq) + IonizationHistory(RECFAST(bg), par, bg) on the
+ * host, FP64 (src/background.jl:104-128, src/ionization/recfast.jl:674-726).
+ * coef_out: [BOLT_NTAB][nx+2]; scal_out: [BOLT_NSCAL]; xq_out, wq_out: [nq]. */
+int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq,
+                     double* coef_out, double* scal_out, double* xq_out, double* wq_out);
+
+/* RECFAST X_e(z) before reionization, Xe_RECFAST(recfastsolve(RECFAST(bg; Tnow)))
+ * (src/ionization/recfast.jl:413-475).  Tnow <= 0 takes the default derived from
+ * Omega_r (recfast.jl:108). */
+int bolt_host_recfast_xe(const bolt_cosmo_params* par, double Tnow, const double* z, int nz,
+                         double* xe_out);
+
+/* Evaluate table `tab` (or its first/second x-derivative, deriv = 0,1,2) at xs. */
+int bolt_host_spline_eval(const bolt_tables* t, int tab, int deriv, const double* xs, int m,
+                          double* out);
+
+/* ---------------------------------------------------------------- device - */
+typedef struct bolt_ctx bolt_ctx;
+
+int bolt_ctx_create(int device, bolt_ctx** out);
+int bolt_ctx_destroy(bolt_ctx* ctx);
+int bolt_ctx_sync(bolt_ctx* ctx);
+
+/* Upload one cosmology's parameters and tables; returns its id. */
+int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_tables* t,
+                      int* cosmo_id);
+int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id);
+
+/* hierarchy!(du, u, h, x) for B modes at one x (src/perturbations.jl:161-271).
+ * u, du: [n][B] (mode fastest).  If mutate != 0 the RSA branch's in-place
+ * update of u (perturbations.jl:217-227) is written back into u. */
+int bolt_rhs_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                   double x, double* u, double* du, int mutate);
+
+/* initial_conditions(x_i, h) for B modes (perturbations.jl:274-338); u0: [n][B]. */
+int bolt_initial_conditions(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg,
+                            const double* k, int B, double* u0);
+
+/* boltsolve(h; reltol, abstol) evaluated at x_save (perturbations.jl:25-33).
+ * u_out: [B][nsave][n]; status: [B]. */
+int bolt_boltsolve_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k,
+                         int B, const double* x_save, int nsave, double* u_out, int* status,
+                         bolt_stats* stats);
+
+/* boltsolve_rsa (perturbations.jl:86-111): the solution on every x node with
+ * the RSA overwrite after the switch.  out: [B][nx][n]. */
+int bolt_boltsolve_rsa_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg,
+                             const double* k, int B, double* out, int* status, bolt_stats* stats);
+
+/* source_grid(par, bg, ih, k_grid, BasicNewtonian(); ℓᵧ, reltol) (spectra.jl:6-23).
+ * S: [nk][nx], i.e. Julia's grid[nx, nk] column-major. */
+int bolt_source_grid(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k,
+                     int nk, double* S, int* status, bolt_stats* stats);
+/* Device-resident source_grid: d_k [nk], d_S [nk][nx], d_status [nk] and
+ * d_counters [nk][4] (accepted, rejected, RHS evaluations, linear solves; may be
+ * NULL) are device pointers; *ms (may be NULL) receives the solver kernel's
+ * device time.  Returns when the kernel has finished. */
+int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg,
+                         const double* d_k, int nk, double* d_S, int* d_status,
+                         int64_t* d_counters, float* ms);
+
+/* cltt(ℓ, sf, kgrid, par, bg) for a list of ℓ (spectra.jl:84-96, 132-150).
+ * S: [nk_src][nx] source grid on k_src; kgrid: the dense k grid (nkd nodes, the
+ * sum runs over its nkd-1 midpoints); bessel_mode 0 = j_ℓ by recurrence. */
+int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src, int nk_src,
+              const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode,
+              double* cl);
+/* Device-resident cltt: S, k_src, kgrid and cl on the device, the ℓ list on the
+ * host.  d_steps (may be NULL) receives the executed recurrence steps per
+ * dense-k midpoint; *ms the LOS kernel's device time. */
+int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
+                  const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
+                  int64_t* d_steps, float* ms);
+
+/* plin(k, par, bg, ih, n_q, ℓᵧ, ℓ_ν, ℓ_mν, x, reltol) for nk wavenumbers
+ * (spectra.jl:163-198); x must be a grid node or 0. */
+int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk,
+              double x, double* pk, int* status, bolt_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BOLT_HIP_H */

@@ -1,0 +1,99 @@
+/*
+ * CPU FP64 oracle for the perturbation hot path -- TEST INFRASTRUCTURE ONLY.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+ * and only as the checker.  The product path never links or calls it.
+ *
+ * It restates Bolt.jl's src/perturbations.jl and src/spectra.jl in plain C on
+ * the same spline tables the device path consumes (bolt_tables layout of
+ * include/bolt_hip.h).
+ */
+#ifndef BOLT_ORACLE_H
+#define BOLT_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* CosmoParams order (src/Bolt.jl:56-66) */
+    double h, Omega_r, Omega_b, Omega_c, A, n, Y_p, N_nu, Sum_m_nu;
+} orc_params;
+
+typedef struct {
+    int nx, nq;
+    double x0, x1;
+    double H0, eta0, rho_crit;
+    const double* xq;   /* [nq] */
+    const double* wq;   /* [nq] */
+    const double* coef; /* [16][nx+2] same order as BOLT_T_* */
+} orc_tables;
+
+typedef struct {
+    int ell_g, ell_nu, ell_mnu, nq;
+    double reltol, abstol;
+    int max_steps;
+    int lu_refine; /* 1: one residual-correction sweep after each stage solve */
+} orc_cfg;
+
+typedef struct {
+    int64_t accepted, rejected, rhs_evals, factorizations;
+    int status; /* 0 ok, 1 max steps, 2 dt underflow, 3 non-finite */
+} orc_stats;
+
+int orc_state_length(const orc_cfg* c);
+/* KenCarp4 ESDIRK tableau used by the integrator: A [6][6] (diagonal γ), b, b̂, c */
+void orc_tableau(double* A, double* b, double* bh, double* c, double* gamma);
+double orc_spline(const orc_tables* t, int tab, double x, int deriv);
+
+/* hierarchy! (perturbations.jl:161-271); mutate != 0 applies the RSA overwrite to u */
+void orc_rhs(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+             double* u, double* du, int mutate);
+/* analytic Jacobian of hierarchy! at (k, x), dense row-major n x n */
+void orc_jacobian(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                  double* J);
+void orc_initial_conditions(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                            double* u0);
+double orc_source(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                  const double* u, const double* du);
+void orc_rho_sigma(const orc_params* p, const orc_tables* t, int nq, double a, const double* M0,
+                   const double* M2, double* rho, double* sigma);
+double orc_theta(const orc_params* p, const orc_tables* t, int nq, double a, const double* M1);
+
+/* boltsolve + dense output at sorted x_save; out [nsave][n] */
+int orc_boltsolve(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                  const double* x_save, int nsave, double* out, orc_stats* st);
+/* boltsolve + accepted-step log (x_n list) for diagnostics */
+int orc_boltsolve_steps(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                        double* xs, int max_xs, int* n_xs, orc_stats* st);
+/* boltsolve_rsa (perturbations.jl:86-111); out [nx][n] */
+int orc_boltsolve_rsa(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                      double* out, orc_stats* st);
+/* source_grid column for one k (spectra.jl:10-18); S [nx] */
+int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                      double* S, orc_stats* st);
+/* source_grid over nk modes with nthreads OpenMP threads; S [nk][nx] */
+int orc_source_grid(const orc_params* p, const orc_tables* t, const orc_cfg* c, const double* k,
+                    int nk, double* S, int* status, int nthreads);
+/* plin (spectra.jl:163-198) */
+double orc_plin(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                orc_stats* st);
+
+/* spherical Bessel j_0..j_lmax at z by normalised downward (Miller) recurrence */
+void orc_sph_bessel(int lmax, double z, double* j);
+/* natural cubic spline table of j_ell on [0, zmax] with 5001 nodes (spectra.jl:49-58):
+ * coef [5003] */
+void orc_bessel_table(int ell, double zmax, double* coef);
+double orc_bessel_table_eval(const double* coef, double zmax, double z);
+
+/* cltt(ℓ⃗, sf, kgrid, par, bg) (spectra.jl:84-96); bessel_mode 0 = exact j_ℓ,
+ * 1 = Bolt's 5001-node spline table.  k-midpoint range [kbeg, kend) of the
+ * nkd-1 midpoints (pass 0, nkd-1 for all); partial sums before the 4π factor
+ * are returned when partial != 0. */
+int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const double* k_src,
+             int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode,
+             double* cl, int nthreads, int kbeg, int kend);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""GPU (libbolt_hip.so) vs the CPU FP64 oracle on identical inputs.
+
+Tolerances: the north-star bars are 1e-6 relative for perturbations and P(k)
+and 1e-5 relative for C_ℓ^TT.  Vector comparisons use the error scaled by the
+largest magnitude of each compared series (perturbation variables cross zero).
+"""
+import numpy as np
+import pytest
+
+import boltmi
+import cosmo
+
+pytestmark = pytest.mark.gpu
+
+
+def scaled_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def rand_state(n, B, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, B))
+
+
+@pytest.mark.parametrize("kind,sizes", [("nonu", (8, 8, 10, 15)), ("massive", (50, 50, 20, 15)),
+                                        ("default", (10, 10, 10, 15))])
+def test_rhs_matches_oracle(kind, sizes):
+    par, bg, ih = cosmo.background(kind)
+    o = cosmo.oracle(kind)
+    ℓg, ℓn, ℓm, nq = sizes
+    cfg = boltmi.hier_cfg(ℓg, ℓn, ℓm, nq)
+    oc = cosmo.ocfg(ℓg, ℓn, ℓm, nq)
+    ks = np.array([1e-4, 3e-3, 0.02, 0.1, 0.4])
+    n = boltmi.state_length(cfg)
+    for x in [-19.3, -12.0, -7.5, -6.9, -3.0, -0.2]:
+        U = rand_state(n, len(ks), 7)
+        dU, _ = boltmi.rhs_batch(par, bg, cfg, ks, x, U)
+        for b, k in enumerate(ks):
+            ref, _ = o.rhs(oc, k, x, U[:, b])
+            assert scaled_err(dU[:, b], ref) < 1e-12, (x, k)
+
+
+def test_rhs_rsa_mutation_matches_oracle():
+    # kη > 240 and -τ′ℋ/η > 100 (perturbations.jl:216) hold only deep in the
+    # radiation era for k > ~50 Mpc⁻¹
+    par, bg, ih = cosmo.background("nonu")
+    o = cosmo.oracle("nonu")
+    cfg = boltmi.hier_cfg(8, 8, 10, 15)
+    oc = cosmo.ocfg(8, 8, 10, 15)
+    ks = np.array([60.0, 80.0, 150.0])
+    x = -11.5
+    assert all(k * bg.eta(x) > 240 for k in ks)
+    assert -ih.tau_p(x) * bg.H(x) / bg.eta(x) > 100
+    n = boltmi.state_length(cfg)
+    U = rand_state(n, len(ks), 11)
+    dU, Um = boltmi.rhs_batch(par, bg, cfg, ks, x, U, mutate=True)
+    for b, k in enumerate(ks):
+        ref, um = o.rhs(oc, k, x, U[:, b], mutate=True)
+        assert np.any(um != U[:, b])
+        assert scaled_err(dU[:, b], ref) < 1e-12
+        assert scaled_err(Um[:, b], um) < 1e-12
+
+
+def test_initial_conditions_match_oracle():
+    par, bg, ih = cosmo.background("massive")
+    o = cosmo.oracle("massive")
+    cfg = boltmi.hier_cfg(50, 50, 20, 15)
+    oc = cosmo.ocfg(50, 50, 20, 15)
+    ks = np.array([1e-4, 0.01, 0.3])
+    U = boltmi.initial_conditions_batch(par, bg, cfg, ks)
+    for b, k in enumerate(ks):
+        ref = o.initial_conditions(oc, k)
+        np.testing.assert_allclose(U[:, b], ref, rtol=1e-13, atol=1e-300)
+
+
+def c1_ks():
+    # 100 log-spaced k in [1e-4, 1] h/Mpc, h = 0.7 (the plain-method workload)
+    return 0.7 * np.logspace(-4, 0, 100)
+
+
+@pytest.mark.parametrize("idx", [0, 40, 70, 99])
+def test_boltsolve_matches_oracle_c1(idx):
+    """δ_c and Φ at 1000 x points, ℓᵧ = ℓ_ν = 10, reltol = abstol = 1e-6."""
+    par, bg, ih = cosmo.background("c1")
+    o = cosmo.oracle("c1")
+    k = c1_ks()[idx]
+    xs = np.linspace(-20, 0, 1000)
+    cfg = boltmi.hier_cfg(10, 10, 10, 15, 1e-6, 1e-6)
+    oc = cosmo.ocfg(10, 10, 10, 15, 1e-6, 1e-6)
+    out, status, st = boltmi.boltsolve_batch(par, bg, cfg, [k], xs)
+    ref, ost = o.boltsolve(oc, k, xs)
+    n = boltmi.state_length(cfg)
+    iphi, idel = n - 5, n - 4
+    assert status[0] == 0
+    assert st["accepted"] == ost.accepted and st["rejected"] == ost.rejected
+    assert scaled_err(out[0][:, iphi], ref[:, iphi]) < 1e-6
+    assert scaled_err(out[0][:, idel], ref[:, idel]) < 1e-6
+    assert scaled_err(out[0], ref) < 1e-6
+
+
+def test_batch_composition_invariance():
+    """A mode's result is bit-identical alone and inside a mixed batch."""
+    par, bg, ih = cosmo.background("c1")
+    ks = c1_ks()[::9]
+    xs = np.linspace(-20, 0, 200)
+    cfg = boltmi.hier_cfg(10, 10, 10, 15, 1e-6, 1e-6)
+    batch, _, _ = boltmi.boltsolve_batch(par, bg, cfg, ks, xs)
+    for b in [0, len(ks) // 2, len(ks) - 1]:
+        alone, _, _ = boltmi.boltsolve_batch(par, bg, cfg, [ks[b]], xs)
+        assert np.array_equal(alone[0], batch[b])
+
+
+@pytest.mark.parametrize("k", [0.0021, 0.05, 0.2])
+def test_source_column_matches_oracle(k):
+    par, bg, ih = cosmo.background("default")
+    o = cosmo.oracle("default")
+    sf = boltmi.source_grid(par, bg, ih, np.array([k]), boltmi.BasicNewtonian(), 8, 1e-11)
+    ref, ost = o.source_column(cosmo.ocfg(8, 8, 10, 15, 1e-11, 1e-6), k)
+    assert sf.stats["accepted"] == ost.accepted
+    assert scaled_err(sf.grid[0], ref) < 1e-6
+
+
+def test_cltt_matches_oracle_same_source():
+    """The LOS kernel against the oracle's exact-j_ℓ cltt on one source grid."""
+    par, bg, ih = cosmo.background("default")
+    o = cosmo.oracle("default")
+    k_src = boltmi.quadratic_k(0.1 * bg.H0, 1000 * bg.H0, 40)
+    S, status = o.source_grid(cosmo.ocfg(8, 8, 10, 15, 1e-8, 1e-6), k_src, nthreads=8)
+    assert not status.any()
+    dense = boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 600)
+    ells = np.array([2, 3, 10, 37, 100, 250, 500, 777])
+    sf = boltmi.SourceGrid(bg.x_grid, k_src, S)
+    cl = boltmi.cltt_on_grid(ells, sf, dense, par, bg)
+    ref = o.cltt(S, k_src, dense, ells, 0, nthreads=8)
+    np.testing.assert_allclose(cl, ref, rtol=1e-6)
+
+
+@pytest.mark.parametrize("k", [0.01, 0.1])
+def test_plin_matches_oracle(k):
+    par, bg, ih = cosmo.background("massive")
+    o = cosmo.oracle("massive")
+    pk = boltmi.plin(k, par, bg, ih)
+    ref, st = o.plin(cosmo.ocfg(50, 50, 20, 15, 1e-5, 1e-6), k)
+    assert abs(pk / ref - 1) < 1e-6
+
+
+def test_boltsolve_rsa_vs_class_fixture(golden):
+    """The reference's own CLASS comparison (runtests.jl:83-147) on the GPU path."""
+    from scipy.interpolate import CubicSpline
+    par, bg, ih = cosmo.background("nonu")
+    d = golden("class_kp03.npz")
+    k = par.h * float(d["k_hMpc"])
+    h = boltmi.Hierarchy(boltmi.BasicNewtonian(), par, bg, ih, k, 50, 50, 20, 15)
+    res = boltmi.boltsolve_rsa(h, reltol=1e-9, abstol=1e-9)
+    n = h.n
+    iphi = n - 5
+    cx = d["x"]
+    phi = CubicSpline(bg.x_grid, res[iphi])(cx)
+    db = CubicSpline(bg.x_grid, res[iphi + 3])(cx)
+    assert np.all(np.abs(phi / CubicSpline(cx, d["phi"])(cx) - 1) < 1e-3)
+    assert np.all(np.abs(-db / CubicSpline(cx, d["delta_b"])(cx) - 1) < 1e-3)
