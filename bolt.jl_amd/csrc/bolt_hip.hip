@@ -7,9 +7,12 @@
 // tight-coupling pair) and s, p, π are the Ψ, Φ′ and Π functionals, the
 // KenCarp4 stage equation (I - hγA) z = r is solved exactly by
 //   * a banded LU (bandwidth <= 2, no pivoting: every chain is a positive
-//     diagonal plus a skew-symmetrisable coupling) of I - hγL, one lane per chain,
+//     diagonal plus a skew-symmetrisable coupling) of I - hγL, one lane per
+//     segment of consecutive chains,
 //   * a 3x3 capacitance system for the Ψ/Φ′/Π border (Woodbury).
-// One wavefront integrates one k-mode; state vectors live in LDS in "chain
+// One workgroup of four wavefronts (one per SIMD) integrates one k-mode: row
+// passes spread over its 256 lanes, the four right-hand sides of a stage solve
+// run on the four wavefronts at once.  State vectors live in LDS in "chain
 // order" (each hierarchy contiguous); a persistent grid pulls modes from an
 // atomic work counter, costliest (largest k) first.
 #include <hip/hip_runtime.h>
@@ -30,6 +33,22 @@ int set_error(int code, const std::string& msg);
 using bolt::set_error;
 
 #define WAVE 64
+#define NWAVE 4                // wavefronts per k-mode in the solver
+#define NTHR (NWAVE * WAVE)
+#define MAXSEG 64              // independent row segments of the banded solve
+
+// thread coordinates in the workgroup that owns one k-mode
+struct Thr {
+    int tid, nt, lane, wv;
+};
+__device__ inline Thr thr_self() {
+    Thr t;
+    t.tid = threadIdx.x;
+    t.nt = blockDim.x;
+    t.lane = t.tid & (WAVE - 1);
+    t.wv = t.tid / WAVE;
+    return t;
+}
 
 // ---------------------------------------------------------------- device data
 struct CosmoDev {
@@ -120,20 +139,48 @@ __device__ inline double spl(const CosmoDev& C, int tab, double x, int deriv = 0
     return (r * q[0] + (-2.0 + 3.0 * s) * q[1] + (-2.0 + 3.0 * r) * q[2] + s * q[3]) / (C.dx * C.dx);
 }
 
+// ------------------------------------------------------------ reductions
+// 64-bit DPP move within a row of 16 lanes; lanes whose source is outside the
+// row read 0
+template <int CTRL>
+__device__ inline double dpp64(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double readlane64(double v, int l) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+// Σ over the wavefront, the same value (same rounding) in every lane: an
+// inclusive scan inside each row of 16 (row_shr 1, 2, 4, 8), then the four row
+// totals added in a fixed order.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int m = 1; m < WAVE; m <<= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
+    v += dpp64<0x111>(v);
+    v += dpp64<0x112>(v);
+    v += dpp64<0x114>(v);
+    v += dpp64<0x118>(v);
+    return ((readlane64(v, 15) + readlane64(v, 31)) + readlane64(v, 47)) + readlane64(v, 63);
 }
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int m = 1; m < WAVE; m <<= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
-    return v;
+
+// 1/d from v_rcp_f64 refined by two Newton steps (pivots here are O(1) to
+// O(1e12): no denormal or overflow range is involved)
+__device__ inline double rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
 }
+
+// largest hierarchy cut-off accepted (row descriptors pack ℓ in 11 bits)
+#define LTAB 512
 
 // x-dependent coefficients of hierarchy! (perturbations.jl:163-172)
 struct XC {
-    double x, a, Hc, eta, tp, tpp, csb2, kH, R, c12, gg, k23;
+    double x, a, Hc, eta, tp, tpp, csb2, kH, R, c12, gg, k23, iHeta;
     bool rsa;
 };
 
@@ -152,124 +199,280 @@ __device__ inline XC make_xc(const CosmoDev& C, double k, double x) {
     X.c12 = 12.0 * H02 / (k * k) / (X.a * X.a);
     X.gg = H02 / (2.0 * X.Hc * X.Hc);
     X.k23 = k * k / (3.0 * X.Hc * X.Hc);
+    X.iHeta = 1.0 / (X.Hc * X.eta);
     X.rsa = (k * X.eta > 240.0) && (-X.tp * X.Hc / X.eta > 100.0);  // perturbations.jl:216
     return X;
 }
 
-// LDS workspace of one wavefront
+// LDS workspace of one k-mode: 20 vectors of n doubles, 7 per-q arrays, two
+// per-row constants, cross-wavefront partial sums, the per-row descriptors,
+// the row evaluation order and the segment table of the banded solve.  Every
+// pointer is base + offset arithmetic.
 struct WS {
-    double *u, *f, *un, *fn, *arg, *w, *Z[6];
-    double *l2, *l1, *d, *u1, *u2, *cS, *cP, *cPi;
-    double *q, *df0, *base, *qe, *eqdf, *sw, *rw;
+    double* b;
+    int n, nq;
+    __device__ double* u() const { return b; }
+    __device__ double* f() const { return b + n; }
+    __device__ double* un() const { return b + 2 * n; }
+    __device__ double* fn() const { return b + 3 * n; }
+    __device__ double* arg() const { return b + 4 * n; }
+    __device__ double* w() const { return b + 5 * n; }
+    __device__ double* Z(int j) const { return b + (6 + j) * n; }
+    __device__ double* l2() const { return b + 12 * n; }
+    __device__ double* l1() const { return b + 13 * n; }
+    __device__ double* d() const { return b + 14 * n; }
+    __device__ double* u1() const { return b + 15 * n; }
+    __device__ double* u2() const { return b + 16 * n; }
+    __device__ double* cS() const { return b + 17 * n; }
+    __device__ double* cP() const { return b + 18 * n; }
+    __device__ double* cPi() const { return b + 19 * n; }
+    __device__ double* q() const { return b + 20 * n; }
+    __device__ double* df0() const { return b + 20 * n + nq; }
+    __device__ double* base() const { return b + 20 * n + 2 * nq; }
+    __device__ double* qe() const { return b + 20 * n + 3 * nq; }
+    __device__ double* eqdf() const { return b + 20 * n + 4 * nq; }
+    __device__ double* sw() const { return b + 20 * n + 5 * nq; }
+    __device__ double* rw() const { return b + 20 * n + 6 * nq; }
+    __device__ double* ra() const { return b + 20 * n + 7 * nq; }   // ℓ/(2ℓ+1)
+    __device__ double* rb() const { return b + 21 * n + 7 * nq; }   // (ℓ+1)/(2ℓ+1)
+    __device__ double* red() const { return b + 22 * n + 7 * nq; }  // [8]
+    __device__ int* desc() const { return (int*)(b + 22 * n + 7 * nq + 8); }
+    __device__ int* order() const { return desc() + n; }            // [n + NTHR]
+    __device__ int* seg() const { return desc() + 2 * n + NTHR; }   // [MAXSEG][2]: c0, m
+    __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg, nslot
 };
 
 __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
-    double* p = lds;
-    W.u = p; p += n; W.f = p; p += n; W.un = p; p += n; W.fn = p; p += n;
-    W.arg = p; p += n; W.w = p; p += n;
-    for (int i = 0; i < 6; ++i) { W.Z[i] = p; p += n; }
-    W.l2 = p; p += n; W.l1 = p; p += n; W.d = p; p += n; W.u1 = p; p += n; W.u2 = p; p += n;
-    W.cS = p; p += n; W.cP = p; p += n; W.cPi = p; p += n;
-    W.q = p; p += nq; W.df0 = p; p += nq; W.base = p; p += nq;
-    W.qe = p; p += nq; W.eqdf = p; p += nq; W.sw = p; p += nq; W.rw = p; p += nq;
+    W.b = lds;
+    W.n = n;
+    W.nq = nq;
 }
-__host__ inline size_t ws_bytes(int n, int nq) { return sizeof(double) * ((size_t)20 * n + 7 * nq); }
+__host__ inline size_t ws_bytes(int n, int nq) {
+    return sizeof(double) * ((size_t)22 * n + 7 * nq + 8) + sizeof(int) * (size_t)(2 * n + NTHR + 2 * MAXSEG + 4);
+}
+
+// row types of the chain-ordered state
+enum {
+    R_DB = 0, R_VB, R_T0, R_T1, R_TL, R_TN, R_P0, R_PL, R_PN, R_N0, R_N1, R_NL, R_NN,
+    R_M0, R_M1, R_ML, R_MN, R_DEL, R_V, R_PHI
+};
+__device__ __host__ inline int pack_desc(int type, int l, int qi) { return type | (l << 5) | (qi << 16); }
+
+__device__ __host__ inline int row_desc(const Geo& G, int c) {
+    if (c < G.cB) {
+        if (c == 0) return pack_desc(R_DB, 0, 0);
+        if (c == 1) return pack_desc(R_VB, 0, 0);
+        int l = c - 2;
+        return pack_desc(l == 0 ? R_T0 : l == 1 ? R_T1 : l < G.ng ? R_TL : R_TN, l, 0);
+    }
+    if (c < G.cC) {
+        int l = c - G.cB;
+        return pack_desc(l == 0 ? R_P0 : l < G.ng ? R_PL : R_PN, l, 0);
+    }
+    if (c < G.cM) {
+        int l = c - G.cC;
+        return pack_desc(l == 0 ? R_N0 : l == 1 ? R_N1 : l < G.nn ? R_NL : R_NN, l, 0);
+    }
+    if (c < G.cD) {
+        int t = c - G.cM, qi = t / (G.nm + 1), l = t % (G.nm + 1);
+        return pack_desc(l == 0 ? R_M0 : l == 1 ? R_M1 : l < G.nm ? R_ML : R_MN, l, qi);
+    }
+    if (c == G.cD) return pack_desc(R_DEL, 0, 0);
+    if (c == G.cD + 1) return pack_desc(R_V, 0, 0);
+    return pack_desc(R_PHI, 0, 0);
+}
+
+__device__ inline bool interior_type(int type) {
+    return type == R_TL || type == R_PL || type == R_NL || type == R_ML;
+}
+
+// per-mode row tables (caller syncs):
+//  * descriptors and the ℓ couplings ℓ/(2ℓ+1), (ℓ+1)/(2ℓ+1) of every row;
+//  * the evaluation order of the row pass: interior hierarchy rows (one
+//    uniform formula) in the full passes, boundary rows grouped by type in the
+//    last pass, spread evenly over the wavefronts (slot -1 = idle lane);
+//  * the segments of the banded solve: consecutive chains packed greedily up
+//    to the longest chain's length.  Chains never couple to each other (the
+//    first row of a chain has l1 = l2 = 0, the last u1 = u2 = 0), so the LU of
+//    a segment is that of its chains and one lane can sweep a whole segment.
+__device__ inline void setup_rows(const Geo& G, const WS& W, const Thr& T) {
+    for (int c = T.tid; c < G.n; c += T.nt) {
+        int dsc = row_desc(G, c), l = (dsc >> 5) & 2047;
+        W.desc()[c] = dsc;
+        W.ra()[c] = (double)l / (double)(2 * l + 1);
+        W.rb()[c] = (double)(l + 1) / (double)(2 * l + 1);
+    }
+    if (T.tid == 0) {
+        int* ord = W.order();
+        const int nfull = G.n / T.nt, r = G.n - nfull * T.nt, nw = T.nt / WAVE;
+        const int nslot = nfull * T.nt + (r > 0 ? T.nt : 0);
+        for (int t = nfull * T.nt; t < nslot; ++t) ord[t] = -1;
+        // rows in list order: interior rows, then boundary rows by type
+        int t = 0, w = 0, wi = 0;
+        for (int pass = 0; pass <= R_PHI + 1; ++pass)
+            for (int c = 0; c < G.n; ++c) {
+                int type = row_desc(G, c) & 31;
+                bool take = pass == 0 ? interior_type(type) : (!interior_type(type) && type == pass - 1);
+                if (!take) continue;
+                if (t < nfull * T.nt) {
+                    ord[t++] = c;
+                } else {
+                    int cnt = r / nw + (w < r % nw ? 1 : 0);
+                    while (wi >= cnt) { ++w; wi = 0; cnt = r / nw + (w < r % nw ? 1 : 0); }
+                    ord[nfull * T.nt + w * WAVE + wi] = c;
+                    ++wi;
+                }
+            }
+        int lmax = 0;
+        for (int ch = 0; ch < G.nch; ++ch) {
+            int c0, m;
+            chain_range(G, ch, c0, m);
+            lmax = m > lmax ? m : lmax;
+        }
+        int ns = 0, cur0 = 0, curm = 0;
+        for (int ch = 0; ch < G.nch; ++ch) {
+            int c0, m;
+            chain_range(G, ch, c0, m);
+            if (curm > 0 && curm + m > lmax) {
+                W.seg()[2 * ns] = cur0; W.seg()[2 * ns + 1] = curm; ++ns;
+                curm = 0;
+            }
+            if (curm == 0) cur0 = c0;
+            curm += m;
+        }
+        W.seg()[2 * ns] = cur0; W.seg()[2 * ns + 1] = curm; ++ns;
+        W.misc()[0] = ns;
+        W.misc()[1] = nslot;
+    }
+}
 
 // per-x q-quantities (perturbations.jl:203-213, 127-145); lanes < nq; caller syncs
-__device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, WS& W, int lane) {
-    if (lane < G.nq) {
-        double q = W.q[lane];
+__device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const Thr& T) {
+    if (T.tid < G.nq) {
+        const int lane = T.tid;
+        double q = W.q()[lane];
         double am = X.a * C.mnu;
         double eps = sqrt(q * q + am * am);
-        W.qe[lane] = q / eps;
-        W.eqdf[lane] = eps / q * W.df0[lane];
-        W.sw[lane] = W.base[lane] * (q * q / eps);
-        W.rw[lane] = W.base[lane] * eps;
+        double ie = 1.0 / eps;
+        W.qe()[lane] = q * ie;
+        W.eqdf()[lane] = eps / q * W.df0()[lane];
+        W.sw()[lane] = W.base()[lane] * (q * q * ie);
+        W.rw()[lane] = W.base()[lane] * eps;
     }
 }
 
-// L bands and border columns c_S, c_P, c_Π at X, chain order
-__device__ void assemble(const CosmoDev& C, const Geo& G, const XC& X, WS& W, int lane) {
+// One row of du = L u + c_S Ψ + c_P Φ′ + c_Π Π (perturbations.jl:196-265)
+struct RowC {
+    double l2, l1, d, u1, u2, cs, cp, cpi;
+};
+
+__device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, int desc, int c) {
+    RowC r = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int type = desc & 31, l = (desc >> 5) & 2047, qi = desc >> 16;
     const double kH = X.kH, tp = X.tp;
-    const bool rsa = X.rsa;
-    const double Heta = X.Hc * X.eta;
-    for (int c = lane; c < G.n; c += WAVE) {
-        double l2 = 0, l1 = 0, d = 0, u1 = 0, u2 = 0, cs = 0, cp = 0, cpi = 0;
-        if (c < G.cB) {  // chain A: δb, vb, Θ
-            if (c == 0) { u1 = kH; cp = -3.0; }
-            else if (c == 1) { l1 = -kH * X.csb2; d = -1.0 + tp * X.R; u2 = 3.0 * tp * X.R; cs = -kH; }
-            else if (!rsa) {
-                int l = c - 2;
-                if (l == 0) { u1 = -kH; cp = -1.0; }
-                else if (l == 1) { l2 = tp / 3.0; l1 = kH / 3.0; d = tp; u1 = -2.0 * kH / 3.0; cs = kH / 3.0; }
-                else if (l < G.ng) {
-                    l1 = l * kH / (2 * l + 1); d = tp; u1 = -(l + 1) * kH / (2 * l + 1);
-                    if (l == 2) cpi = -tp / 10.0;
-                } else { l1 = kH; d = -((G.ng + 1) / Heta - tp); }
-            }
-        } else if (c < G.cC) {  // chain B: Θᵖ
-            if (!rsa) {
-                int l = c - G.cB;
-                if (l == 0) { d = tp; u1 = -kH; cpi = -tp / 2.0; }
-                else if (l < G.ng) {
-                    l1 = l * kH / (2 * l + 1); d = tp; u1 = -(l + 1) * kH / (2 * l + 1);
-                    if (l == 2) cpi = -tp / 10.0;
-                } else { l1 = kH; d = -((G.ng + 1) / Heta - tp); }
-            }
-        } else if (c < G.cM) {  // chain C: 𝒩
-            if (!rsa) {
-                int l = c - G.cC;
-                if (l == 0) { u1 = -kH; cp = -1.0; }
-                else if (l == 1) { l1 = kH / 3.0; u1 = -2.0 * kH / 3.0; cs = kH / 3.0; }
-                else if (l < G.nn) { l1 = l * kH / (2 * l + 1); u1 = -(l + 1) * kH / (2 * l + 1); }
-                else { l1 = kH; d = -(G.nn + 1) / Heta; }
-            }
-        } else if (c < G.cD) {  // chains M_i
-            int t = c - G.cM, qi = t / (G.nm + 1), l = t % (G.nm + 1);
-            double qe = W.qe[qi];
-            if (l == 0) { u1 = -kH * qe; cp = W.df0[qi]; }
-            else if (l == 1) { l1 = kH * qe / 3.0; u1 = -2.0 * kH * qe / 3.0; cs = -kH / 3.0 * W.eqdf[qi]; }
-            else if (l < G.nm) { l1 = kH * qe * l / (2 * l + 1); u1 = -kH * qe * (l + 1) / (2 * l + 1); }
-            else { l1 = qe * kH; d = -(G.nm + 1) / Heta; }
-        } else if (c == G.cD) { u1 = kH; cp = -3.0; }      // δ
-        else if (c == G.cD + 1) { d = -1.0; cs = -kH; }   // v
-        else { cp = 1.0; }                                  // Φ
-        W.l2[c] = l2; W.l1[c] = l1; W.d[c] = d; W.u1[c] = u1; W.u2[c] = u2;
-        W.cS[c] = cs; W.cP[c] = cp; W.cPi[c] = cpi;
+    const double third = 1.0 / 3.0;
+    const bool photon = type >= R_T0 && type <= R_PN;
+    const bool photon_nu = type >= R_T0 && type <= R_NN;
+    if (photon_nu && X.rsa) return r;  // RSA: Θ′ = Θᵖ′ = 𝒩′ = 0 (perturbations.jl:229-232)
+    if (type == R_TL || type == R_PL || type == R_NL || type == R_ML) {
+        // interior of a hierarchy: ℓ/(2ℓ+1) and (ℓ+1)/(2ℓ+1) couplings
+        double kf = (type == R_ML) ? kH * W.qe()[qi] : kH;
+        r.l1 = kf * W.ra()[c];
+        r.u1 = -kf * W.rb()[c];
+        r.d = photon ? tp : 0.0;
+        r.cpi = (photon && l == 2) ? -tp * 0.1 : 0.0;
+        return r;
     }
+    if (type == R_TN || type == R_PN || type == R_NN || type == R_MN) {
+        // truncation rows
+        double kf = (type == R_MN) ? kH * W.qe()[qi] : kH;
+        // value selects (a select between struct fields would index the struct)
+        const double Lg = (double)(G.ng + 1), Ln = (double)(G.nn + 1), Lm = (double)(G.nm + 1);
+        double L1 = (type == R_NN) ? Ln : Lg;
+        L1 = (type == R_MN) ? Lm : L1;
+        r.l1 = kf;
+        r.d = -L1 * X.iHeta + (photon ? tp : 0.0);
+        return r;
+    }
+    switch (type) {
+        case R_DB: r.u1 = kH; r.cp = -3.0; break;
+        case R_VB: r.l1 = -kH * X.csb2; r.d = -1.0 + tp * X.R; r.u2 = 3.0 * tp * X.R; r.cs = -kH; break;
+        case R_T0: case R_N0: r.u1 = -kH; r.cp = -1.0; break;
+        case R_T1: r.l2 = tp * third; r.l1 = kH * third; r.d = tp; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
+        case R_P0: r.d = tp; r.u1 = -kH; r.cpi = -tp * 0.5; break;
+        case R_N1: r.l1 = kH * third; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
+        case R_M0: r.u1 = -kH * W.qe()[qi]; r.cp = W.df0()[qi]; break;
+        case R_M1: {
+            double qe = W.qe()[qi];
+            r.l1 = kH * qe * third; r.u1 = -2.0 * kH * qe * third; r.cs = -kH * third * W.eqdf()[qi];
+            break;
+        }
+        case R_DEL: r.u1 = kH; r.cp = -3.0; break;
+        case R_V: r.d = -1.0; r.cs = -kH; break;
+        default: r.cp = 1.0; break;  // R_PHI
+    }
+    return r;
 }
 
-// Ψ, Φ′, Π of a chain-ordered vector (perturbations.jl:184-194, 247)
-__device__ inline void functionals(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
-                                   int lane, double& S, double& P, double& Pi) {
+__device__ inline double row_apply(const RowC& r, const double* v, int c, int n, double S, double P, double Pi) {
+    double acc = r.d * v[c];
+    if (c >= 1) acc += r.l1 * v[c - 1];
+    if (c >= 2) acc += r.l2 * v[c - 2];
+    if (c + 1 < n) acc += r.u1 * v[c + 1];
+    if (c + 2 < n) acc += r.u2 * v[c + 2];
+    return acc + r.cs * S + r.cp * P + r.cpi * Pi;
+}
+
+// Ψ, Φ′, Π of chain-ordered vectors (perturbations.jl:184-194, 247)
+struct Fun {
+    double S, P, Pi;
+};
+
+__device__ inline Fun fun_finish(const CosmoDev& C, const Geo& G, const XC& X, const double* v, double s2,
+                                 double s0) {
+    Fun F;
+    double a = X.a;
+    F.S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2] + s2 / C.rho_crit / 4.0);
+    F.P = F.S - X.k23 * v[G.cE] +
+          X.gg * (C.Oc / a * v[G.cD] + C.Ob / a * v[0] + 4.0 * C.Or / (a * a) * v[2] +
+                  4.0 * C.Onu / (a * a) * v[G.cC] + s0 / C.rho_crit / (a * a));
+    F.Pi = v[4] + v[G.cB + 2] + v[G.cB];
+    return F;
+}
+
+// every wavefront forms the sums over its own lanes (identical in all of them)
+__device__ inline Fun functionals(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
+                                  const Thr& T) {
     double s2 = 0.0, s0 = 0.0;
+    const int lane = T.lane;
     if (lane < G.nq) {
         int b = G.cM + lane * (G.nm + 1);
-        s2 = W.sw[lane] * v[b + 2];
-        s0 = W.rw[lane] * v[b];
+        s2 = W.sw()[lane] * v[b + 2];
+        s0 = W.rw()[lane] * v[b];
     }
-    s2 = wave_sum(s2);
-    s0 = wave_sum(s0);
-    double a = X.a;
-    S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2] + s2 / C.rho_crit / 4.0);
-    P = S - X.k23 * v[G.cE] +
-        X.gg * (C.Oc / a * v[G.cD] + C.Ob / a * v[0] + 4.0 * C.Or / (a * a) * v[2] +
-                4.0 * C.Onu / (a * a) * v[G.cC] + s0 / C.rho_crit / (a * a));
-    Pi = v[4] + v[G.cB + 2] + v[G.cB];
+    return fun_finish(C, G, X, v, wave_sum(s2), wave_sum(s0));
 }
 
-// out = scale * (L v + c_S S + c_P P + c_Π Π)
-__device__ inline void apply_rhs(const Geo& G, const WS& W, const double* v, double S, double P, double Pi,
-                                 double scale, double* out, int lane) {
-    for (int c = lane; c < G.n; c += WAVE) {
-        double acc = W.d[c] * v[c];
-        if (c >= 1) acc += W.l1[c] * v[c - 1];
-        if (c >= 2) acc += W.l2[c] * v[c - 2];
-        if (c + 1 < G.n) acc += W.u1[c] * v[c + 1];
-        if (c + 2 < G.n) acc += W.u2[c] * v[c + 2];
-        acc += W.cS[c] * S + W.cP[c] * P + W.cPi[c] * Pi;
-        out[c] = scale * acc;
+// du = f(v) on all rows (out), optionally writing W = I - hγL bands and the
+// border columns for the stage solve; returns the functionals of v
+__device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
+                                     double scale, double* out, double hg, bool make_w, const Thr& T) {
+    Fun F = functionals(C, G, X, W, v, T);
+    const int* desc = W.desc();
+    const int* order = W.order();
+    const int nslot = W.misc()[1];
+    for (int t = T.tid; t < nslot; t += T.nt) {
+        const int c = order[t];
+        if (c < 0) continue;
+        RowC r = row_coef(G, X, W, desc[c], c);
+        out[c] = scale * row_apply(r, v, c, G.n, F.S, F.P, F.Pi);
+        if (make_w) {
+            W.l2()[c] = -hg * r.l2; W.l1()[c] = -hg * r.l1; W.d()[c] = 1.0 - hg * r.d;
+            W.u1()[c] = -hg * r.u1; W.u2()[c] = -hg * r.u2;
+            W.cS()[c] = r.cs; W.cP()[c] = r.cp; W.cPi()[c] = r.cpi;
+        }
     }
+    return F;
 }
 
 // RSA overwrite of u (perturbations.jl:217-227); one lane; caller syncs around
@@ -284,78 +487,105 @@ __device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v
     v[G.cC + 2] = 0.0;
 }
 
-// I - hγL in place
-__device__ inline void to_w(const Geo& G, WS& W, double hg, int lane) {
-    for (int c = lane; c < G.n; c += WAVE) {
-        W.l2[c] *= -hg; W.l1[c] *= -hg; W.u1[c] *= -hg; W.u2[c] *= -hg;
-        W.d[c] = 1.0 - hg * W.d[c];
+// Banded LU (bandwidth 2, no pivoting) of every segment, one lane per segment, in
+// left-looking form: row j is finished from rows j-1, j-2 only, so the carried
+// dependency is the pivot recurrence.  Rows are processed in blocks of CB: a
+// block's inputs are loaded into registers before its sequential sweep.
+// Stored: d <- 1/pivot, l1/l2 <- multipliers, u1 <- updated upper entry (u2 is
+// never modified by the elimination).  Reads past the chain end stay inside the
+// LDS workspace and are never stored back.
+#define CB 8
+__device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
+    if (T.tid >= W.misc()[0]) return;
+    const int c0 = W.seg()[2 * T.tid], m = W.seg()[2 * T.tid + 1];
+    double* l2 = W.l2() + c0; double* l1 = W.l1() + c0; double* d = W.d() + c0;
+    double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
+    double ip1 = 0.0, ip2 = 0.0, u1p1 = 0.0, u1p2 = 0.0, u2p1 = 0.0, u2p2 = 0.0;
+    for (int jb = 0; jb < m; jb += CB) {
+        double L2[CB], L1[CB], D[CB], U1[CB], U2[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            L2[i] = l2[jb + i]; L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; U2[i] = u2[jb + i];
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            double m2 = L2[i] * ip2;
+            double m1 = (L1[i] - m2 * u1p2) * ip1;
+            double dd = (D[i] - m2 * u2p2) - m1 * u1p1;
+            double uu = U1[i] - m1 * u2p1;
+            double inv = rcp(dd);
+            L2[i] = m2; L1[i] = m1; D[i] = inv; U1[i] = uu;
+            ip2 = ip1; ip1 = inv;
+            u1p2 = u1p1; u1p1 = uu;
+            u2p2 = u2p1; u2p1 = U2[i];
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            if (jb + i < m) { l2[jb + i] = L2[i]; l1[jb + i] = L1[i]; d[jb + i] = D[i]; u1[jb + i] = U1[i]; }
+        }
     }
 }
 
-__device__ inline double rcp(double d) { return 1.0 / d; }
-
-// factor each chain (no pivoting, bandwidth 2) and solve for w, cS, cP, cPi
-__device__ void chain_factor_solve4(const Geo& G, WS& W, int lane) {
-    if (lane >= G.nch) return;
-    int c0, m;
-    chain_range(G, lane, c0, m);
-    double* l2 = W.l2; double* l1 = W.l1; double* d = W.d; double* u1 = W.u1; double* u2 = W.u2;
-    double* b0 = W.w; double* b1 = W.cS; double* b2 = W.cP; double* b3 = W.cPi;
-    for (int j = 0; j < m; ++j) {
-        int c = c0 + j;
-        double inv = rcp(d[c]);
-        d[c] = inv;
-        double x0 = b0[c], x1 = b1[c], x2 = b2[c], x3 = b3[c];
-        double uu1 = u1[c], uu2 = u2[c];
-        if (j + 1 < m) {
-            double m1 = l1[c + 1] * inv;
-            l1[c + 1] = m1;
-            d[c + 1] -= m1 * uu1;
-            u1[c + 1] -= m1 * uu2;
-            b0[c + 1] -= m1 * x0; b1[c + 1] -= m1 * x1; b2[c + 1] -= m1 * x2; b3[c + 1] -= m1 * x3;
+// forward + back substitution of one chain for one right-hand side, in blocks
+__device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
+    const double* l2 = W.l2() + c0; const double* l1 = W.l1() + c0; const double* d = W.d() + c0;
+    const double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
+    double* b = bvec + c0;
+    double y1 = 0.0, y2 = 0.0;
+    for (int jb = 0; jb < m; jb += CB) {
+        double B[CB], L1[CB], L2[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; L2[i] = l2[jb + i]; }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            double t = B[i] - L2[i] * y2;
+            double y = t - L1[i] * y1;
+            B[i] = y;
+            y2 = y1; y1 = y;
         }
-        if (j + 2 < m) {
-            double m2 = l2[c + 2] * inv;
-            l2[c + 2] = m2;
-            l1[c + 2] -= m2 * uu1;
-            d[c + 2] -= m2 * uu2;
-            b0[c + 2] -= m2 * x0; b1[c + 2] -= m2 * x1; b2[c + 2] -= m2 * x2; b3[c + 2] -= m2 * x3;
-        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
     }
-    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // x[c+1], x[c+2]
-    for (int j = m - 1; j >= 0; --j) {
-        int c = c0 + j;
-        double inv = d[c], a1 = (j + 1 < m) ? u1[c] : 0.0, a2 = (j + 2 < m) ? u2[c] : 0.0;
-        double y0 = (b0[c] - a1 * p0 - a2 * q0) * inv;
-        double y1 = (b1[c] - a1 * p1 - a2 * q1) * inv;
-        double y2 = (b2[c] - a1 * p2 - a2 * q2) * inv;
-        double y3 = (b3[c] - a1 * p3 - a2 * q3) * inv;
-        b0[c] = y0; b1[c] = y1; b2[c] = y2; b3[c] = y3;
-        q0 = p0; q1 = p1; q2 = p2; q3 = p3;
-        p0 = y0; p1 = y1; p2 = y2; p3 = y3;
+    // backward, blocks from the top; rows >= m read as zero contributions
+    double x1 = 0.0, x2 = 0.0;
+    int top = ((m + CB - 1) / CB) * CB;
+    for (int jb = top - CB; jb >= 0; jb -= CB) {
+        double B[CB], U1[CB], U2[CB], D[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            bool in = jb + i < m;
+            B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; U2[i] = in ? u2[jb + i] : 0.0;
+            D[i] = in ? d[jb + i] : 0.0;
+        }
+#pragma unroll
+        for (int i = CB - 1; i >= 0; --i) {
+            double t = B[i] - U2[i] * x2;
+            double x = (t - U1[i] * x1) * D[i];
+            B[i] = x;
+            x2 = x1; x1 = x;
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
     }
 }
 
-// solve with stored factors for one right-hand side
-__device__ void chain_solve1(const Geo& G, const WS& W, double* b, int lane) {
-    if (lane >= G.nch) return;
-    int c0, m;
-    chain_range(G, lane, c0, m);
-    for (int j = 0; j < m; ++j) {
-        int c = c0 + j;
-        double x = b[c];
-        if (j + 1 < m) b[c + 1] -= W.l1[c + 1] * x;
-        if (j + 2 < m) b[c + 2] -= W.l2[c + 2] * x;
+// the four right-hand sides w, c_S, c_P, c_Π after chain_factor: one
+// (segment, rhs) unit per lane; with four wavefronts, wavefront r takes
+// right-hand side r
+__device__ inline void chain_solve4(const Geo& G, const WS& W, const Thr& T) {
+    const int nseg = W.misc()[0];
+    for (int u = T.tid; u < 4 * WAVE; u += T.nt) {
+        const int r = u / WAVE, sg = u % WAVE;
+        if (sg >= nseg) continue;
+        double* bv = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
+        chain_subst(W, W.seg()[2 * sg], W.seg()[2 * sg + 1], bv);
     }
-    double p = 0, q = 0;
-    for (int j = m - 1; j >= 0; --j) {
-        int c = c0 + j;
-        double a1 = (j + 1 < m) ? W.u1[c] : 0.0, a2 = (j + 2 < m) ? W.u2[c] : 0.0;
-        double y = (b[c] - a1 * p - a2 * q) * W.d[c];
-        b[c] = y;
-        q = p;
-        p = y;
-    }
+}
+
+// one right-hand side with the stored factors, one lane per segment
+__device__ inline void chain_solve1(const Geo& G, const WS& W, double* bvec, const Thr& T) {
+    if (T.tid >= W.misc()[0]) return;
+    chain_subst(W, W.seg()[2 * T.tid], W.seg()[2 * T.tid + 1], bvec);
 }
 
 // 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
@@ -363,52 +593,84 @@ struct Cap {
     double m[3][3];
 };
 
-__device__ inline Cap build_cap(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double hg, int lane) {
+__device__ inline void swap_rows_if(bool c, double* x, double* y) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        double t = x[j];
+        x[j] = c ? y[j] : x[j];
+        y[j] = c ? t : y[j];
+    }
+}
+
+// Gaussian elimination with partial pivoting, fully unrolled (all indices
+// compile-time, so the 3x4 array stays in registers)
+__device__ inline void solve3(const Cap& K, const double* r, double* y) {
+    double a0[4] = {K.m[0][0], K.m[0][1], K.m[0][2], r[0]};
+    double a1[4] = {K.m[1][0], K.m[1][1], K.m[1][2], r[1]};
+    double a2[4] = {K.m[2][0], K.m[2][1], K.m[2][2], r[2]};
+    swap_rows_if(fabs(a1[0]) > fabs(a0[0]), a0, a1);
+    swap_rows_if(fabs(a2[0]) > fabs(a0[0]), a0, a2);
+    double f1 = a1[0] / a0[0], f2 = a2[0] / a0[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) { a1[j] -= f1 * a0[j]; a2[j] -= f2 * a0[j]; }
+    swap_rows_if(fabs(a2[1]) > fabs(a1[1]), a1, a2);
+    double g2 = a2[1] / a1[1];
+#pragma unroll
+    for (int j = 2; j < 4; ++j) a2[j] -= g2 * a1[j];
+    y[2] = a2[3] / a2[2];
+    y[1] = (a1[3] - a1[2] * y[2]) / a1[1];
+    y[0] = (a0[3] - a0[1] * y[1] - a0[2] * y[2]) / a0[0];
+}
+
+// After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
+// matrix and finish z = w + hγ Y M⁻¹ D w in place
+__device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double hg,
+                                         const Thr& T) {
+    const int lane = T.lane;
+    double a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = 0.0;
+    if (lane < G.nq) {
+        int b = G.cM + lane * (G.nm + 1);
+        double sw = W.sw()[lane], rw = W.rw()[lane];
+        a[0] = sw * W.w()[b + 2];   a[1] = rw * W.w()[b];
+        a[2] = sw * W.cS()[b + 2];  a[3] = rw * W.cS()[b];
+        a[4] = sw * W.cP()[b + 2];  a[5] = rw * W.cP()[b];
+        a[6] = sw * W.cPi()[b + 2]; a[7] = rw * W.cPi()[b];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = wave_sum(a[i]);
+    Fun Fw = fun_finish(C, G, X, W.w(), a[0], a[1]);
+    Fun Fs = fun_finish(C, G, X, W.cS(), a[2], a[3]);
+    Fun Fp = fun_finish(C, G, X, W.cP(), a[4], a[5]);
+    Fun Fq = fun_finish(C, G, X, W.cPi(), a[6], a[7]);
     Cap K;
-    double s, p, pi;
-    functionals(C, G, X, W, W.cS, lane, s, p, pi);
-    K.m[0][0] = 1.0 - hg * s; K.m[1][0] = -hg * p; K.m[2][0] = -hg * pi;
-    functionals(C, G, X, W, W.cP, lane, s, p, pi);
-    K.m[0][1] = -hg * s; K.m[1][1] = 1.0 - hg * p; K.m[2][1] = -hg * pi;
-    functionals(C, G, X, W, W.cPi, lane, s, p, pi);
-    K.m[0][2] = -hg * s; K.m[1][2] = -hg * p; K.m[2][2] = 1.0 - hg * pi;
+    K.m[0][0] = 1.0 - hg * Fs.S; K.m[0][1] = -hg * Fp.S; K.m[0][2] = -hg * Fq.S;
+    K.m[1][0] = -hg * Fs.P; K.m[1][1] = 1.0 - hg * Fp.P; K.m[1][2] = -hg * Fq.P;
+    K.m[2][0] = -hg * Fs.Pi; K.m[2][1] = -hg * Fp.Pi; K.m[2][2] = 1.0 - hg * Fq.Pi;
+    double r[3] = {Fw.S, Fw.P, Fw.Pi}, y[3];
+    solve3(K, r, y);
+    __syncthreads();
+    for (int c = T.tid; c < G.n; c += T.nt)
+        W.w()[c] += hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
     return K;
 }
 
-__device__ inline void solve3(const Cap& K, const double* r, double* y) {
-    // Gaussian elimination with partial pivoting on a 3x3 copy
-    double a[3][4];
-    for (int i = 0; i < 3; ++i) { for (int j = 0; j < 3; ++j) a[i][j] = K.m[i][j]; a[i][3] = r[i]; }
-    for (int col = 0; col < 3; ++col) {
-        int p = col;
-        for (int i = col + 1; i < 3; ++i) if (fabs(a[i][col]) > fabs(a[p][col])) p = i;
-        if (p != col) for (int j = 0; j < 4; ++j) { double t = a[col][j]; a[col][j] = a[p][j]; a[p][j] = t; }
-        for (int i = col + 1; i < 3; ++i) {
-            double f = a[i][col] / a[col][col];
-            for (int j = col; j < 4; ++j) a[i][j] -= f * a[col][j];
-        }
-    }
-    for (int i = 2; i >= 0; --i) {
-        double s = a[i][3];
-        for (int j = i + 1; j < 3; ++j) s -= a[i][j] * y[j];
-        y[i] = s / a[i][i];
-    }
-}
-
-// solve (I - hγA) z = b given chain factors and capacitance; b in place -> z
-__device__ inline void border_finish(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const Cap& K,
-                                     double hg, double* b, int lane) {
-    double r[3], y[3];
-    functionals(C, G, X, W, b, lane, r[0], r[1], r[2]);
+// b = (I - hγA)⁻¹ b with stored chain factors, Y columns and capacitance K
+__device__ inline void border_solve(const CosmoDev& C, const Geo& G, const XC& X, const WS& W,
+                                          const Cap& K, double hg, double* b, const Thr& T) {
+    Fun F = functionals(C, G, X, W, b, T);
+    double r[3] = {F.S, F.P, F.Pi}, y[3];
     solve3(K, r, y);
     __syncthreads();
-    for (int c = lane; c < G.n; c += WAVE)
-        b[c] += hg * (W.cS[c] * y[0] + W.cP[c] * y[1] + W.cPi[c] * y[2]);
+    for (int c = T.tid; c < G.n; c += T.nt)
+        b[c] += hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
 }
 
-// source_function (perturbations.jl:343-383) on chain-ordered u (mutated) and du
+// source_function (perturbations.jl:343-383) from the chain-ordered (mutated)
+// u, the derivative rows du of Θ₁..₃, Θᵖ₀..₃, 𝒩₂, v_b, ℳ₂,ᵢ, and Φ′
 __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double k, const double* u,
-                            const double* du, int lane) {
+                            const double* du, double Phip, int lane) {
     double x = X.x;
     double Hc = X.Hc, Hcp = spl(C, BOLT_T_HC_P, x), Hcpp = spl(C, BOLT_T_HC_PP, x);
     double tau = spl(C, BOLT_T_TAU, x), tp = X.tp, tpp = X.tpp;
@@ -417,8 +679,8 @@ __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const 
     double s2 = 0.0, s2p = 0.0;
     if (lane < G.nq) {
         int b = G.cM + lane * (G.nm + 1) + 2;
-        s2 = W.sw[lane] * u[b];
-        s2p = W.sw[lane] * du[b];
+        s2 = W.sw()[lane] * u[b];
+        s2p = W.sw()[lane] * du[b];
     }
     double sig = wave_sum(s2), sigp = wave_sum(s2p);
     const double* T = u + 2;
@@ -426,7 +688,7 @@ __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const 
     const double* dT = du + 2;
     const double* dP = du + G.cB;
     double N2 = u[G.cC + 2], dN2 = du[G.cC + 2];
-    double Phip = du[G.cE], vb = u[1], vbp = du[1];
+    double vb = u[1], vbp = du[1];
     double pre = 12.0 * H02 / (k * k) / (a * a);
     double Psi = -u[G.cE] - pre * (C.Or * T[2] + C.Onu * N2 + sig / C.rho_crit / 4.0);
     double Psip = -Phip - pre * (C.Or * (dT[2] - 2.0 * T[2]) + C.Onu * (dN2 - 2.0 * N2) +
@@ -442,9 +704,21 @@ __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const 
     return term1 + term2 + term3;
 }
 
-// initial_conditions (perturbations.jl:274-338), chain order; lane 0 does the
-// photon/ν recursions, lanes 1..nq the ℳ chains
-__device__ void initial_conditions(const CosmoDev& C, const Geo& G, const WS& W, double k, double* u, int lane) {
+// derivative rows the source function reads (one per lane)
+__device__ inline int source_row(const Geo& G, int lane) {
+    if (lane < 3) return 3 + lane;                 // Θ₁, Θ₂, Θ₃
+    if (lane < 7) return G.cB + (lane - 3);        // Θᵖ₀..₃
+    if (lane == 7) return G.cC + 2;                // 𝒩₂
+    if (lane == 8) return 1;                       // v_b
+    if (lane < 9 + G.nq) return G.cM + (lane - 9) * (G.nm + 1) + 2;  // ℳ₂,ᵢ
+    return -1;
+}
+
+// initial_conditions (perturbations.jl:274-338), chain order; thread 0 does the
+// photon/ν recursions, threads 1..nq the ℳ chains
+__device__ void initial_conditions(const CosmoDev& C, const Geo& G, const WS& W, double k, double* u,
+                                   const Thr& T) {
+    const int lane = T.tid;
     double xi = C.x0;
     double Hc = spl(C, BOLT_T_HC, xi), eta = spl(C, BOLT_T_ETA, xi), tp = spl(C, BOLT_T_TAU_P, xi);
     double ai = sqrt(exp(xi) * exp(xi));
@@ -481,9 +755,9 @@ __device__ void initial_conditions(const CosmoDev& C, const Geo& G, const WS& W,
         u[G.cE] = Phi;
     } else if (lane <= G.nq) {
         int i = lane - 1;
-        double q = W.q[i];
+        double q = W.q()[i];
         double eps = sqrt(q * q + (ai * C.mnu) * (ai * C.mnu));
-        double df0 = W.df0[i];
+        double df0 = W.df0()[i];
         double* M = u + G.cM + i * (G.nm + 1);
         M[0] = -T0 * df0;
         M[1] = -eps / q * T1 * df0;
@@ -522,37 +796,44 @@ struct SolveArgs {
     int n_state_stride;    // n of the output layout (mode 0)
 };
 
-// evaluate f at X on v (chain order) into out; returns Φ′
-__device__ inline double eval_rhs(const CosmoDev& C, const Geo& G, const XC& X, WS& W, const double* v,
-                                  double* out, int lane) {
-    double S, P, Pi;
-    functionals(C, G, X, W, v, lane, S, P, Pi);
-    apply_rhs(G, W, v, S, P, Pi, 1.0, out, lane);
-    return P;
+
+// Σ over the workgroup, the same value in every thread: wavefront sums, then
+// the wavefront totals in a fixed order
+__device__ inline double block_sum(double v, const WS& W, const Thr& T) {
+    double s = wave_sum(v);
+    if (T.nt == WAVE) return s;
+    if (T.lane == 0) W.red()[T.wv] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < T.nt / WAVE; ++w) tot += W.red()[w];
+    __syncthreads();
+    return tot;
 }
 
-__device__ inline double wrms(const Geo& G, const double* e, const double* ua, const double* ub, double rt,
-                              double at, int lane) {
+__device__ inline double wrms(const Geo& G, const WS& W, const double* e, const double* ua, const double* ub,
+                              double rt, double at, const Thr& T) {
     double s = 0.0;
-    for (int c = lane; c < G.n; c += WAVE) {
+    for (int c = T.tid; c < G.n; c += T.nt) {
         double sc = at + rt * fmax(fabs(ua[c]), fabs(ub[c]));
         double r = e[c] / sc;
         s += r * r;
     }
-    s = wave_sum(s);
+    s = block_sum(s, W, T);
     return sqrt(s / G.n);
 }
 
+__device__ inline double node_x(const CosmoDev& C, int j) { return (C.x0 * (C.nx - 1 - j) + C.x1 * j) / (C.nx - 1); }
+
 // output at the node/save point xv from the interval (xa, ua, fa) .. (xb, ub, fb)
-__device__ void emit(const CosmoDev& C, const Geo& G, WS& W, const SolveArgs& A, int out, double k, double xa,
-                     const double* ua, const double* fa, double xb, const double* ub, const double* fb, double xv,
-                     int idx, int lane) {
+__device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const SolveArgs& A, int out,
+                                  double k, double xa, const double* ua, const double* fa, double xb,
+                                  const double* ub, const double* fb, double xv, int idx, const Thr& T) {
     double h = xb - xa;
     if (h == 0.0) {
-        for (int c = lane; c < G.n; c += WAVE) W.arg[c] = ua[c];
+        for (int c = T.tid; c < G.n; c += T.nt) W.arg()[c] = ua[c];
     } else {
         double th = (xv - xa) / h;
-        for (int c = lane; c < G.n; c += WAVE) {
+        for (int c = T.tid; c < G.n; c += T.nt) {
             double v;
             if (th == 1.0) v = ub[c];
             else {
@@ -560,38 +841,46 @@ __device__ void emit(const CosmoDev& C, const Geo& G, WS& W, const SolveArgs& A,
                 v = (1.0 - th) * ua[c] + th * ub[c] +
                     th * (th - 1.0) * ((1.0 - 2.0 * th) * dd + (th - 1.0) * h * fa[c] + th * h * fb[c]);
             }
-            W.arg[c] = v;
+            W.arg()[c] = v;
         }
     }
     if (A.out_mode == 0) {
         double* dst = A.out + ((size_t)out * A.nsave + idx) * A.n_state_stride;
-        for (int c = lane; c < G.n; c += WAVE) dst[chain_to_state(G, c)] = W.arg[c];
+        for (int c = T.tid; c < G.n; c += T.nt) dst[chain_to_state(G, c)] = W.arg()[c];
         return;
     }
-    // source at a grid node: du = hierarchy!(u) (mutating u), then source_function
+    // source at a grid node: hierarchy!(du, u) (mutating u, spectra.jl:16) on the
+    // rows source_function reads, then source_function
     XC X = make_xc(C, k, xv);
-    qfill(C, G, X, W, lane);
+    qfill(C, G, X, W, T);
     __syncthreads();
-    assemble(C, G, X, W, lane);
+    Fun F = functionals(C, G, X, W, W.arg(), T);
+    if (T.wv == 0) {
+        int c = source_row(G, T.lane);
+        if (c >= 0) {
+            RowC r = row_coef(G, X, W, W.desc()[c], c);
+            W.w()[c] = row_apply(r, W.arg(), c, G.n, F.S, F.P, F.Pi);
+        }
+    }
     __syncthreads();
-    double P = eval_rhs(C, G, X, W, W.arg, W.w, lane);
+    if (X.rsa && T.tid == 0) rsa_mutate(G, X, k, W.arg(), F.P);
     __syncthreads();
-    if (X.rsa && lane == 0) rsa_mutate(G, X, k, W.arg, P);
-    __syncthreads();
-    double s = source_fn(C, G, X, W, k, W.arg, W.w, lane);
-    if (lane == 0) A.out[(size_t)out * C.nx + idx] = s;
+    if (T.wv == 0) {
+        double s = source_fn(C, G, X, W, k, W.arg(), W.w(), F.P, T.lane);
+        if (T.lane == 0) A.out[(size_t)out * C.nx + idx] = s;
+    }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
+// one workgroup of NWAVE wavefronts per k-mode
+__global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
     __shared__ int s_mode;
+    const Thr T = thr_self();
     for (;;) {
-        if (lane == 0) s_mode = atomicAdd(A.work_counter, 1);
+        if (T.tid == 0) s_mode = atomicAdd(A.work_counter, 1);
         __syncthreads();
-        int mi = s_mode;
-        __syncthreads();
+        const int mi = s_mode;
         if (mi >= A.nmodes) return;
         ModeDesc md = A.modes[mi];
         const CosmoDev C = A.cosmos[md.cosmo];
@@ -600,10 +889,11 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
         WS W;
         ws_bind(W, lds, G.n, G.nq);
         const double k = md.k;
-        if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
-        for (int c = lane; c < G.n; c += WAVE) W.u[c] = 0.0;
+        if (T.tid < G.nq) { W.q()[T.tid] = C.qv[T.tid]; W.df0()[T.tid] = C.df0[T.tid]; W.base()[T.tid] = C.base[T.tid]; }
+        for (int c = T.tid; c < G.n; c += T.nt) W.u()[c] = 0.0;
+        setup_rows(G, W, T);
         __syncthreads();
-        initial_conditions(C, G, W, k, W.u, lane);
+        initial_conditions(C, G, W, k, W.u(), T);
         __syncthreads();
         long long nacc = 0, nrej = 0, nrhs = 0, nsol = 0;
         double x = C.x0, xend = C.x1;
@@ -612,44 +902,39 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
         int next = 0;  // next output index
         const int nout = (A.out_mode == 0) ? A.nsave : C.nx;
         // outputs at the starting point take the initial condition as saved
-        {
-            while (next < nout) {
-                double xv = (A.out_mode == 0) ? A.x_save[next]
-                                              : (C.x0 * (C.nx - 1 - next) + C.x1 * next) / (C.nx - 1);
-                if (!(xv <= x)) break;
-                emit(C, G, W, A, md.out, k, x, W.u, W.u, x, W.u, W.u, xv, next, lane);
-                ++next;
-            }
+        while (next < nout) {
+            double xv = (A.out_mode == 0) ? A.x_save[next] : node_x(C, next);
+            if (!(xv <= x)) break;
+            emit(C, G, W, A, md.out, k, x, W.u(), W.u(), x, W.u(), W.u(), xv, next, T);
+            ++next;
         }
         // FSAL start: f(u0) (the RSA overwrite, if on, persists in u)
-        XC X = make_xc(C, k, x);
-        qfill(C, G, X, W, lane);
-        __syncthreads();
-        assemble(C, G, X, W, lane);
-        __syncthreads();
-        double P0 = eval_rhs(C, G, X, W, W.u, W.f, lane);
-        ++nrhs;
-        __syncthreads();
-        if (X.rsa && lane == 0) rsa_mutate(G, X, k, W.u, P0);
-        __syncthreads();
+        {
+            XC X = make_xc(C, k, x);
+            qfill(C, G, X, W, T);
+            __syncthreads();
+            Fun F = rhs_rows(C, G, X, W, W.u(), 1.0, W.f(), 0.0, false, T);
+            ++nrhs;
+            __syncthreads();
+            if (X.rsa && T.tid == 0) rsa_mutate(G, X, k, W.u(), F.P);
+            __syncthreads();
+        }
         // Hairer's starting step (order 4)
         double h;
         {
-            double d0 = wrms(G, W.u, W.u, W.u, rt, at, lane);
-            double d1 = wrms(G, W.f, W.u, W.u, rt, at, lane);
+            double d0 = wrms(G, W, W.u(), W.u(), W.u(), rt, at, T);
+            double d1 = wrms(G, W, W.f(), W.u(), W.u(), rt, at, T);
             double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
             if (h0 > xend - x) h0 = xend - x;
-            for (int c = lane; c < G.n; c += WAVE) W.arg[c] = W.u[c] + h0 * W.f[c];
+            for (int c = T.tid; c < G.n; c += T.nt) W.arg()[c] = W.u()[c] + h0 * W.f()[c];
             XC X1 = make_xc(C, k, x + h0);
-            qfill(C, G, X1, W, lane);
+            qfill(C, G, X1, W, T);
             __syncthreads();
-            assemble(C, G, X1, W, lane);
-            __syncthreads();
-            eval_rhs(C, G, X1, W, W.arg, W.w, lane);
+            rhs_rows(C, G, X1, W, W.arg(), 1.0, W.w(), 0.0, false, T);
             ++nrhs;
             __syncthreads();
-            for (int c = lane; c < G.n; c += WAVE) W.w[c] -= W.f[c];
-            double d2 = wrms(G, W.w, W.u, W.u, rt, at, lane) / h0;
+            for (int c = T.tid; c < G.n; c += T.nt) W.w()[c] -= W.f()[c];
+            double d2 = wrms(G, W, W.w(), W.u(), W.u(), rt, at, T) / h0;
             double dm = fmax(d1, d2);
             double h1 = (dm <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / dm, 1.0 / 5.0);
             h = fmin(100.0 * h0, h1);
@@ -662,55 +947,50 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
             bool last = false;
             if (h >= xend - x) { h = xend - x; last = true; }
             const double hg = h * KC_G;
-            for (int c = lane; c < G.n; c += WAVE) W.Z[0][c] = h * W.f[c];
+            for (int c = T.tid; c < G.n; c += T.nt) W.Z(0)[c] = h * W.f()[c];
             Cap K;
             XC Xs;
             for (int s = 1; s < 6; ++s) {
-                for (int c = lane; c < G.n; c += WAVE) {
-                    double acc = W.u[c];
-                    for (int j = 0; j < s; ++j) acc += KC_A[s][j] * W.Z[j][c];
-                    W.arg[c] = acc;
+                for (int c = T.tid; c < G.n; c += T.nt) {
+                    double acc = W.u()[c];
+                    for (int j = 0; j < s; ++j) acc += KC_A[s][j] * W.Z(j)[c];
+                    W.arg()[c] = acc;
                 }
                 Xs = make_xc(C, k, x + KC_C[s] * h);
-                qfill(C, G, Xs, W, lane);
+                qfill(C, G, Xs, W, T);
                 __syncthreads();
-                assemble(C, G, Xs, W, lane);
-                __syncthreads();
-                {
-                    double S, P, Pi;
-                    functionals(C, G, Xs, W, W.arg, lane, S, P, Pi);
-                    apply_rhs(G, W, W.arg, S, P, Pi, h, W.w, lane);
-                }
+                // rhs = h f(ψ) and W = I - hγL, border columns
+                rhs_rows(C, G, Xs, W, W.arg(), h, W.w(), hg, true, T);
                 ++nrhs;
-                to_w(G, W, hg, lane);
                 __syncthreads();
-                chain_factor_solve4(G, W, lane);
+                chain_factor(G, W, T);
+                __syncthreads();
+                chain_solve4(G, W, T);
                 ++nsol;
                 __syncthreads();
-                K = build_cap(C, G, Xs, W, hg, lane);
-                border_finish(C, G, Xs, W, K, hg, W.w, lane);
-                for (int c = lane; c < G.n; c += WAVE) W.Z[s][c] = W.w[c];
+                K = stage_border(C, G, Xs, W, hg, T);
+                for (int c = T.tid; c < G.n; c += T.nt) W.Z(s)[c] = W.w()[c];
                 __syncthreads();
             }
             // u_{n+1} (stiffly accurate: last stage) and the embedded error
-            for (int c = lane; c < G.n; c += WAVE) {
-                double acc = W.u[c], e = 0.0;
+            for (int c = T.tid; c < G.n; c += T.nt) {
+                double acc = W.u()[c], e = 0.0;
                 for (int j = 0; j < 5; ++j) {
-                    acc += KC_A[5][j] * W.Z[j][c];
-                    e += (KC_A[5][j] - KC_BH[j]) * W.Z[j][c];
+                    acc += KC_A[5][j] * W.Z(j)[c];
+                    e += (KC_A[5][j] - KC_BH[j]) * W.Z(j)[c];
                 }
-                acc += KC_G * W.Z[5][c];
-                e += (KC_G - KC_BH[5]) * W.Z[5][c];
-                W.un[c] = acc;
-                W.w[c] = e;
+                acc += KC_G * W.Z(5)[c];
+                e += (KC_G - KC_BH[5]) * W.Z(5)[c];
+                W.un()[c] = acc;
+                W.w()[c] = e;
             }
             __syncthreads();
             // Shampine's smoothed estimate: W⁻¹ e with the last stage's W
-            chain_solve1(G, W, W.w, lane);
+            chain_solve1(G, W, W.w(), T);
             ++nsol;
             __syncthreads();
-            border_finish(C, G, Xs, W, K, hg, W.w, lane);
-            double en = wrms(G, W.w, W.u, W.un, rt, at, lane);
+            border_solve(C, G, Xs, W, K, hg, W.w(), T);
+            double en = wrms(G, W, W.w(), W.u(), W.un(), rt, at, T);
             __syncthreads();
             if (!isfinite(en)) {
                 ++nrej;
@@ -722,24 +1002,21 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
                 double xn = last ? xend : x + h;
                 // FSAL at x_{n+1}
                 XC Xn = make_xc(C, k, xn);
-                qfill(C, G, Xn, W, lane);
+                qfill(C, G, Xn, W, T);
                 __syncthreads();
-                assemble(C, G, Xn, W, lane);
-                __syncthreads();
-                double Pn = eval_rhs(C, G, Xn, W, W.un, W.fn, lane);
+                Fun Fn = rhs_rows(C, G, Xn, W, W.un(), 1.0, W.fn(), 0.0, false, T);
                 ++nrhs;
                 __syncthreads();
-                if (Xn.rsa && lane == 0) rsa_mutate(G, Xn, k, W.un, Pn);
+                if (Xn.rsa && T.tid == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
                 __syncthreads();
                 ++nacc;
                 while (next < nout) {
-                    double xv = (A.out_mode == 0) ? A.x_save[next]
-                                                  : (C.x0 * (C.nx - 1 - next) + C.x1 * next) / (C.nx - 1);
+                    double xv = (A.out_mode == 0) ? A.x_save[next] : node_x(C, next);
                     if (!(xv <= xn)) break;
-                    emit(C, G, W, A, md.out, k, x, W.u, W.f, xn, W.un, W.fn, xv, next, lane);
+                    emit(C, G, W, A, md.out, k, x, W.u(), W.f(), xn, W.un(), W.fn(), xv, next, T);
                     ++next;
                 }
-                for (int c = lane; c < G.n; c += WAVE) { W.u[c] = W.un[c]; W.f[c] = W.fn[c]; }
+                for (int c = T.tid; c < G.n; c += T.nt) { W.u()[c] = W.un()[c]; W.f()[c] = W.fn()[c]; }
                 __syncthreads();
                 x = xn;
                 double fac = (en == 0.0) ? 10.0 : 0.9 * pow(en, -0.25);
@@ -752,7 +1029,7 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
                 if (h < 1e-14 * fmax(1.0, fabs(x))) { status = BOLT_MODE_DTMIN; break; }
             }
         }
-        if (lane == 0) {
+        if (T.tid == 0) {
             A.status[md.out] = status;
             if (A.counters) {
                 long long* cc = A.counters + (size_t)md.out * 4;
@@ -767,7 +1044,7 @@ __global__ void __launch_bounds__(WAVE) k_boltsolve(SolveArgs A) {
 __global__ void __launch_bounds__(WAVE) k_rhs(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes,
                                                int B, double x, double* u, double* du, int mutate) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
+    const Thr T = thr_self();
     const int mi = blockIdx.x;
     if (mi >= B) return;
     ModeDesc md = modes[mi];
@@ -775,28 +1052,27 @@ __global__ void __launch_bounds__(WAVE) k_rhs(const CosmoDev* cosmos, const CfgD
     Geo G = make_geo(cfgs[md.cfg]);
     WS W;
     ws_bind(W, lds, G.n, G.nq);
-    if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
-    for (int c = lane; c < G.n; c += WAVE) W.arg[c] = u[(size_t)chain_to_state(G, c) * B + mi];
+    if (T.tid < G.nq) { W.q()[T.tid] = C.qv[T.tid]; W.df0()[T.tid] = C.df0[T.tid]; W.base()[T.tid] = C.base[T.tid]; }
+    for (int c = T.tid; c < G.n; c += T.nt) W.arg()[c] = u[(size_t)chain_to_state(G, c) * B + mi];
+    setup_rows(G, W, T);
     XC X = make_xc(C, md.k, x);
-    qfill(C, G, X, W, lane);
+    qfill(C, G, X, W, T);
     __syncthreads();
-    assemble(C, G, X, W, lane);
+    Fun F = rhs_rows(C, G, X, W, W.arg(), 1.0, W.w(), 0.0, false, T);
     __syncthreads();
-    double P = eval_rhs(C, G, X, W, W.arg, W.w, lane);
+    if (mutate && X.rsa && T.tid == 0) rsa_mutate(G, X, md.k, W.arg(), F.P);
     __syncthreads();
-    if (mutate && X.rsa && lane == 0) rsa_mutate(G, X, md.k, W.arg, P);
-    __syncthreads();
-    for (int c = lane; c < G.n; c += WAVE) {
+    for (int c = T.tid; c < G.n; c += T.nt) {
         size_t o = (size_t)chain_to_state(G, c) * B + mi;
-        du[o] = W.w[c];
-        if (mutate) u[o] = W.arg[c];
+        du[o] = W.w()[c];
+        if (mutate) u[o] = W.arg()[c];
     }
 }
 
 __global__ void __launch_bounds__(WAVE) k_ics(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes,
                                                int B, double* u0) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
+    const Thr T = thr_self();
     const int mi = blockIdx.x;
     if (mi >= B) return;
     ModeDesc md = modes[mi];
@@ -804,12 +1080,12 @@ __global__ void __launch_bounds__(WAVE) k_ics(const CosmoDev* cosmos, const CfgD
     Geo G = make_geo(cfgs[md.cfg]);
     WS W;
     ws_bind(W, lds, G.n, G.nq);
-    if (lane < G.nq) { W.q[lane] = C.qv[lane]; W.df0[lane] = C.df0[lane]; W.base[lane] = C.base[lane]; }
-    for (int c = lane; c < G.n; c += WAVE) W.u[c] = 0.0;
+    if (T.tid < G.nq) { W.q()[T.tid] = C.qv[T.tid]; W.df0()[T.tid] = C.df0[T.tid]; W.base()[T.tid] = C.base[T.tid]; }
+    for (int c = T.tid; c < G.n; c += T.nt) W.u()[c] = 0.0;
     __syncthreads();
-    initial_conditions(C, G, W, md.k, W.u, lane);
+    initial_conditions(C, G, W, md.k, W.u(), T);
     __syncthreads();
-    for (int c = lane; c < G.n; c += WAVE) u0[(size_t)chain_to_state(G, c) * B + mi] = W.u[c];
+    for (int c = T.tid; c < G.n; c += T.nt) u0[(size_t)chain_to_state(G, c) * B + mi] = W.u()[c];
 }
 
 // rsa_perts! after the switch node (perturbations.jl:35-111); one thread per (mode, node)
@@ -1125,8 +1401,10 @@ struct bolt_ctx {
 
 static int check_cfg(const bolt_hier_cfg* c) {
     if (!c) return set_error(BOLT_E_ARG, "cfg is NULL");
-    if (c->ell_g < 3 || c->ell_nu < 2 || c->ell_mnu < 2 || c->nq < 1 || c->nq > 64)
-        return set_error(BOLT_E_ARG, "hierarchy sizes need ell_g >= 3, ell_nu >= 2, ell_mnu >= 2, 1 <= nq <= 64");
+    if (c->ell_g < 3 || c->ell_nu < 2 || c->ell_mnu < 2 || c->nq < 1 || c->nq > 48)
+        return set_error(BOLT_E_ARG, "hierarchy sizes need ell_g >= 3, ell_nu >= 2, ell_mnu >= 2, 1 <= nq <= 48");
+    if (c->ell_g >= LTAB || c->ell_nu >= LTAB || c->ell_mnu >= LTAB)
+        return set_error(BOLT_E_ARG, "hierarchy cut-offs must stay below 512");
     if (!(c->reltol > 0) || !(c->abstol > 0)) return set_error(BOLT_E_ARG, "reltol and abstol must be positive");
     return BOLT_OK;
 }
@@ -1318,14 +1596,14 @@ static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, 
     size_t lds = ws_bytes(cd.n, cd.nq);
     if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
     int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, WAVE, lds));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, NTHR, lds));
     if (per_cu < 1) per_cu = 1;
     int grid = std::min(B, per_cu * ctx->n_cu);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(WAVE), lds, ctx->stream, A);
+    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(NTHR), lds, ctx->stream, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipEventSynchronize(e1));
