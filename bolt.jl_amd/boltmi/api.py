@@ -339,6 +339,34 @@ def source_grid(par, bg, ih, k_grid, integrator=None, â„“áµ§=8, reltol=1e-11, â„
     return SourceGrid(bg.x_grid, ks, S, _stats_dict(st))
 
 
+def source_grid_P(par, bg, ih, k_grid, integrator=None, â„“áµ§=8, reltol=1e-11, â„“_Î½=8, â„“_mÎ½=10, nq=15, abstol=1e-6,
+                  ctx=None):
+    """source_grid_P(par, bg, ih, k_grid, BasicNewtonian(); â„“áµ§=8, reltol=1e-11)
+    (spectra.jl:25-42): the polarization source (perturbations.jl:385-404) on
+    every x node; Â±Inf at the last node, where y = k(Î·(0) âˆ’ Î·(x)) = 0."""
+    return source_grids_TP(par, bg, ih, k_grid, integrator, â„“áµ§, reltol, â„“_Î½, â„“_mÎ½, nq, abstol, ctx,
+                           want_t=False)[1]
+
+
+def source_grids_TP(par, bg, ih, k_grid, integrator=None, â„“áµ§=8, reltol=1e-11, â„“_Î½=8, â„“_mÎ½=10, nq=15, abstol=1e-6,
+                    ctx=None, want_t=True, want_p=True):
+    """(source_grid(...), source_grid_P(...)) from one solve per k-mode."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(k_grid, dtype=np.float64)
+    cfg = hier_cfg(â„“áµ§, â„“_Î½, â„“_mÎ½, nq, reltol, abstol)
+    S = np.zeros((len(ks), bg.nx)) if want_t else None
+    SP = np.zeros((len(ks), bg.nx)) if want_p else None
+    status = np.zeros(len(ks), dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        check(lib().bolt_source_grid_tp(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks),
+                                        pd(S) if want_t else None, pd(SP) if want_p else None, pi(status),
+                                        ctypes.byref(st)))
+    stats = _stats_dict(st)
+    return (SourceGrid(bg.x_grid, ks, S, stats) if want_t else None,
+            SourceGrid(bg.x_grid, ks, SP, stats) if want_p else None)
+
+
 def quadratic_k(kmin, kmax, nk):
     """quadratic_k (spectra.jl:60-63): k_i = kmin + (kmax-kmin)(i/nk)Â², i = 1..nk."""
     return np.array([kmin + (kmax - kmin) * (i / nk) ** 2 for i in range(1, nk + 1)])
@@ -369,6 +397,54 @@ def cltt(ells, par, bg, ih, sf, ctx=None):
     grid is quadratic_k(0.01Hâ‚€, 1000Hâ‚€, 5000)."""
     dense = quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
     return cltt_on_grid(ells, sf, dense, par, bg, 0, ctx)
+
+
+def cl_on_grid(ells, sf_t, sf_p, kgrid, par, bg, bessel_mode=0, ctx=None):
+    """(C_â„“^TT, C_â„“^TE, C_â„“^EE) from one LOS pass (spectra.jl:84-130); sf_t or
+    sf_p may be None, and then the spectra that need it come back as None."""
+    ctx = ctx or default_context()
+    ells_a = np.ascontiguousarray(np.atleast_1d(ells), dtype=np.int32)
+    src = sf_t if sf_t is not None else sf_p
+    ksrc = np.ascontiguousarray(src.k_grid, dtype=np.float64)
+    S = None if sf_t is None else np.ascontiguousarray(sf_t.grid, dtype=np.float64)
+    SP = None if sf_p is None else np.ascontiguousarray(sf_p.grid, dtype=np.float64)
+    if sf_t is not None and sf_p is not None and not np.array_equal(sf_t.k_grid, sf_p.k_grid):
+        raise ValueError("the temperature and polarization grids must share k_grid")
+    kg = np.ascontiguousarray(kgrid, dtype=np.float64)
+    n = len(ells_a)
+    tt = np.zeros(n) if S is not None else None
+    te = np.zeros(n) if (S is not None and SP is not None) else None
+    ee = np.zeros(n) if SP is not None else None
+    with ctx.lock:
+        check(lib().bolt_cl(ctx.handle, bg.cosmo_id(ctx), None if S is None else pd(S), None if SP is None else pd(SP),
+                            pd(ksrc), len(ksrc), pd(kg), len(kg), pi(ells_a), n, bessel_mode,
+                            None if tt is None else pd(tt), None if te is None else pd(te),
+                            None if ee is None else pd(ee)))
+    if np.ndim(ells):
+        return tt, te, ee
+    return tuple(None if v is None else float(v[0]) for v in (tt, te, ee))
+
+
+def clte_on_grid(ells, sf_t, sf_e, kgrid, par, bg, ctx=None):
+    """clte(â„“, s_itp_t, s_itp_e, kgrid, par, bg) (spectra.jl:98-114)."""
+    return cl_on_grid(ells, sf_t, sf_e, kgrid, par, bg, 0, ctx)[1]
+
+
+def clee_on_grid(ells, sf_p, kgrid, par, bg, ctx=None):
+    """clee(â„“, s_itp_p, kgrid, par, bg) (spectra.jl:116-130)."""
+    return cl_on_grid(ells, None, sf_p, kgrid, par, bg, 0, ctx)[2]
+
+
+def clte(ells, par, bg, ih, sf, sf_P, ctx=None):
+    """clte(â„“, par, bg, ih, sf, sf_P) and clte(â„“âƒ—, ...) (spectra.jl:137-140, 152-155)."""
+    dense = quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
+    return clte_on_grid(ells, sf, sf_P, dense, par, bg, ctx)
+
+
+def clee(ells, par, bg, ih, sf_P, ctx=None):
+    """clee(â„“, par, bg, ih, sf_P) and clee(â„“âƒ—, ...) (spectra.jl:142-145, 157-160)."""
+    dense = quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
+    return clee_on_grid(ells, sf_P, dense, par, bg, ctx)
 
 
 def plin(k, par, bg, ih, n_q=15, â„“áµ§=50, â„“_Î½=50, â„“_mÎ½=20, x=0.0, reltol=1e-5, abstol=1e-6, ctx=None):

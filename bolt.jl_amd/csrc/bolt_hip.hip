@@ -123,6 +123,8 @@ __device__ inline void chain_range(const Geo& g, int ch, int& c0, int& m) {
     else { c0 = g.cE; m = 1; }
 }
 
+__device__ inline double node_x(const CosmoDev& C, int j) { return (C.x0 * (C.nx - 1 - j) + C.x1 * j) / (C.nx - 1); }
+
 // uniform cubic B-spline (src/util.jl:11-13)
 __device__ inline double spl(const CosmoDev& C, int tab, double x, int deriv = 0) {
     double t = (x - C.x0) / C.dx;
@@ -492,8 +494,13 @@ __device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v
 // dependency is the pivot recurrence.  Rows are processed in blocks of CB: a
 // block's inputs are loaded into registers before its sequential sweep.
 // Stored: d <- 1/pivot, l1/l2 <- multipliers, u1 <- updated upper entry (u2 is
-// never modified by the elimination).  Reads past the chain end stay inside the
-// LDS workspace and are never stored back.
+// never modified by the elimination).  Reads past the segment end stay inside
+// the LDS workspace and are never stored back.
+//
+// The second band is nonzero on two rows only, both in the first block of the
+// first segment: u2 on v_b (its Θ₁ coupling) and l2 on Θ₁ (its v_b coupling).
+// Every later block is therefore tridiagonal, and there u1 is left unchanged
+// (u1 -= m1·u2 with u2 = 0), which halves the work of those rows.
 #define CB 8
 __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
     if (T.tid >= W.misc()[0]) return;
@@ -501,12 +508,10 @@ __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
     double* l2 = W.l2() + c0; double* l1 = W.l1() + c0; double* d = W.d() + c0;
     double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
     double ip1 = 0.0, ip2 = 0.0, u1p1 = 0.0, u1p2 = 0.0, u2p1 = 0.0, u2p2 = 0.0;
-    for (int jb = 0; jb < m; jb += CB) {
+    {   // first block: the full bandwidth-2 recurrence
         double L2[CB], L1[CB], D[CB], U1[CB], U2[CB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            L2[i] = l2[jb + i]; L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; U2[i] = u2[jb + i];
-        }
+        for (int i = 0; i < CB; ++i) { L2[i] = l2[i]; L1[i] = l1[i]; D[i] = d[i]; U1[i] = u1[i]; U2[i] = u2[i]; }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
             double m2 = L2[i] * ip2;
@@ -520,52 +525,94 @@ __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
             u2p2 = u2p1; u2p1 = U2[i];
         }
 #pragma unroll
+        for (int i = 0; i < CB; ++i)
+            if (i < m) { l2[i] = L2[i]; l1[i] = L1[i]; d[i] = D[i]; u1[i] = U1[i]; }
+    }
+    for (int jb = CB; jb < m; jb += CB) {  // tridiagonal blocks
+        double L1[CB], D[CB], U1[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) { L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; }
+#pragma unroll
         for (int i = 0; i < CB; ++i) {
-            if (jb + i < m) { l2[jb + i] = L2[i]; l1[jb + i] = L1[i]; d[jb + i] = D[i]; u1[jb + i] = U1[i]; }
+            double m1 = L1[i] * ip1;
+            double inv = rcp(D[i] - m1 * u1p1);
+            L1[i] = m1; D[i] = inv;
+            ip1 = inv;
+            u1p1 = U1[i];
         }
+#pragma unroll
+        for (int i = 0; i < CB; ++i)
+            if (jb + i < m) { l1[jb + i] = L1[i]; d[jb + i] = D[i]; }
     }
 }
 
-// forward + back substitution of one chain for one right-hand side, in blocks
+// forward + back substitution of one segment for one right-hand side, in
+// blocks (the first block with both bands, the others tridiagonal)
 __device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
     const double* l2 = W.l2() + c0; const double* l1 = W.l1() + c0; const double* d = W.d() + c0;
     const double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
     double* b = bvec + c0;
     double y1 = 0.0, y2 = 0.0;
-    for (int jb = 0; jb < m; jb += CB) {
+    {
         double B[CB], L1[CB], L2[CB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; L2[i] = l2[jb + i]; }
+        for (int i = 0; i < CB; ++i) { B[i] = b[i]; L1[i] = l1[i]; L2[i] = l2[i]; }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
-            double t = B[i] - L2[i] * y2;
-            double y = t - L1[i] * y1;
+            double y = (B[i] - L2[i] * y2) - L1[i] * y1;
             B[i] = y;
             y2 = y1; y1 = y;
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) if (i < m) b[i] = B[i];
+    }
+    for (int jb = CB; jb < m; jb += CB) {
+        double B[CB], L1[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            double y = B[i] - L1[i] * y1;
+            B[i] = y;
+            y1 = y;
         }
 #pragma unroll
         for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
     }
     // backward, blocks from the top; rows >= m read as zero contributions
     double x1 = 0.0, x2 = 0.0;
-    int top = ((m + CB - 1) / CB) * CB;
-    for (int jb = top - CB; jb >= 0; jb -= CB) {
-        double B[CB], U1[CB], U2[CB], D[CB];
+    const int top = ((m + CB - 1) / CB) * CB;
+    for (int jb = top - CB; jb >= CB; jb -= CB) {
+        double B[CB], U1[CB], D[CB];
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
             bool in = jb + i < m;
-            B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; U2[i] = in ? u2[jb + i] : 0.0;
-            D[i] = in ? d[jb + i] : 0.0;
+            B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; D[i] = in ? d[jb + i] : 0.0;
         }
 #pragma unroll
         for (int i = CB - 1; i >= 0; --i) {
-            double t = B[i] - U2[i] * x2;
-            double x = (t - U1[i] * x1) * D[i];
+            double x = (B[i] - U1[i] * x1) * D[i];
             B[i] = x;
             x2 = x1; x1 = x;
         }
 #pragma unroll
         for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
+    }
+    {
+        double B[CB], U1[CB], U2[CB], D[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+            bool in = i < m;
+            B[i] = in ? b[i] : 0.0; U1[i] = in ? u1[i] : 0.0; U2[i] = in ? u2[i] : 0.0; D[i] = in ? d[i] : 0.0;
+        }
+#pragma unroll
+        for (int i = CB - 1; i >= 0; --i) {
+            double x = ((B[i] - U2[i] * x2) - U1[i] * x1) * D[i];
+            B[i] = x;
+            x2 = x1; x1 = x;
+        }
+#pragma unroll
+        for (int i = 0; i < CB; ++i) if (i < m) b[i] = B[i];
     }
 }
 
@@ -704,6 +751,15 @@ __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const 
     return term1 + term2 + term3;
 }
 
+// source_function_P (perturbations.jl:385-404): (3/(4y²)) g̃ Π with
+// y = k(η(x_grid[end]) − η(x)); ±Inf at the last node, where y = 0
+__device__ inline double source_fn_P(const CosmoDev& C, const Geo& G, const XC& X, double k, const double* u) {
+    double g = spl(C, BOLT_T_G, X.x);
+    double y = k * (spl(C, BOLT_T_ETA, node_x(C, C.nx - 1)) - X.eta);
+    double Pi = u[4] + u[G.cB + 2] + u[G.cB];
+    return (3.0 / (4.0 * y * y)) * g * Pi;
+}
+
 // derivative rows the source function reads (one per lane)
 __device__ inline int source_row(const Geo& G, int lane) {
     if (lane < 3) return 3 + lane;                 // Θ₁, Θ₂, Θ₃
@@ -787,10 +843,12 @@ struct SolveArgs {
     const ModeDesc* modes;
     int nmodes;
     int* work_counter;
-    int out_mode;          // 0: save states at x_save, 1: source at every x node
+    int out_mode;          // 0: save states at x_save; else bit 1: temperature source, bit 2: polarization
+                           // source, at every x node
     const double* x_save;  // [nsave] sorted
     int nsave;
-    double* out;           // mode 0: [out][nsave][n_state]; mode 1: [out][nx]
+    double* out;           // mode 0: [out][nsave][n_state]; source modes: temperature [out][nx]
+    double* out2;          // polarization source [out][nx]
     int* status;           // [out]
     long long* counters;   // [out][4]: accepted, rejected, rhs evals, solves
     int n_state_stride;    // n of the output layout (mode 0)
@@ -822,7 +880,6 @@ __device__ inline double wrms(const Geo& G, const WS& W, const double* e, const 
     return sqrt(s / G.n);
 }
 
-__device__ inline double node_x(const CosmoDev& C, int j) { return (C.x0 * (C.nx - 1 - j) + C.x1 * j) / (C.nx - 1); }
 
 // output at the node/save point xv from the interval (xa, ua, fa) .. (xb, ub, fb)
 __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const SolveArgs& A, int out,
@@ -849,13 +906,15 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
         for (int c = T.tid; c < G.n; c += T.nt) dst[chain_to_state(G, c)] = W.arg()[c];
         return;
     }
-    // source at a grid node: hierarchy!(du, u) (mutating u, spectra.jl:16) on the
-    // rows source_function reads, then source_function
+    // sources at a grid node: hierarchy!(du, u) (mutating u, spectra.jl:16, 36)
+    // on the rows source_function reads, then source_function and/or
+    // source_function_P
+    const bool want_t = (A.out_mode & 1) != 0, want_p = (A.out_mode & 2) != 0;
     XC X = make_xc(C, k, xv);
     qfill(C, G, X, W, T);
     __syncthreads();
     Fun F = functionals(C, G, X, W, W.arg(), T);
-    if (T.wv == 0) {
+    if (want_t && T.wv == 0) {
         int c = source_row(G, T.lane);
         if (c >= 0) {
             RowC r = row_coef(G, X, W, W.desc()[c], c);
@@ -865,10 +924,11 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
     __syncthreads();
     if (X.rsa && T.tid == 0) rsa_mutate(G, X, k, W.arg(), F.P);
     __syncthreads();
-    if (T.wv == 0) {
+    if (want_t && T.wv == 0) {
         double s = source_fn(C, G, X, W, k, W.arg(), W.w(), F.P, T.lane);
         if (T.lane == 0) A.out[(size_t)out * C.nx + idx] = s;
     }
+    if (want_p && T.tid == T.nt - 1) A.out2[(size_t)out * C.nx + idx] = source_fn_P(C, G, X, k, W.arg());
     __syncthreads();
 }
 
@@ -1171,24 +1231,26 @@ __global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const Mo
     pk[md.out] = (2.0 * M_PI * M_PI / (k * k * k)) * dm * dm * Pprim;
 }
 
-// ---------------------------------------------------------------- LOS (cltt)
+// ---------------------------------------------------------------- LOS (C_ℓ)
 // T_ℓ(k) = Σ_x j_ℓ(k(η₀-η(x))) S(x,k) Δx  (spectra.jl:70-82), all ℓ <= lmax at
 // once by the upward recurrence j_{ℓ+1} = (2ℓ+1)/z j_ℓ - j_{ℓ-1}.  Past the
 // turning point ((2ℓ+1)/z > 2) the slot ends once |j_ℓ| < 1e-7 max|j| or once
 // |j_ℓ| grows (there the true j_ℓ decays monotonically, so growth means the
 // recurrence's growing solution has taken over).
 // One block of four wavefronts per dense-k midpoint; lanes own x nodes (slots
-// of 64 consecutive nodes); ℓ runs in chunks of 32 reduced across lanes by a
+// of 64 consecutive nodes); ℓ runs in chunks reduced across lanes by a
 // butterfly reduce-scatter and across the four wavefronts through LDS.
+// NS = 1: one source (C_ℓ^TT, or C_ℓ^EE with the spin factor); NS = 2: the
+// temperature and polarization sources share each j_ℓ and give TT, TE, EE.
 #define LOS_SLOTS 16   // x slots of 64 nodes (up to 1024 nodes after x = -8)
 #define LOS_WAVES 4
 #define LOS_SPW (LOS_SLOTS / LOS_WAVES)
-#define LOS_CHUNK 32
 #define LOS_TAU 1e-7
 
 struct LosArgs {
     const CosmoDev* cosmo;
     const double* S;      // [nk_src][nx]
+    const double* SP;     // [nk_src][nx] second source (NS = 2)
     const double* k_src;  // [nk_src]
     int nk_src;
     const double* kgrid;  // [nkd]
@@ -1199,7 +1261,8 @@ struct LosArgs {
     const double* etaj;   // [nxi] η at the integration nodes
     const double* dxj;    // [nxi] x_{j+1}-x_j
     int i0, nxi;
-    double* contrib;      // [nell][nkd-1]
+    int spin;             // NS = 1: multiply T_ℓ by ℓð (E modes)
+    double* contrib;      // [nout][nell][nkd-1], nout = 1 (NS = 1) or 3 (TT, TE, EE)
     long long* steps;     // executed recurrence steps (per k), may be NULL
 };
 
@@ -1213,15 +1276,21 @@ __global__ void k_los_prep(const CosmoDev* cosmo, int i0, int nxi, double* etaj,
     dxj[j] = xb - xa;
 }
 
-// advance one x slot through ℓ = L0+I0 .. L0+LOS_CHUNK-1, accumulating j_ℓ s
-template <int I0>
-__device__ inline void los_slot(double& jp, double& jc, double iz, double s, double& M, bool& al, int L0,
-                                double* acc, long long& nstep) {
+// ℓð = √((ℓ+2)(ℓ+1)ℓ(ℓ−1)) (spectra.jl:101, 119), the product exact in 64 bits
+__device__ inline double spin_factor(int l) {
+    long long L = l;
+    return sqrt((double)((L + 2) * (L + 1) * L * (L - 1)));
+}
+
+// advance one x slot through ℓ = L0+I0 .. L0+CH-1, accumulating j_ℓ s (and j_ℓ sp)
+template <int NS, int CH, int I0>
+__device__ inline void los_slot(double& jp, double& jc, double iz, double s, double sp, double& M, bool& al, int L0,
+                                double* acc, double* accp, long long& nstep) {
     double a_jp = jp, a_jc = jc, MM = M;
     bool live = al;
     const double base = (double)(2 * L0 - 1);
 #pragma unroll
-    for (int i = I0; i < LOS_CHUNK; ++i) {
+    for (int i = I0; i < CH; ++i) {
         double t = (base + 2.0 * i) * iz;  // (2ℓ-1)/z produces j_ℓ, ℓ = L0+i
         double jn = t * a_jc - a_jp;
         a_jp = a_jc;
@@ -1234,19 +1303,42 @@ __device__ inline void los_slot(double& jp, double& jc, double iz, double s, dou
         double v = live ? jn : 0.0;
         nstep += live ? 1 : 0;
         acc[i] += v * s;
+        if constexpr (NS == 2) accp[i] += v * sp;
     }
     jp = a_jp; jc = a_jc; M = MM; al = live;
 }
 
+// butterfly reduce-scatter of acc[CH] over the wavefront: lane l returns Σ_lanes acc[l % CH]
+template <int CH>
+__device__ inline double los_reduce(double* acc, int lane) {
+#pragma unroll
+    for (int w = CH / 2; w >= 1; w >>= 1) {
+        bool up = (lane & w) != 0;
+#pragma unroll
+        for (int i = 0; i < w; ++i) {
+            double send = up ? acc[i] : acc[i + w];
+            double keep = up ? acc[i + w] : acc[i];
+            acc[i] = keep + __shfl_xor(send, w, WAVE);
+        }
+    }
+    double T = acc[0];
+#pragma unroll
+    for (int w = CH; w < WAVE; w <<= 1) T += __shfl_xor(T, w, WAVE);
+    return T;
+}
+
+template <int NS>
 __global__ void __launch_bounds__(256) k_los(LosArgs A) {
+    constexpr int CH = 16;  // ℓ per chunk (NS·CH accumulators per lane)
     // one block (4 wavefronts) per dense-k midpoint; wave w owns x slots w, w+4, w+8, w+12
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
     const int ik = blockIdx.x;
     if (ik >= A.nkd - 1) return;
-    __shared__ double part[LOS_WAVES][LOS_CHUNK];
+    __shared__ double part[LOS_WAVES][NS][CH];
     const double eta0 = A.cosmo->eta0;
     const int nx = A.cosmo->nx;
+    const int nkm = A.nkd - 1;
     double k = (A.kgrid[ik] + A.kgrid[ik + 1]) / 2.0;
     double dk = A.kgrid[ik + 1] - A.kgrid[ik];
     // Gridded-linear interpolation in k with Line extrapolation
@@ -1259,8 +1351,10 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
     double tt = (k - A.k_src[ia]) / (A.k_src[ia + 1] - A.k_src[ia]);
     const double* Sa = A.S + (size_t)ia * nx + A.i0;
     const double* Sb = A.S + (size_t)(ia + 1) * nx + A.i0;
+    const double* Pa = NS == 2 ? A.SP + (size_t)ia * nx + A.i0 : nullptr;
+    const double* Pb = NS == 2 ? A.SP + (size_t)(ia + 1) * nx + A.i0 : nullptr;
 
-    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], mx[LOS_SPW];
+    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], spv[LOS_SPW], mx[LOS_SPW];
     bool alive[LOS_SPW];
 #pragma unroll
     for (int sl = 0; sl < LOS_SPW; ++sl) {
@@ -1268,6 +1362,7 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
         if (j < A.nxi) {
             double z = k * (eta0 - A.etaj[j]);
             double s = ((1.0 - tt) * Sa[j] + tt * Sb[j]) * A.dxj[j];
+            double sp = NS == 2 ? ((1.0 - tt) * Pa[j] + tt * Pb[j]) * A.dxj[j] : 0.0;
             double sn, cs;
             sincos(z, &sn, &cs);
             double j0 = sn / z, j1;
@@ -1280,26 +1375,30 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
             jp[sl] = j0; jc[sl] = j1;
             iz[sl] = 1.0 / z;
             sv[sl] = s;
+            spv[sl] = sp;
             mx[sl] = fmax(fabs(j0), fabs(j1));
             alive[sl] = true;
         } else {
-            jp[sl] = jc[sl] = iz[sl] = sv[sl] = mx[sl] = 0.0;
+            jp[sl] = jc[sl] = iz[sl] = sv[sl] = spv[sl] = mx[sl] = 0.0;
             alive[sl] = false;
         }
     }
     long long nstep = 0;
     const double Pw = A.cosmo->Pprim_A * pow(k / 0.05, A.cosmo->Pprim_n - 1.0) * dk / k;
-    for (int L0 = 0; L0 <= A.lmax; L0 += LOS_CHUNK) {
-        double acc[LOS_CHUNK];
+    for (int L0 = 0; L0 <= A.lmax; L0 += CH) {
+        double acc[CH], accp[NS == 2 ? CH : 1];
 #pragma unroll
-        for (int i = 0; i < LOS_CHUNK; ++i) acc[i] = 0.0;
+        for (int i = 0; i < CH; ++i) acc[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < (NS == 2 ? CH : 1); ++i) accp[i] = 0.0;
         bool any_live = false;
         if (L0 == 0) {
 #pragma unroll
             for (int sl = 0; sl < LOS_SPW; ++sl) {
                 acc[0] += jp[sl] * sv[sl];
                 acc[1] += jc[sl] * sv[sl];
-                los_slot<2>(jp[sl], jc[sl], iz[sl], sv[sl], mx[sl], alive[sl], 0, acc, nstep);
+                if constexpr (NS == 2) { accp[0] += jp[sl] * spv[sl]; accp[1] += jc[sl] * spv[sl]; }
+                los_slot<NS, CH, 2>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], 0, acc, accp, nstep);
             }
             any_live = true;
         } else {
@@ -1307,64 +1406,74 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
             for (int sl = 0; sl < LOS_SPW; ++sl) {
                 if (!__any(alive[sl])) continue;
                 any_live = true;
-                los_slot<0>(jp[sl], jc[sl], iz[sl], sv[sl], mx[sl], alive[sl], L0, acc, nstep);
+                los_slot<NS, CH, 0>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], L0, acc, accp, nstep);
             }
         }
-        double T = 0.0;
+        double Tv = 0.0, Ev = 0.0;
         if (any_live) {  // wave-uniform
-            // butterfly reduce-scatter: lane l ends with Σ_lanes acc[l & 31]
-#pragma unroll
-            for (int w = LOS_CHUNK / 2; w >= 1; w >>= 1) {
-                bool up = (lane & w) != 0;
-#pragma unroll
-                for (int i = 0; i < w; ++i) {
-                    double send = up ? acc[i] : acc[i + w];
-                    double keep = up ? acc[i + w] : acc[i];
-                    acc[i] = keep + __shfl_xor(send, w, WAVE);
-                }
-            }
-            T = acc[0] + __shfl_xor(acc[0], 32, WAVE);
+            Tv = los_reduce<CH>(acc, lane);
+            if constexpr (NS == 2) Ev = los_reduce<CH>(accp, lane);
         }
-        if (lane < LOS_CHUNK) part[wv][lane] = T;
+        if (lane < CH) {
+            part[wv][0][lane] = Tv;
+            if (NS == 2) part[wv][NS - 1][lane] = Ev;
+        }
         int live_block = __syncthreads_or(any_live ? 1 : 0);
-        if (wv == 0 && lane < LOS_CHUNK) {
-            double Tt = part[0][lane];
+        if (wv == 0 && lane < CH) {
+            double Tt = part[0][0][lane], Et = NS == 2 ? part[0][NS - 1][lane] : 0.0;
 #pragma unroll
-            for (int w = 1; w < LOS_WAVES; ++w) Tt += part[w][lane];
+            for (int w = 1; w < LOS_WAVES; ++w) {
+                Tt += part[w][0][lane];
+                if (NS == 2) Et += part[w][NS - 1][lane];
+            }
             int l = L0 + lane;
             if (l <= A.lmax) {
                 int e = A.lidx[l];
-                if (e >= 0) A.contrib[(size_t)e * (A.nkd - 1) + ik] = Tt * Tt * Pw;
+                if (e >= 0) {
+                    if (NS == 1) {
+                        double v = A.spin ? Tt * spin_factor(l) : Tt;
+                        A.contrib[(size_t)e * nkm + ik] = v * v * Pw;
+                    } else {
+                        double ep = Et * spin_factor(l);
+                        A.contrib[(size_t)e * nkm + ik] = Tt * Tt * Pw;
+                        A.contrib[((size_t)A.nell + e) * nkm + ik] = Tt * ep * Pw;
+                        A.contrib[((size_t)2 * A.nell + e) * nkm + ik] = ep * ep * Pw;
+                    }
+                }
             }
         }
         __syncthreads();
         if (!live_block) {  // every term is past its cut: the remaining T_ℓ are zero
-            for (int l = L0 + LOS_CHUNK + threadIdx.x; l <= A.lmax; l += blockDim.x) {
+            for (int l = L0 + CH + threadIdx.x; l <= A.lmax; l += blockDim.x) {
                 int e = A.lidx[l];
-                if (e >= 0) A.contrib[(size_t)e * (A.nkd - 1) + ik] = 0.0;
+                if (e < 0) continue;
+                for (int o = 0; o < (NS == 1 ? 1 : 3); ++o) A.contrib[((size_t)o * A.nell + e) * nkm + ik] = 0.0;
             }
             break;
         }
     }
     if (A.steps) {
         double ns = wave_sum((double)nstep);
-        if (lane == 0) part[wv][0] = ns;
+        if (lane == 0) part[wv][0][0] = ns;
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0;
-            for (int w = 0; w < LOS_WAVES; ++w) t += part[w][0];
+            for (int w = 0; w < LOS_WAVES; ++w) t += part[w][0][0];
             A.steps[ik] = (long long)t;
         }
     }
 }
 
-// C_ℓ = 4π Σ_k contrib (fixed order: per-thread strided sums, then a tree)
-__global__ void __launch_bounds__(256) k_los_reduce(const double* contrib, int nkm, int nell, double* cl) {
+// C_ℓ = 4π Σ_k contrib (fixed order: per-thread strided sums, then a tree);
+// block o·nell + e reduces row e of output o into cl_o (skipped when NULL)
+__global__ void __launch_bounds__(256) k_los_reduce(const double* contrib, int nkm, int nell, double* cl0,
+                                                    double* cl1, double* cl2) {
     __shared__ double red[256];
-    int e = blockIdx.x;
-    if (e >= nell) return;
+    const int o = blockIdx.x / nell, e = blockIdx.x % nell;
+    double* cl = o == 0 ? cl0 : o == 1 ? cl1 : cl2;
+    if (o > 2 || !cl) return;
     double s = 0.0;
-    for (int i = threadIdx.x; i < nkm; i += blockDim.x) s += contrib[(size_t)e * nkm + i];
+    for (int i = threadIdx.x; i < nkm; i += blockDim.x) s += contrib[((size_t)o * nell + e) * nkm + i];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w >= 1; w >>= 1) {
@@ -1559,7 +1668,8 @@ int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id) {
 // shared launch helper for the integrator
 static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, const double* d_k_or_host, int B,
                         bool k_on_device, int out_mode, const double* d_xsave, int nsave, double* d_out,
-                        int* d_status, long long* d_counters, int n_stride, double* kernel_ms) {
+                        int* d_status, long long* d_counters, int n_stride, double* kernel_ms,
+                        double* d_out2 = nullptr) {
     CfgDev cd = to_cfgdev(cfg);
     std::vector<double> kh(B);
     if (k_on_device) HIPCHK(hipMemcpyAsync(kh.data(), d_k_or_host, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
@@ -1590,6 +1700,7 @@ static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, 
     A.x_save = d_xsave;
     A.nsave = nsave;
     A.out = d_out;
+    A.out2 = d_out2;
     A.status = d_status;
     A.counters = d_counters;
     A.n_state_stride = n_stride;
@@ -1771,50 +1882,74 @@ int bolt_boltsolve_rsa_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* c
     return BOLT_OK;
 }
 
-int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
-                         double* d_S, int* d_status, int64_t* d_counters, float* ms) {
+int bolt_source_grid_tp_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                            double* d_ST, double* d_SP, int* d_status, int64_t* d_counters, float* ms) {
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((r = check_cfg(cfg))) return r;
-    if (!d_k || !d_S || !d_status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid_dev: bad argument");
+    if (!d_k || (!d_ST && !d_SP) || !d_status || nk <= 0)
+        return set_error(BOLT_E_ARG, "bolt_source_grid_tp_dev: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     double dms = 0;
-    r = launch_solve(ctx, cs, cfg, d_k, nk, true, 1, nullptr, 0, d_S, d_status, (long long*)d_counters, 0, &dms);
+    const int mode = (d_ST ? 1 : 0) | (d_SP ? 2 : 0);
+    r = launch_solve(ctx, cs, cfg, d_k, nk, true, mode, nullptr, 0, d_ST, d_status, (long long*)d_counters, 0, &dms,
+                     d_SP);
     if (ms) *ms = (float)dms;
     return r;
 }
 
-int bolt_source_grid(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S,
-                     int* status, bolt_stats* stats) {
+int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                         double* d_S, int* d_status, int64_t* d_counters, float* ms) {
+    if (!d_S) return set_error(BOLT_E_ARG, "bolt_source_grid_dev: bad argument");
+    return bolt_source_grid_tp_dev(ctx, cosmo_id, cfg, d_k, nk, d_S, nullptr, d_status, d_counters, ms);
+}
+
+int bolt_source_grid_tp(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S_T,
+                        double* S_P, int* status, bolt_stats* stats) {
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((r = check_cfg(cfg))) return r;
-    if (!k || !S || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid: bad argument");
+    if (!k || (!S_T && !S_P) || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int nx = cs->host.nx;
-    DBuf dS, dst, dctr;
+    DBuf dS, dP, dst, dctr;
     size_t nb = sizeof(double) * (size_t)nk * nx;
-    HIPCHK(dS.alloc(nb));
+    if (S_T) HIPCHK(dS.alloc(nb));
+    if (S_P) HIPCHK(dP.alloc(nb));
     HIPCHK(dst.alloc(sizeof(int) * nk));
     HIPCHK(dctr.alloc(sizeof(long long) * 4 * nk));
     double ms = 0;
-    if ((r = launch_solve(ctx, cs, cfg, k, nk, false, 1, nullptr, 0, dS.as<double>(), dst.as<int>(),
-                          dctr.as<long long>(), 0, &ms)))
+    const int mode = (S_T ? 1 : 0) | (S_P ? 2 : 0);
+    if ((r = launch_solve(ctx, cs, cfg, k, nk, false, mode, nullptr, 0, dS.as<double>(), dst.as<int>(),
+                          dctr.as<long long>(), 0, &ms, dP.as<double>())))
         return r;
     std::vector<long long> ctr(4 * (size_t)nk);
-    HIPCHK(hipMemcpyAsync(S, dS.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (S_T) HIPCHK(hipMemcpyAsync(S_T, dS.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (S_P) HIPCHK(hipMemcpyAsync(S_P, dP.p, nb, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * nk, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * nk, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     fill_stats(stats, ctr, nk, ms);
     for (int i = 0; i < nk; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
     return BOLT_OK;
+}
+
+int bolt_source_grid(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S,
+                     int* status, bolt_stats* stats) {
+    if (!S) return set_error(BOLT_E_ARG, "bolt_source_grid: bad argument");
+    return bolt_source_grid_tp(ctx, cosmo_id, cfg, k, nk, S, nullptr, status, stats);
+}
+
+int bolt_source_grid_P(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S_P,
+                       int* status, bolt_stats* stats) {
+    if (!S_P) return set_error(BOLT_E_ARG, "bolt_source_grid_P: bad argument");
+    return bolt_source_grid_tp(ctx, cosmo_id, cfg, k, nk, nullptr, S_P, status, stats);
 }
 
 int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double x, double* pk,
@@ -1862,11 +1997,18 @@ int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const doubl
 
 }  // extern "C"
 
-// cltt core on device buffers; ells given on the host (validated there)
-static int cltt_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double* d_k_src, int nk_src,
-                     const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode, double* d_cl,
-                     long long* d_steps, float* ms_out) {
+// C_ℓ core on device buffers; ells given on the host (validated there).
+// d_S: temperature source or NULL, d_SP: polarization source or NULL; outputs
+// d_tt (needs d_S), d_te (needs both), d_ee (needs d_SP), each may be NULL.
+static int cl_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double* d_SP, const double* d_k_src,
+                   int nk_src, const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode,
+                   double* d_tt, double* d_te, double* d_ee, long long* d_steps, float* ms_out) {
     if (bessel_mode != 0) return set_error(BOLT_E_ARG, "bessel_mode 1 (spline table) is not implemented on device");
+    if ((d_tt && !d_S) || (d_te && (!d_S || !d_SP)) || (d_ee && !d_SP))
+        return set_error(BOLT_E_ARG, "a requested spectrum lacks its source grid");
+    if (!d_tt && !d_te && !d_ee) return set_error(BOLT_E_ARG, "no spectrum requested");
+    // two sources share the recurrence only when a cross spectrum or both autos are wanted
+    const bool two = d_te || (d_tt && d_ee);
     int nell = (int)ells.size();
     int lmax = 0;
     for (int l : ells) {
@@ -1888,13 +2030,17 @@ static int cltt_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const doub
     HIPCHK(dl.alloc(sizeof(int) * (lmax + 1)));
     HIPCHK(deta.alloc(sizeof(double) * nxi));
     HIPCHK(ddx.alloc(sizeof(double) * nxi));
-    HIPCHK(dcon.alloc(sizeof(double) * (size_t)nell * (nkd - 1)));
+    const int nout = two ? 3 : 1;
+    HIPCHK(dcon.alloc(sizeof(double) * (size_t)nout * nell * (nkd - 1)));
     HIPCHK(hipMemcpyAsync(dl.p, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, ctx->stream, cs->d_self, i0, nxi,
                        deta.as<double>(), ddx.as<double>());
     HIPCHK(hipGetLastError());
     LosArgs A;
-    A.cosmo = cs->d_self; A.S = d_S; A.k_src = d_k_src; A.nk_src = nk_src; A.kgrid = d_kgrid; A.nkd = nkd;
+    A.cosmo = cs->d_self; A.k_src = d_k_src; A.nk_src = nk_src; A.kgrid = d_kgrid; A.nkd = nkd;
+    A.S = two ? d_S : (d_tt ? d_S : d_SP);
+    A.SP = two ? d_SP : nullptr;
+    A.spin = (!two && !d_tt) ? 1 : 0;
     A.lidx = dl.as<int>(); A.lmax = lmax; A.nell = nell; A.etaj = deta.as<double>(); A.dxj = ddx.as<double>();
     A.i0 = i0; A.nxi = nxi; A.contrib = dcon.as<double>(); A.steps = d_steps;
     hipEvent_t e0, e1;
@@ -1902,10 +2048,16 @@ static int cltt_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const doub
     HIPCHK(hipEventCreate(&e1));
     int nkm = nkd - 1;
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    hipLaunchKernelGGL(k_los, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
+    if (two) hipLaunchKernelGGL(k_los<2>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(k_los<1>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, ctx->stream));
-    hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell, d_cl);
+    if (two)
+        hipLaunchKernelGGL(k_los_reduce, dim3(3 * nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell, d_tt,
+                           d_te, d_ee);
+    else
+        hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell,
+                           d_tt ? d_tt : d_ee, (double*)nullptr, (double*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -1940,8 +2092,8 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
     HIPCHK(hipMemcpyAsync(dks.p, k_src, sizeof(double) * nk_src, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
     std::vector<int> ev(ells, ells + nell);
-    if ((r = cltt_core(ctx, cs, dS.as<double>(), dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev, bessel_mode,
-                       dcl.as<double>(), nullptr, nullptr)))
+    if ((r = cl_core(ctx, cs, dS.as<double>(), nullptr, dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
+                     bessel_mode, dcl.as<double>(), nullptr, nullptr, nullptr, nullptr)))
         return r;
     HIPCHK(hipMemcpy(cl, dcl.p, sizeof(double) * nell, hipMemcpyDeviceToHost));
     return BOLT_OK;
@@ -1960,7 +2112,57 @@ int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* 
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<int> ev(ells, ells + nell);
-    return cltt_core(ctx, cs, d_S, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_cl, (long long*)d_steps, ms);
+    return cl_core(ctx, cs, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_cl, nullptr, nullptr,
+                   (long long*)d_steps, ms);
+}
+
+int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, const double* k_src, int nk_src,
+            const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cltt, double* clte,
+            double* clee) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((!S_T && !S_P) || !k_src || !kgrid || !ells || nk_src < 2 || nkd < 2 || nell < 1)
+        return set_error(BOLT_E_ARG, "bolt_cl: bad argument");
+    for (int i = 1; i < nk_src; ++i) if (!(k_src[i] > k_src[i - 1])) return set_error(BOLT_E_ARG, "k_src must increase");
+    for (int i = 1; i < nkd; ++i) if (!(kgrid[i] > kgrid[i - 1])) return set_error(BOLT_E_ARG, "kgrid must increase");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    int nx = cs->host.nx;
+    size_t sb = sizeof(double) * (size_t)nk_src * nx;
+    DBuf dS, dP, dks, dkg, dcl;
+    if (S_T) { HIPCHK(dS.alloc(sb)); HIPCHK(hipMemcpyAsync(dS.p, S_T, sb, hipMemcpyHostToDevice, ctx->stream)); }
+    if (S_P) { HIPCHK(dP.alloc(sb)); HIPCHK(hipMemcpyAsync(dP.p, S_P, sb, hipMemcpyHostToDevice, ctx->stream)); }
+    HIPCHK(dks.alloc(sizeof(double) * nk_src));
+    HIPCHK(dkg.alloc(sizeof(double) * nkd));
+    HIPCHK(dcl.alloc(sizeof(double) * 3 * nell));
+    HIPCHK(hipMemcpyAsync(dks.p, k_src, sizeof(double) * nk_src, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<int> ev(ells, ells + nell);
+    double* d = dcl.as<double>();
+    if ((r = cl_core(ctx, cs, dS.as<double>(), dP.as<double>(), dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
+                     bessel_mode, cltt ? d : nullptr, clte ? d + nell : nullptr, clee ? d + 2 * nell : nullptr,
+                     nullptr, nullptr)))
+        return r;
+    if (cltt) HIPCHK(hipMemcpy(cltt, d, sizeof(double) * nell, hipMemcpyDeviceToHost));
+    if (clte) HIPCHK(hipMemcpy(clte, d + nell, sizeof(double) * nell, hipMemcpyDeviceToHost));
+    if (clee) HIPCHK(hipMemcpy(clee, d + 2 * nell, sizeof(double) * nell, hipMemcpyDeviceToHost));
+    return BOLT_OK;
+}
+
+int bolt_cl_dev(bolt_ctx* ctx, int cosmo_id, const double* d_ST, const double* d_SP, const double* d_k_src,
+                int nk_src, const double* d_kgrid, int nkd, const int* ells, int nell, double* d_tt, double* d_te,
+                double* d_ee, int64_t* d_steps, float* ms) {
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((!d_ST && !d_SP) || !d_k_src || !d_kgrid || !ells || nk_src < 2 || nkd < 2 || nell < 1)
+        return set_error(BOLT_E_ARG, "bolt_cl_dev: bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<int> ev(ells, ells + nell);
+    return cl_core(ctx, cs, d_ST, d_SP, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_tt, d_te, d_ee, (long long*)d_steps,
+                   ms);
 }
 
 }  // extern "C"

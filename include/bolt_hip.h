@@ -161,6 +161,21 @@ int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg,
                          const double* d_k, int nk, double* d_S, int* d_status,
                          int64_t* d_counters, float* ms);
 
+/* source_grid_P(par, bg, ih, k_grid, BasicNewtonian(); ℓᵧ, reltol)
+ * (spectra.jl:25-42) with source_function_P (perturbations.jl:385-404).
+ * S_P: [nk][nx]; its last-node value is ±Inf (y = 0 there), as in Bolt. */
+int bolt_source_grid_P(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k,
+                       int nk, double* S_P, int* status, bolt_stats* stats);
+/* Both grids from ONE solve per k-mode: replaces the pair source_grid +
+ * source_grid_P that cltt/clte/clee callers run back to back (runtests.jl:156-157).
+ * Either S_T or S_P may be NULL. */
+int bolt_source_grid_tp(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k,
+                        int nk, double* S_T, double* S_P, int* status, bolt_stats* stats);
+/* Device-resident form of bolt_source_grid_tp (pointers as in bolt_source_grid_dev). */
+int bolt_source_grid_tp_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg,
+                            const double* d_k, int nk, double* d_ST, double* d_SP,
+                            int* d_status, int64_t* d_counters, float* ms);
+
 /* cltt(ℓ, sf, kgrid, par, bg) for a list of ℓ (spectra.jl:84-96, 132-150).
  * S: [nk_src][nx] source grid on k_src; kgrid: the dense k grid (nkd nodes, the
  * sum runs over its nkd-1 midpoints); bessel_mode 0 = j_ℓ by recurrence. */
@@ -173,6 +188,19 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
 int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
                   const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
                   int64_t* d_steps, float* ms);
+
+/* cltt, clte and clee for a list of ℓ (spectra.jl:84-130, 132-160) from one
+ * LOS pass: T_ℓ of the temperature source S_T and the polarization source S_P
+ * share each j_ℓ; E terms carry the spin factor √((ℓ+2)(ℓ+1)ℓ(ℓ−1)).
+ * S_T / S_P: [nk_src][nx] or NULL; any of cltt/clte/clee may be NULL, a
+ * requested spectrum needs its sources. */
+int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, const double* k_src,
+            int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode,
+            double* cltt, double* clte, double* clee);
+/* Device-resident bolt_cl (ℓ list on the host; d_steps and ms as in bolt_cltt_dev). */
+int bolt_cl_dev(bolt_ctx* ctx, int cosmo_id, const double* d_ST, const double* d_SP,
+                const double* d_k_src, int nk_src, const double* d_kgrid, int nkd, const int* ells,
+                int nell, double* d_tt, double* d_te, double* d_ee, int64_t* d_steps, float* ms);
 
 /* plin(k, par, bg, ih, n_q, ℓᵧ, ℓ_ν, ℓ_mν, x, reltol) for nk wavenumbers
  * (spectra.jl:163-198); x must be a grid node or 0. */

@@ -439,6 +439,20 @@ double orc_source(const orc_params* p, const orc_tables* t, const orc_cfg* c, do
     return term1 + term2 + term3;
 }
 
+/* source_function_P (perturbations.jl:385-404): (3/(4y²)) g̃ Π with
+ * y = k(η(x_grid[end]) − η(x)) and Π from the (RSA-mutated) u.  At the last
+ * node y = 0 and the value is ±Inf (or NaN for Π = 0), as in Julia. */
+double orc_source_P(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                    const double* u) {
+    (void)p;
+    lay_t L = layout(c);
+    double gt = orc_spline(t, T_G, x, 0);
+    double y = k * (orc_spline(t, T_ETA, xnode(t, t->nx - 1), 0) - orc_spline(t, T_ETA, x, 0));
+    const double* T = u + L.iT; const double* P = u + L.iP;
+    double Pi = T[2] + P[2] + P[0];
+    return (3.0 / (4.0 * y * y)) * gt * Pi;
+}
+
 /* ------------------------------------------------ sparse-skipping GEPP on W */
 typedef struct {
     int n;
@@ -723,8 +737,9 @@ typedef struct {
     const orc_tables* t;
     const orc_cfg* c;
     double k;
-    int mode; /* 0 save u, 1 source */
+    int mode; /* 0 save u; bit 1: temperature source into out, bit 2: polarization source into out2 */
     double* work;
+    double* out2;
 } save_t;
 
 static void save_obs(obs_t* o, double xa, const double* ua, const double* fa, double xb, const double* ub,
@@ -736,10 +751,11 @@ static void save_obs(obs_t* o, double xa, const double* ua, const double* fa, do
         if (!take) break;
         double* dst = (s->mode == 0) ? s->out + (size_t)s->next * n : s->work;
         hermite(xa, ua, fa, xb, ub, fb, n, xv, dst);
-        if (s->mode == 1) {
+        if (s->mode != 0) {
             double* du = s->work + n;
-            orc_rhs(s->p, s->t, s->c, s->k, xv, dst, du, 1); /* spectra.jl:16 mutates u */
-            s->out[s->next] = orc_source(s->p, s->t, s->c, s->k, xv, dst, du);
+            orc_rhs(s->p, s->t, s->c, s->k, xv, dst, du, 1); /* spectra.jl:16, 36 mutate u */
+            if (s->mode & 1) s->out[s->next] = orc_source(s->p, s->t, s->c, s->k, xv, dst, du);
+            if (s->mode & 2) s->out2[s->next] = orc_source_P(s->p, s->t, s->c, s->k, xv, dst);
         }
         s->next++;
     }
@@ -747,7 +763,7 @@ static void save_obs(obs_t* o, double xa, const double* ua, const double* fa, do
 
 int orc_boltsolve(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, const double* x_save,
                   int nsave, double* out, orc_stats* st) {
-    save_t s = {x_save, nsave, 0, out, p, t, c, k, 0, NULL};
+    save_t s = {x_save, nsave, 0, out, p, t, c, k, 0, NULL, NULL};
     obs_t o = {save_obs, &s};
     return integrate(p, t, c, k, &o, st);
 }
@@ -770,18 +786,25 @@ int orc_boltsolve_steps(const orc_params* p, const orc_tables* t, const orc_cfg*
     return r;
 }
 
-int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S,
-                      orc_stats* st) {
+/* one solve, temperature (spectra.jl:6-23) and/or polarization (spectra.jl:25-42)
+ * source columns at every x node; either output may be NULL */
+int orc_source_column_tp(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S_T,
+                         double* S_P, orc_stats* st) {
     int n = orc_state_length(c);
     double* xs = (double*)malloc(sizeof(double) * t->nx);
     double* work = (double*)malloc(sizeof(double) * 2 * n);
     for (int j = 0; j < t->nx; ++j) xs[j] = xnode(t, j);
-    save_t s = {xs, t->nx, 0, S, p, t, c, k, 1, work};
+    save_t s = {xs, t->nx, 0, S_T, p, t, c, k, (S_T ? 1 : 0) | (S_P ? 2 : 0), work, S_P};
     obs_t o = {save_obs, &s};
     int r = integrate(p, t, c, k, &o, st);
     free(xs);
     free(work);
     return r;
+}
+
+int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S,
+                      orc_stats* st) {
+    return orc_source_column_tp(p, t, c, k, S, NULL, st);
 }
 
 int orc_source_grid(const orc_params* p, const orc_tables* t, const orc_cfg* c, const double* k, int nk,
@@ -964,10 +987,15 @@ double orc_bessel_table_eval(const double* c, double zmax, double z) {
            (2.0 / 3.0 - r * r + 0.5 * r * r * r) * q[2] + s * s * s / 6.0 * q[3];
 }
 
-/* ------------------------------------------------------------------- cltt */
-int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const double* k_src, int nk_src,
-             const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cl, int nthreads,
-             int kbeg, int kend) {
+/* ---------------------------------------------------- cltt / clte / clee */
+/* C_ℓ^TT (spectra.jl:84-96), C_ℓ^TE (spectra.jl:98-114), C_ℓ^EE (116-130):
+ * T_ℓ(k) = Σ_x j_ℓ(k(η₀−η(x))) S(x,k) Δx for each source (Tl, spectra.jl:70-82),
+ * E-mode terms times the spin factor ℓð = √((ℓ+2)(ℓ+1)ℓ(ℓ−1)), summed over the
+ * dense-k midpoints with A(k/0.05)^(n−1) dk/k and 4π.  S_P may be NULL (TT
+ * only); any output may be NULL. */
+int orc_cl(const orc_params* p, const orc_tables* t, const double* S, const double* S_P, const double* k_src,
+           int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cl,
+           double* clte, double* clee, int nthreads, int kbeg, int kend) {
     int nx = t->nx;
     int lmax = 0;
     for (int e = 0; e < nell; ++e) if (ells[e] > lmax) lmax = ells[e];
@@ -995,13 +1023,15 @@ int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const do
     if (kbeg < 0) kbeg = 0;
     if (kend > nkd - 1) kend = nkd - 1;
     int nkm = kend - kbeg;
-    double* contrib = (double*)calloc((size_t)(nkm > 0 ? nkm : 1) * nell, sizeof(double));
+    const int nout = 3;
+    double* contrib = (double*)calloc((size_t)(nkm > 0 ? nkm : 1) * nell * nout, sizeof(double));
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
 #endif
     {
         double* jl = (double*)malloc(sizeof(double) * (lmax + 1));
         double* th = (double*)malloc(sizeof(double) * nell);
+        double* te = (double*)malloc(sizeof(double) * nell);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 8)
 #endif
@@ -1011,35 +1041,56 @@ int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const do
             int ia = 0;
             while (ia < nk_src - 2 && k >= k_src[ia + 1]) ++ia;
             double tt = (k - k_src[ia]) / (k_src[ia + 1] - k_src[ia]);
-            const double* Sa = S + (size_t)ia * nx;
-            const double* Sb = S + (size_t)(ia + 1) * nx;
-            for (int e = 0; e < nell; ++e) th[e] = 0.0;
+            const double* Sa = S ? S + (size_t)ia * nx : NULL;
+            const double* Sb = S ? S + (size_t)(ia + 1) * nx : NULL;
+            const double* Pa = S_P ? S_P + (size_t)ia * nx : NULL;
+            const double* Pb = S_P ? S_P + (size_t)(ia + 1) * nx : NULL;
+            for (int e = 0; e < nell; ++e) th[e] = te[e] = 0.0;
             for (int j = 0; j < nxi; ++j) {
                 int jx = i0 + j;
-                double s = (1.0 - tt) * Sa[jx] + tt * Sb[jx];
+                double s = Sa ? (1.0 - tt) * Sa[jx] + tt * Sb[jx] : 0.0;
+                double sp = Pa ? (1.0 - tt) * Pa[jx] + tt * Pb[jx] : 0.0;
                 double z = k * (eta0 - etaj[j]);
-                if (bessel_mode == 0) {
-                    orc_sph_bessel(lmax, z, jl);
-                    for (int e = 0; e < nell; ++e) th[e] += jl[ells[e]] * s * dxj[j];
-                } else {
-                    for (int e = 0; e < nell; ++e)
-                        th[e] += orc_bessel_table_eval(tabs + 5003 * (size_t)e, zmax, z) * s * dxj[j];
+                if (bessel_mode == 0) orc_sph_bessel(lmax, z, jl);
+                for (int e = 0; e < nell; ++e) {
+                    double jv = bessel_mode == 0 ? jl[ells[e]] : orc_bessel_table_eval(tabs + 5003 * (size_t)e, zmax, z);
+                    th[e] += jv * s * dxj[j];
+                    te[e] += jv * sp * dxj[j];
                 }
             }
             double Pprim = p->A * pow(k / 0.05, p->n - 1.0);
-            for (int e = 0; e < nell; ++e) contrib[(size_t)(ik - kbeg) * nell + e] = th[e] * th[e] * Pprim * dk / k;
+            for (int e = 0; e < nell; ++e) {
+                long long l = ells[e];
+                double lf = sqrt((double)((l + 2) * (l + 1) * l * (l - 1)));
+                double ep = te[e] * lf;
+                double* cc = contrib + ((size_t)(ik - kbeg) * nell + e) * nout;
+                cc[0] = th[e] * th[e] * Pprim * dk / k;
+                cc[1] = th[e] * ep * Pprim * dk / k;
+                cc[2] = ep * ep * Pprim * dk / k;
+            }
         }
         free(jl);
         free(th);
+        free(te);
     }
     for (int e = 0; e < nell; ++e) {
-        double s = 0.0;
-        for (int ik = 0; ik < nkm; ++ik) s += contrib[(size_t)ik * nell + e];
-        cl[e] = 4.0 * M_PI * s;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int ik = 0; ik < nkm; ++ik)
+            for (int o = 0; o < nout; ++o) acc[o] += contrib[((size_t)ik * nell + e) * nout + o];
+        if (cl) cl[e] = 4.0 * M_PI * acc[0];
+        if (clte) clte[e] = 4.0 * M_PI * acc[1];
+        if (clee) clee[e] = 4.0 * M_PI * acc[2];
     }
     free(contrib);
     free(etaj);
     free(dxj);
     free(tabs);
     return 0;
+}
+
+int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const double* k_src, int nk_src,
+             const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cl, int nthreads,
+             int kbeg, int kend) {
+    return orc_cl(p, t, S, NULL, k_src, nk_src, kgrid, nkd, ells, nell, bessel_mode, cl, NULL, NULL, nthreads, kbeg,
+                  kend);
 }

@@ -69,6 +69,14 @@ int orc_boltsolve_steps(const orc_params* p, const orc_tables* t, const orc_cfg*
 int orc_boltsolve_rsa(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
                       double* out, orc_stats* st);
 /* source_grid column for one k (spectra.jl:10-18); S [nx] */
+/* source_function_P (perturbations.jl:385-404) at x from the (mutated) u */
+double orc_source_P(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                    const double* u);
+
+/* one solve; temperature and/or polarization columns (either may be NULL) */
+int orc_source_column_tp(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S_T,
+                         double* S_P, orc_stats* st);
+
 int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
                       double* S, orc_stats* st);
 /* source_grid over nk modes with nthreads OpenMP threads; S [nk][nx] */
@@ -89,6 +97,12 @@ double orc_bessel_table_eval(const double* coef, double zmax, double z);
  * 1 = Bolt's 5001-node spline table.  k-midpoint range [kbeg, kend) of the
  * nkd-1 midpoints (pass 0, nkd-1 for all); partial sums before the 4π factor
  * are returned when partial != 0. */
+/* TT, TE, EE from temperature S and polarization S_P source grids (S_P may
+ * be NULL; any output may be NULL) */
+int orc_cl(const orc_params* p, const orc_tables* t, const double* S, const double* S_P, const double* k_src,
+           int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cl,
+           double* clte, double* clee, int nthreads, int kbeg, int kend);
+
 int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const double* k_src,
              int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode,
              double* cl, int nthreads, int kbeg, int kend);

@@ -63,6 +63,9 @@ def lib():
         L.orc_boltsolve_steps.argtypes = [P, T, C, _D, _PD, _I, _PI, ctypes.POINTER(OrcStats)]
         L.orc_boltsolve_rsa.argtypes = [P, T, C, _D, _PD, ctypes.POINTER(OrcStats)]
         L.orc_source_column.argtypes = [P, T, C, _D, _PD, ctypes.POINTER(OrcStats)]
+        L.orc_source_column_tp.argtypes = [P, T, C, _D, _PD, _PD, ctypes.POINTER(OrcStats)]
+        L.orc_source_P.argtypes = [P, T, C, _D, _D, _PD]
+        L.orc_source_P.restype = _D
         L.orc_source_grid.argtypes = [P, T, C, _PD, _I, _PD, _PI, _I]
         L.orc_plin.argtypes = [P, T, C, _D, _D, ctypes.POINTER(OrcStats)]
         L.orc_plin.restype = _D
@@ -71,6 +74,7 @@ def lib():
         L.orc_bessel_table_eval.argtypes = [_PD, _D, _D]
         L.orc_bessel_table_eval.restype = _D
         L.orc_cltt.argtypes = [P, T, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _I, _I, _I]
+        L.orc_cl.argtypes = [P, T, _PD, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _PD, _PD, _I, _I, _I]
         L.orc_tableau.argtypes = [_PD, _PD, _PD, _PD, _PD]
         L.orc_rho_sigma.argtypes = [P, T, _I, _D, _PD, _PD, _PD, _PD]
         L.orc_theta.argtypes = [P, T, _I, _D, _PD]
@@ -162,6 +166,19 @@ class Oracle:
                                 ctypes.byref(st))
         return S, st
 
+    def source_P(self, cfg, k, x, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_source_P(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, x, _pd(u))
+
+    def source_column_tp(self, cfg, k):
+        """temperature and polarization source columns from one solve"""
+        S = np.zeros(self.nx)
+        SP = np.zeros(self.nx)
+        st = OrcStats()
+        lib().orc_source_column_tp(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, _pd(S),
+                                   _pd(SP), ctypes.byref(st))
+        return S, SP, st
+
     def source_grid(self, cfg, ks, nthreads=0):
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         S = np.zeros((len(ks), self.nx))
@@ -189,6 +206,23 @@ class Oracle:
                            nthreads, kbeg, kend)
         assert r == 0
         return cl
+
+
+    def cl(self, S, S_P, k_src, kgrid, ells, bessel_mode=0, nthreads=0, kbeg=0, kend=None):
+        """(C_ℓ^TT, C_ℓ^TE, C_ℓ^EE); S_P may be None (then TE/EE are zero)"""
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        SP = None if S_P is None else np.ascontiguousarray(S_P, dtype=np.float64)
+        k_src = np.ascontiguousarray(k_src, dtype=np.float64)
+        kgrid = np.ascontiguousarray(kgrid, dtype=np.float64)
+        ells = np.ascontiguousarray(ells, dtype=np.int32)
+        tt, te, ee = np.zeros(len(ells)), np.zeros(len(ells)), np.zeros(len(ells))
+        if kend is None:
+            kend = len(kgrid) - 1
+        r = lib().orc_cl(ctypes.byref(self.par), ctypes.byref(self.tab), _pd(S), None if SP is None else _pd(SP),
+                         _pd(k_src), len(k_src), _pd(kgrid), len(kgrid), ells.ctypes.data_as(_PI), len(ells),
+                         bessel_mode, _pd(tt), _pd(te), _pd(ee), nthreads, kbeg, kend)
+        assert r == 0
+        return tt, te, ee
 
 
 def tableau():

@@ -137,6 +137,53 @@ def test_cltt_matches_oracle_same_source():
     np.testing.assert_allclose(cl, ref, rtol=1e-6)
 
 
+@pytest.mark.parametrize("k", [0.0021, 0.05, 0.2])
+def test_source_columns_tp_match_oracle(k):
+    """source_grid + source_grid_P from one solve (spectra.jl:6-42) against the
+    oracle's columns: the same step count, temperature within 1e-6 of its
+    largest value, polarization within 1e-6 relative node by node on the
+    line-of-sight range, and ±Inf at the last node (y = 0) on both."""
+    par, bg, ih = cosmo.background("default")
+    o = cosmo.oracle("default")
+    sf_t, sf_p = boltmi.source_grids_TP(par, bg, ih, np.array([k]), boltmi.BasicNewtonian(), 8, 1e-11)
+    ref_t, ref_p, ost = o.source_column_tp(cosmo.ocfg(8, 8, 10, 15, 1e-11, 1e-6), k)
+    assert sf_t.stats["accepted"] == ost.accepted
+    assert scaled_err(sf_t.grid[0], ref_t) < 1e-6
+    gp = sf_p.grid[0]
+    assert np.isinf(gp[-1]) and np.isinf(ref_p[-1]) and np.sign(gp[-1]) == np.sign(ref_p[-1])
+    assert scaled_err(gp[:-1], ref_p[:-1]) < 1e-6
+    los = (bg.x_grid > -8.0)[:-1]
+    big = np.abs(ref_p[:-1]) > 1e-8 * np.max(np.abs(ref_p[:-1][los]))
+    sel = los & big
+    assert np.max(np.abs(gp[:-1][sel] / ref_p[:-1][sel] - 1)) < 1e-6
+    alone = boltmi.source_grid_P(par, bg, ih, np.array([k]), boltmi.BasicNewtonian(), 8, 1e-11)
+    assert np.array_equal(alone.grid, sf_p.grid, equal_nan=True)
+
+
+def test_cl_tt_te_ee_match_oracle_same_sources():
+    """One LOS pass for TT, TE and EE (spectra.jl:84-130) against the oracle's
+    exact-j_ℓ sums on the same two source grids; the EE-only pass (one source
+    with the spin factor) and the TT-only pass agree with it bit for bit."""
+    par, bg, ih = cosmo.background("default")
+    o = cosmo.oracle("default")
+    k_src = boltmi.quadratic_k(0.1 * bg.H0, 1000 * bg.H0, 40)
+    cfg = cosmo.ocfg(8, 8, 10, 15, 1e-8, 1e-6)
+    cols = [o.source_column_tp(cfg, float(k)) for k in k_src]
+    S = np.array([c[0] for c in cols])
+    SP = np.array([c[1] for c in cols])
+    dense = boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 600)
+    ells = np.array([2, 3, 10, 37, 100, 250, 500, 777])
+    sf_t = boltmi.SourceGrid(bg.x_grid, k_src, S)
+    sf_p = boltmi.SourceGrid(bg.x_grid, k_src, SP)
+    tt, te, ee = boltmi.cl_on_grid(ells, sf_t, sf_p, dense, par, bg)
+    rtt, rte, ree = o.cl(S, SP, k_src, dense, ells, 0, nthreads=8)
+    np.testing.assert_allclose(tt, rtt, rtol=1e-6)
+    np.testing.assert_allclose(ee, ree, rtol=1e-6)
+    np.testing.assert_allclose(te, rte, rtol=1e-6, atol=1e-6 * np.max(np.abs(rte)))
+    assert np.array_equal(boltmi.clee_on_grid(ells, sf_p, dense, par, bg), ee)
+    np.testing.assert_allclose(boltmi.cltt_on_grid(ells, sf_t, dense, par, bg), tt, rtol=1e-14)
+
+
 @pytest.mark.parametrize("k", [0.01, 0.1])
 def test_plin_matches_oracle(k):
     par, bg, ih = cosmo.background("massive")

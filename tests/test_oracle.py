@@ -174,19 +174,72 @@ def test_cltt_single_shell_known_answer():
     np.testing.assert_allclose(cl, expect, rtol=2e-3)
 
 
+def test_cl_polarization_spin_factor_single_shell():
+    """With the polarization source equal to the temperature one, TE = ℓð·TT
+    and EE = ℓð²·TT, ℓð = √((ℓ+2)(ℓ+1)ℓ(ℓ−1)) (spectra.jl:98-130); TT itself
+    is the single-shell closed form above."""
+    par, bg, ih = cosmo.background("default")
+    p2 = boltmi.CosmoParams(**{**par.__dict__, "n": 1.0})
+    o = pyoracle.Oracle(p2.as_tuple(), bg.tables_dict())
+    j0 = 1300
+    k_src = np.array([1e-7, 10.0])
+    S = np.zeros((2, bg.nx))
+    S[:, j0] = 1.0
+    dense = boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
+    ells = np.array([2, 10, 40])
+    tt, te, ee = o.cl(S, S, k_src, dense, ells, 0, nthreads=8)
+    lf = np.sqrt((ells + 2.0) * (ells + 1) * ells * (ells - 1))
+    np.testing.assert_allclose(te, lf * tt, rtol=1e-13)
+    np.testing.assert_allclose(ee, lf ** 2 * tt, rtol=1e-13)
+    np.testing.assert_array_equal(tt, o.cltt(S, k_src, dense, ells, 0, nthreads=8))
+    dx = bg.x_grid[j0 + 1] - bg.x_grid[j0]
+    np.testing.assert_allclose(tt, 4 * np.pi * p2.A * dx ** 2 / (2 * ells * (ells + 1)), rtol=2e-3)
+
+
+def test_source_P_last_node_and_shape():
+    """source_function_P = (3/(4y²)) g̃ Π with y = k(η(x_end) − η(x)): the
+    value scales as 1/k² at fixed u and x, is linear in Π = Θ₂ + Θᵖ₂ + Θᵖ₀, and
+    is infinite at the last node, where y = 0 (as Julia's 3/(4·0²))."""
+    par, bg, ih = cosmo.background("default")
+    o = cosmo.oracle("default")
+    cfg = cosmo.ocfg(8, 8, 10, 15, 1e-6, 1e-6)
+    n = o.n(cfg)
+    u = np.zeros(n)
+    ng = 8
+    u[2] = 0.3             # Θ₂
+    u[ng + 1] = 0.2        # Θᵖ₀
+    u[ng + 3] = -0.1       # Θᵖ₂
+    x = -6.9
+    a = o.source_P(cfg, 0.01, x, u)
+    b = o.source_P(cfg, 0.02, x, u)
+    assert a != 0 and np.isfinite(a)
+    np.testing.assert_allclose(b, a / 4, rtol=1e-14)
+    u2 = u.copy(); u2[2] = 0.6
+    np.testing.assert_allclose(o.source_P(cfg, 0.01, x, u2), a * 0.7 / 0.4, rtol=1e-13)
+    assert np.isinf(o.source_P(cfg, 0.01, bg.x_grid[-1], u))
+
+
 @pytest.mark.slow
-def test_oracle_cltt_vs_camb_fixture(golden):
-    """runtests.jl:149-185: source_grid (100 k, ℓᵧ=8, reltol 1e-11) + cltt at
-    ℓ = 10:10:2500 vs CAMB unlensed D_ℓ^TT at 1.1e-1 relative."""
+def test_oracle_cl_vs_camb_fixture(golden):
+    """runtests.jl:149-185: source_grid and source_grid_P (100 k, ℓᵧ=8,
+    reltol 1e-11), then cltt and clee at ℓ = 10:10:2500 vs CAMB unlensed D_ℓ^TT
+    and D_ℓ^EE at 1.1e-1 relative (TE is not asserted there: zero crossings)."""
+    from concurrent.futures import ThreadPoolExecutor
     par, bg, ih = cosmo.background("default")
     o = cosmo.oracle("default")
     k_grid = boltmi.quadratic_k(0.1 * bg.H0, 1000 * bg.H0, 100)
-    S, status = o.source_grid(cosmo.ocfg(8, 8, 10, 15, 1e-11, 1e-6), k_grid, nthreads=8)
+    cfg = cosmo.ocfg(8, 8, 10, 15, 1e-11, 1e-6)
+    with ThreadPoolExecutor(8) as ex:
+        cols = list(ex.map(lambda k: o.source_column_tp(cfg, float(k)), k_grid))
+    S = np.array([c[0] for c in cols])
+    SP = np.array([c[1] for c in cols])
+    S_only, status = o.source_grid(cfg, k_grid, nthreads=8)
     assert not status.any()
+    np.testing.assert_array_equal(S, S_only)
     ells = np.arange(10, 2501, 10)
     dense = boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 5000)
-    cl = o.cltt(S, k_grid, dense, ells, 0, nthreads=8)
+    tt, te, ee = o.cl(S, SP, k_grid, dense, ells, 0, nthreads=8)
     d = golden("camb_cl.npz")
-    camb = np.interp(ells, d["ell"], d["tt"])
-    dl = ells * (ells + 1) / (2 * np.pi) * cl
-    assert np.max(np.abs(dl / camb - 1)) < 1.1e-1
+    fac = ells * (ells + 1) / (2 * np.pi)
+    assert np.max(np.abs(fac * tt / np.interp(ells, d["ell"], d["tt"]) - 1)) < 1.1e-1
+    assert np.max(np.abs(fac * ee / np.interp(ells, d["ell"], d["ee"]) - 1)) < 1.1e-1
