@@ -4,7 +4,7 @@ from .api import (MASS_NATURAL, Background, BasicNewtonian, Context, CosmoParams
                   boltsolve_batch, boltsolve_rsa, cl_on_grid, clee, clee_on_grid, clte, clte_on_grid, cltt,
                   cltt_on_grid, default_context, hier_cfg,
                   initial_conditions_batch, log10_k, plin, quadratic_k, recfast_xe, rhs_batch, source_grid,
-                  source_grid_P, source_grids_TP,
+                  source_grid_P, source_grid_batch, source_grids_TP,
                   state_length)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -72,6 +72,10 @@ SIGNATURES = {
                                  ctypes.POINTER(StatsC)]),
     "bolt_source_grid_tp_dev": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _VP, _I, _VP, _VP, _VP, _VP,
                                      ctypes.POINTER(ctypes.c_float)]),
+    "bolt_source_grid_batch": (_I, [_VP, _I, _PI, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _PD, _PI,
+                                    ctypes.POINTER(StatsC)]),
+    "bolt_source_grid_batch_dev": (_I, [_VP, _I, _PI, ctypes.POINTER(HierCfgC), _VP, _I, _VP, _VP, _VP, _VP,
+                                        ctypes.POINTER(ctypes.c_float)]),
     "bolt_cltt": (_I, [_VP, _I, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD]),
     "bolt_cl": (_I, [_VP, _I, _PD, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _PD, _PD]),
     "bolt_cl_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, _VP, _VP,

@@ -367,6 +367,44 @@ def source_grids_TP(par, bg, ih, k_grid, integrator=None, ℓᵧ=8, reltol=1e-11
             SourceGrid(bg.x_grid, ks, SP, stats) if want_p else None)
 
 
+def _per(v, n):
+    return list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
+
+
+def source_grid_batch(cosmos, k_grids, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν=10, nq=15, abstol=1e-6, want_t=True,
+                      want_p=False, ctx=None):
+    """source_grid (and/or source_grid_P) for many cosmologies in one launch.
+
+    cosmos: [(par, bg, ih), ...]; k_grids: one k grid for all, or one per
+    cosmology (equal lengths); hierarchy sizes and tolerances: a scalar or one
+    per cosmology.  Returns ([SourceGrid] or None, [SourceGrid] or None), each
+    grid identical to what source_grid gives for that cosmology alone."""
+    ctx = ctx or default_context()
+    nc = len(cosmos)
+    kg = np.asarray(k_grids, dtype=np.float64)
+    if kg.ndim == 1:
+        kg = np.tile(kg, (nc, 1))
+    kg = np.ascontiguousarray(kg)
+    nk = kg.shape[1]
+    cfgs = (_abi.HierCfgC * nc)()
+    for c, (g, n_, m_, q_, rt, at) in enumerate(zip(_per(ℓᵧ, nc), _per(ℓ_ν, nc), _per(ℓ_mν, nc), _per(nq, nc),
+                                                    _per(reltol, nc), _per(abstol, nc))):
+        cfgs[c] = hier_cfg(g, n_, m_, q_, rt, at)
+    ids = np.array([bg.cosmo_id(ctx) for (_, bg, _) in cosmos], dtype=np.int32)
+    nx = cosmos[0][1].nx
+    S = np.zeros((nc, nk, nx)) if want_t else None
+    SP = np.zeros((nc, nk, nx)) if want_p else None
+    status = np.zeros((nc, nk), dtype=np.int32)
+    st = _abi.StatsC()
+    with ctx.lock:
+        check(lib().bolt_source_grid_batch(ctx.handle, nc, pi(ids), cfgs, pd(kg), nk, pd(S) if want_t else None,
+                                           pd(SP) if want_p else None, pi(status), ctypes.byref(st)))
+    stats = _stats_dict(st)
+    out_t = [SourceGrid(cosmos[c][1].x_grid, kg[c], S[c], stats) for c in range(nc)] if want_t else None
+    out_p = [SourceGrid(cosmos[c][1].x_grid, kg[c], SP[c], stats) for c in range(nc)] if want_p else None
+    return out_t, out_p
+
+
 def quadratic_k(kmin, kmax, nk):
     """quadratic_k (spectra.jl:60-63): k_i = kmin + (kmax-kmin)(i/nk)², i = 1..nk."""
     return np.array([kmin + (kmax - kmin) * (i / nk) ** 2 for i in range(1, nk + 1)])

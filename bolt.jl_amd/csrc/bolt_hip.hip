@@ -1665,33 +1665,28 @@ int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id) {
 
 }  // extern "C"
 
-// shared launch helper for the integrator
-static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, const double* d_k_or_host, int B,
-                        bool k_on_device, int out_mode, const double* d_xsave, int nsave, double* d_out,
-                        int* d_status, long long* d_counters, int n_stride, double* kernel_ms,
-                        double* d_out2 = nullptr) {
-    CfgDev cd = to_cfgdev(cfg);
-    std::vector<double> kh(B);
-    if (k_on_device) HIPCHK(hipMemcpyAsync(kh.data(), d_k_or_host, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
-    else std::memcpy(kh.data(), d_k_or_host, sizeof(double) * B);
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < B; ++i)
-        if (!(kh[i] > 0.0) || !std::isfinite(kh[i])) return set_error(BOLT_E_ARG, "k must be positive and finite");
-    // costliest (largest k) first
-    std::vector<int> order(B);
-    for (int i = 0; i < B; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return kh[a] > kh[b]; });
-    std::vector<ModeDesc> md(B);
-    for (int i = 0; i < B; ++i) { md[i].k = kh[order[i]]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = order[i]; md[i].pad = 0; }
-    DBuf dmodes, dcfg, dctr;
+// the persistent integrator over a mode list whose modes may differ in
+// cosmology and hierarchy; modes are sorted by k, largest (most steps) first
+static int launch_modes(bolt_ctx* ctx, const std::vector<CosmoDev>& cosmos, const std::vector<CfgDev>& cfgs,
+                        std::vector<ModeDesc>& md, int out_mode, const double* d_xsave, int nsave, double* d_out,
+                        double* d_out2, int* d_status, long long* d_counters, int n_stride, double* kernel_ms) {
+    const int B = (int)md.size();
+    std::stable_sort(md.begin(), md.end(), [](const ModeDesc& a, const ModeDesc& b) { return a.k > b.k; });
+    size_t lds = 0;
+    for (const CfgDev& c : cfgs) lds = std::max(lds, ws_bytes(c.n, c.nq));
+    if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
+    DBuf dmodes, dcfg, dcos, dctr;
     HIPCHK(dmodes.alloc(sizeof(ModeDesc) * B));
-    HIPCHK(dcfg.alloc(sizeof(CfgDev)));
+    HIPCHK(dcfg.alloc(sizeof(CfgDev) * cfgs.size()));
+    HIPCHK(dcos.alloc(sizeof(CosmoDev) * cosmos.size()));
     HIPCHK(dctr.alloc(sizeof(int)));
     HIPCHK(hipMemcpyAsync(dmodes.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dcfg.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dcfg.p, cfgs.data(), sizeof(CfgDev) * cfgs.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dcos.p, cosmos.data(), sizeof(CosmoDev) * cosmos.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
     HIPCHK(hipMemsetAsync(dctr.p, 0, sizeof(int), ctx->stream));
     SolveArgs A;
-    A.cosmos = cs->d_self;
+    A.cosmos = dcos.as<CosmoDev>();
     A.cfgs = dcfg.as<CfgDev>();
     A.modes = dmodes.as<ModeDesc>();
     A.nmodes = B;
@@ -1704,8 +1699,6 @@ static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, 
     A.status = d_status;
     A.counters = d_counters;
     A.n_state_stride = n_stride;
-    size_t lds = ws_bytes(cd.n, cd.nq);
-    if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, NTHR, lds));
     if (per_cu < 1) per_cu = 1;
@@ -1724,6 +1717,33 @@ static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, 
     (void)hipEventDestroy(e1);
     if (kernel_ms) *kernel_ms = ms;
     return BOLT_OK;
+}
+
+static int read_positive_k(bolt_ctx* ctx, const double* k, int B, bool on_device, std::vector<double>& kh) {
+    kh.resize(B);
+    if (on_device) {
+        HIPCHK(hipMemcpyAsync(kh.data(), k, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } else {
+        std::memcpy(kh.data(), k, sizeof(double) * B);
+    }
+    for (int i = 0; i < B; ++i)
+        if (!(kh[i] > 0.0) || !std::isfinite(kh[i])) return set_error(BOLT_E_ARG, "k must be positive and finite");
+    return BOLT_OK;
+}
+
+// one cosmology, one hierarchy
+static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, const double* d_k_or_host, int B,
+                        bool k_on_device, int out_mode, const double* d_xsave, int nsave, double* d_out,
+                        int* d_status, long long* d_counters, int n_stride, double* kernel_ms,
+                        double* d_out2 = nullptr) {
+    std::vector<double> kh;
+    int r = read_positive_k(ctx, d_k_or_host, B, k_on_device, kh);
+    if (r) return r;
+    std::vector<ModeDesc> md(B);
+    for (int i = 0; i < B; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
+    return launch_modes(ctx, {cs->host}, {to_cfgdev(cfg)}, md, out_mode, d_xsave, nsave, d_out, d_out2, d_status,
+                        d_counters, n_stride, kernel_ms);
 }
 
 static void fill_stats(bolt_stats* st, const std::vector<long long>& ctr, int B, double ms) {
@@ -1950,6 +1970,89 @@ int bolt_source_grid_P(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, co
                        int* status, bolt_stats* stats) {
     if (!S_P) return set_error(BOLT_E_ARG, "bolt_source_grid_P: bad argument");
     return bolt_source_grid_tp(ctx, cosmo_id, cfg, k, nk, nullptr, S_P, status, stats);
+}
+
+}  // extern "C"
+
+// many cosmologies, each with its own hierarchy and k grid, in one launch
+static int source_grid_batch_core(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids, const bolt_hier_cfg* cfgs,
+                                  const double* k, int nk, bool k_on_device, double* d_ST, double* d_SP,
+                                  int* d_status, long long* d_counters, double* ms) {
+    if (ncosmo < 1 || !cosmo_ids || !cfgs || !k || nk < 1 || (!d_ST && !d_SP) || !d_status)
+        return set_error(BOLT_E_ARG, "bolt_source_grid_batch: bad argument");
+    std::vector<CosmoDev> cos(ncosmo);
+    std::vector<CfgDev> cf(ncosmo);
+    int nx = -1;
+    for (int c = 0; c < ncosmo; ++c) {
+        CosmoSlot* cs;
+        int r = get_cosmo(ctx, cosmo_ids[c], &cs);
+        if (r) return r;
+        if ((r = check_cfg(&cfgs[c]))) return r;
+        if (cfgs[c].nq != cs->host.nq) return set_error(BOLT_E_ARG, "a cfg's nq differs from its tables' nq");
+        if (nx >= 0 && cs->host.nx != nx) return set_error(BOLT_E_ARG, "the cosmologies' x grids differ in length");
+        nx = cs->host.nx;
+        cos[c] = cs->host;
+        cf[c] = to_cfgdev(&cfgs[c]);
+    }
+    const int B = ncosmo * nk;
+    std::vector<double> kh;
+    int r = read_positive_k(ctx, k, B, k_on_device, kh);
+    if (r) return r;
+    std::vector<ModeDesc> md(B);
+    for (int c = 0; c < ncosmo; ++c)
+        for (int i = 0; i < nk; ++i) {
+            ModeDesc& m = md[(size_t)c * nk + i];
+            m.k = kh[(size_t)c * nk + i]; m.cosmo = c; m.cfg = c; m.out = c * nk + i; m.pad = 0;
+        }
+    const int mode = (d_ST ? 1 : 0) | (d_SP ? 2 : 0);
+    return launch_modes(ctx, cos, cf, md, mode, nullptr, 0, d_ST, d_SP, d_status, d_counters, 0, ms);
+}
+
+extern "C" {
+
+int bolt_source_grid_batch(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids, const bolt_hier_cfg* cfgs,
+                           const double* k, int nk, double* S_T, double* S_P, int* status, bolt_stats* stats) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    if (ncosmo < 1 || nk < 1 || !cosmo_ids || !status || (!S_T && !S_P))
+        return set_error(BOLT_E_ARG, "bolt_source_grid_batch: bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    CosmoSlot* cs0;
+    int r = get_cosmo(ctx, cosmo_ids[0], &cs0);
+    if (r) return r;
+    const int B = ncosmo * nk, nx = cs0->host.nx;
+    DBuf dS, dP, dst, dctr;
+    size_t nb = sizeof(double) * (size_t)B * nx;
+    if (S_T) HIPCHK(dS.alloc(nb));
+    if (S_P) HIPCHK(dP.alloc(nb));
+    HIPCHK(dst.alloc(sizeof(int) * B));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * B));
+    double ms = 0;
+    if ((r = source_grid_batch_core(ctx, ncosmo, cosmo_ids, cfgs, k, nk, false, dS.as<double>(), dP.as<double>(),
+                                    dst.as<int>(), dctr.as<long long>(), &ms)))
+        return r;
+    std::vector<long long> ctr(4 * (size_t)B);
+    if (S_T) HIPCHK(hipMemcpyAsync(S_T, dS.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (S_P) HIPCHK(hipMemcpyAsync(S_P, dP.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctr.data(), dctr.p, sizeof(long long) * 4 * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    fill_stats(stats, ctr, B, ms);
+    for (int i = 0; i < B; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed; see status");
+    return BOLT_OK;
+}
+
+int bolt_source_grid_batch_dev(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids, const bolt_hier_cfg* cfgs,
+                               const double* d_k, int nk, double* d_ST, double* d_SP, int* d_status,
+                               int64_t* d_counters, float* ms) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    double dms = 0;
+    int r = source_grid_batch_core(ctx, ncosmo, cosmo_ids, cfgs, d_k, nk, true, d_ST, d_SP, d_status,
+                                   (long long*)d_counters, &dms);
+    if (ms) *ms = (float)dms;
+    return r;
 }
 
 int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double x, double* pk,

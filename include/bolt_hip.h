@@ -189,6 +189,21 @@ int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* 
                   const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
                   int64_t* d_steps, float* ms);
 
+/* source_grid for many cosmologies at once: the @qthreads loop over k_grid
+ * (spectra.jl:10) widened to every (cosmology, k) pair of the batch, run by one
+ * persistent launch with per-mode adaptive steps (largest k first).
+ * cosmo_ids[c] and cfgs[c] give cosmology c and its hierarchy (each cfg's nq
+ * must match its tables; all x grids must have the same length nx); k is
+ * [ncosmo][nk]; S_T / S_P (either may be NULL) are [ncosmo][nk][nx]; status is
+ * [ncosmo][nk].  Each mode's result is the same as in a single-cosmology call. */
+int bolt_source_grid_batch(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids,
+                           const bolt_hier_cfg* cfgs, const double* k, int nk, double* S_T,
+                           double* S_P, int* status, bolt_stats* stats);
+/* Device-resident form (cosmo_ids and cfgs on the host, the rest on the device). */
+int bolt_source_grid_batch_dev(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids,
+                               const bolt_hier_cfg* cfgs, const double* d_k, int nk, double* d_ST,
+                               double* d_SP, int* d_status, int64_t* d_counters, float* ms);
+
 /* cltt, clte and clee for a list of ℓ (spectra.jl:84-130, 132-160) from one
  * LOS pass: T_ℓ of the temperature source S_T and the polarization source S_P
  * share each j_ℓ; E terms carry the spin factor √((ℓ+2)(ℓ+1)ℓ(ℓ−1)).

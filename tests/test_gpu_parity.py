@@ -184,6 +184,21 @@ def test_cl_tt_te_ee_match_oracle_same_sources():
     np.testing.assert_allclose(boltmi.cltt_on_grid(ells, sf_t, dense, par, bg), tt, rtol=1e-14)
 
 
+def test_source_grid_batch_matches_single_calls():
+    """Many cosmologies in one persistent launch (the batched-cosmology
+    workload): with and without massive ν, different ℓ_mν, both sources; every
+    grid equals the single-cosmology call bit for bit."""
+    kinds = ["default", "massive", "nonu"]
+    cosmos = [cosmo.background(k) for k in kinds]
+    ks = np.array([0.003, 0.02, 0.1])
+    lmnu = [10, 20, 10]
+    St, Sp = boltmi.source_grid_batch(cosmos, ks, ℓᵧ=8, reltol=1e-8, ℓ_mν=lmnu, want_t=True, want_p=True)
+    for c, (par, bg, ih) in enumerate(cosmos):
+        t, p = boltmi.source_grids_TP(par, bg, ih, ks, None, 8, 1e-8, 8, lmnu[c])
+        assert np.array_equal(St[c].grid, t.grid), kinds[c]
+        assert np.array_equal(Sp[c].grid, p.grid, equal_nan=True), kinds[c]
+
+
 @pytest.mark.parametrize("k", [0.01, 0.1])
 def test_plin_matches_oracle(k):
     par, bg, ih = cosmo.background("massive")
