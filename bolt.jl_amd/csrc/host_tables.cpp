@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <functional>
 
 namespace bolt {
 
@@ -469,8 +470,14 @@ struct Recfast {
     }
 
     // Evaluate X_e (pre-reionization, Xe_RECFAST) and Tmat (Tmat_RECFAST) on a
-    // set of redshifts; zs need not be sorted.
-    void solve(const std::vector<double>& zs, std::vector<double>& xe, std::vector<double>& tm) {
+    // set of redshifts; zs need not be sorted.  With reio_dtm set, the
+    // reionized matter temperature (recfast.jl:513-534) is integrated jointly
+    // with the H+He+Tmat system from z_reio (which must be one of zs) down:
+    // reio_dtm(z, X_e, T) is its derivative given the RECFAST X_e at z, and
+    // tm_reio[i] receives it at zs[i] <= z_reio (entries above are unset).
+    void solve(const std::vector<double>& zs, std::vector<double>& xe, std::vector<double>& tm,
+               const std::function<double(double, double, double)>* reio_dtm = nullptr, double z_reio = 0.0,
+               std::vector<double>* tm_reio = nullptr) {
         z_He_evo_start = bisect(*this, &Recfast::end_of_saha, 3500.0, 0.0);
         z_H_He_evo_start = bisect(*this, &Recfast::end_He_evo, z_He_evo_start, 0.0);
         // outputs per stage, descending z
@@ -503,7 +510,29 @@ struct Recfast {
         double y3[3] = {x_H0_saha(z_H_He_evo_start), oHe.back()[0], oHe.back()[1]};
         auto fHH = [this](double z, const double* y, double* dy) { rates(y, z, dy); };
         std::vector<std::array<double, 3>> oHH;
-        dopri_outputs<3>(fHH, z_H_He_evo_start, y3, zHH, oHH, 1e-11, 1e-14);
+        std::vector<double> oReio;
+        if (!reio_dtm) {
+            dopri_outputs<3>(fHH, z_H_He_evo_start, y3, zHH, oHH, 1e-11, 1e-14);
+        } else {
+            // down to z_reio with three components, then four
+            std::vector<double> zhi, zlo;
+            for (double z : zHH) (z > z_reio ? zhi : zlo).push_back(z);
+            zhi.push_back(z_reio);
+            dopri_outputs<3>(fHH, z_H_He_evo_start, y3, zhi, oHH, 1e-11, 1e-14);
+            zhi.pop_back();
+            oHH.pop_back();
+            double y4[4] = {y3[0], y3[1], y3[2], y3[2]};
+            auto f4 = [this, reio_dtm](double z, const double* y, double* dy) {
+                rates(y, z, dy);
+                dy[3] = (*reio_dtm)(z, y[0] + fHe * y[1], y[3]);
+            };
+            std::vector<std::array<double, 4>> o4;
+            dopri_outputs<4>(f4, z_reio, y4, zlo, o4, 1e-11, 1e-14);
+            for (const auto& o : o4) {
+                oHH.push_back({o[0], o[1], o[2]});
+                oReio.push_back(o[3]);
+            }
+        }
         xe.resize(zs.size());
         tm.resize(zs.size());
         for (size_t i = 0; i < zs.size(); ++i) {
@@ -519,6 +548,10 @@ struct Recfast {
                 size_t j = std::lower_bound(zHH.begin(), zHH.end(), z, std::greater<double>()) - zHH.begin();
                 xe[i] = oHH[j][0] + fHe * oHH[j][1];
                 tm[i] = oHH[j][2];
+                if (tm_reio && reio_dtm && z <= z_reio) {
+                    tm_reio->resize(zs.size());
+                    (*tm_reio)[i] = oReio[j - (oHH.size() - oReio.size())];
+                }
             }
         }
     }
@@ -586,9 +619,6 @@ void Background::ionization(double Tnow_override) {
     for (int j = 0; j < nx; ++j)
         if (!(xg[j] < x_init)) zs.push_back(1.0 / std::exp(xg[j]) - 1.0);
     zs.push_back(zre_ini);
-    std::vector<double> xe_r, tm_r;
-    rf.solve(zs, xe_r, tm_r);
-    double Tm50 = tm_r.back();
     // reionization (recfast.jl:478-536)
     double X_fin = 1 + rf.Yp / (rf.not4 * (1 - rf.Yp));
     const double zre = 7.6711, alpha = 1.5, dH = 0.5, zHe = 3.5, dHe = 0.5;
@@ -600,43 +630,28 @@ void Background::ionization(double Tnow_override) {
         double xHe = fHe_r / 2 * (1 + std::tanh((zHe - z) / dHe));
         return xH + xHe;
     };
-    // the X_e(z) before reionization for z <= 50 is needed inside the Tmat ODE:
-    // tabulate it densely and interpolate with a cubic Hermite on the solution.
-    std::vector<double> zlate;
-    for (size_t i = 0; i + 1 < zs.size(); ++i) if (zs[i] <= zre_ini) zlate.push_back(zs[i]);
-    std::sort(zlate.begin(), zlate.end(), std::greater<double>());
-    std::vector<double> tm_reio(zlate.size());
-    {
-        // dense X_e table on [0, 50] for the ODE right-hand side
-        const int nd = 20001;
-        std::vector<double> zd(nd);
-        for (int i = 0; i < nd; ++i) zd[i] = zre_ini * (1.0 - (double)i / (nd - 1));
-        std::vector<double> xd, td;
-        Recfast rf2 = rf;
-        rf2.solve(zd, xd, td);
-        auto xe_at = [&](double z) {
-            double t = (zre_ini - z) / zre_ini * (nd - 1);
-            int i = (int)std::floor(t);
-            if (i < 0) i = 0;
-            if (i > nd - 4) i = nd - 4;
-            if (i < 1) i = 1;
-            // 4-point Lagrange on the uniform table
-            double s = t - i;
-            double p0 = xd[i - 1], p1 = xd[i], p2 = xd[i + 1], p3 = xd[i + 2];
-            return p1 + 0.5 * s * (p2 - p0 + s * (2 * p0 - 5 * p1 + 4 * p2 - p3 + s * (3 * (p1 - p2) + p3 - p0)));
-        };
-        auto ftm = [&](double z, const double* y, double* dy) {
-            double xr = reio_xe(z, xe_at(z));
-            double aa = 1 / (1 + z);
-            double Hz = rf.Hc(std::log(aa)) / aa / u.H0_conv;
-            double Trad = rf.Tnow * (1 + z);
-            dy[0] = rf.CT * std::pow(Trad, 4) * xr / (1 + xr + rf.fHe) * (y[0] - Trad) / (Hz * (1 + z)) +
-                    2 * y[0] / (1 + z);
-        };
-        double y1[1] = {Tm50};
-        std::vector<std::array<double, 1>> o;
-        dopri_outputs<1>(ftm, zre_ini, y1, zlate, o, 1e-11, 1e-14);
-        for (size_t i = 0; i < zlate.size(); ++i) tm_reio[i] = o[i][0];
+    // the reionized matter temperature from z = 50 (recfast.jl:513-534), its
+    // X_e taken from the RECFAST state integrated alongside
+    std::function<double(double, double, double)> dtm = [&](double z, double xe_orig, double T) {
+        double xr = reio_xe(z, xe_orig);
+        double aa = 1 / (1 + z);
+        double Hz = rf.Hc(std::log(aa)) / aa / u.H0_conv;
+        double Trad = rf.Tnow * (1 + z);
+        return rf.CT * std::pow(Trad, 4) * xr / (1 + xr + rf.fHe) * (T - Trad) / (Hz * (1 + z)) + 2 * T / (1 + z);
+    };
+    std::vector<double> xe_r, tm_r, tmre_r;
+    rf.solve(zs, xe_r, tm_r, &dtm, zre_ini, &tmre_r);
+    std::vector<double> zlate, tm_reio;
+    for (size_t i = 0; i + 1 < zs.size(); ++i)
+        if (zs[i] <= zre_ini) { zlate.push_back(zs[i]); tm_reio.push_back(tmre_r[i]); }
+    {   // zlate descending, tm_reio alongside
+        std::vector<size_t> o(zlate.size());
+        for (size_t i = 0; i < o.size(); ++i) o[i] = i;
+        std::sort(o.begin(), o.end(), [&](size_t a, size_t b) { return zlate[a] > zlate[b]; });
+        std::vector<double> z2(o.size()), t2(o.size());
+        for (size_t i = 0; i < o.size(); ++i) { z2[i] = zlate[o[i]]; t2[i] = tm_reio[o[i]]; }
+        zlate.swap(z2);
+        tm_reio.swap(t2);
     }
     // X_e(x), Tmat(x) on the grid (recfast.jl:692-696)
     std::vector<double> Xe(nx), Tm(nx);
