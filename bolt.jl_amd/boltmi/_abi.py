@@ -45,6 +45,21 @@ class StatsC(ctypes.Structure):
                 ("linear_solves", _I64), ("kernel_ms", _D)]
 
 
+REQ_PK, REQ_CLTT = 0, 1
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+
+
+class RequestC(ctypes.Structure):
+    _fields_ = [("par", CosmoParamsC), ("kind", _I), ("nk", _I), ("kmin", _D), ("kmax", _D), ("k", _PD),
+                ("ell_max", _I),
+                ("nkd", _I), ("cfg", HierCfgC), ("out", _PD)]
+
+
+class ResultC(ctypes.Structure):
+    _fields_ = [("state", _I), ("status", _I), ("queue_ms", _D), ("host_ms", _D), ("device_ms", _D),
+                ("latency_ms", _D)]
+
+
 # every exported symbol and its signature: (restype, argtypes)
 SIGNATURES = {
     "bolt_abi_version": (_I, []),
@@ -82,6 +97,12 @@ SIGNATURES = {
                          ctypes.POINTER(ctypes.c_float)]),
     "bolt_cltt_dev": (_I, [_VP, _I, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, ctypes.POINTER(ctypes.c_float)]),
     "bolt_plin": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PI, ctypes.POINTER(StatsC)]),
+    "bolt_serve_start": (_I, [_VP, _I, _D]),
+    "bolt_serve_slots": (_I, [_VP]),
+    "bolt_submit": (_I, [_VP, ctypes.POINTER(RequestC), ctypes.POINTER(ctypes.c_uint64)]),
+    "bolt_poll": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),
+    "bolt_wait": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),
+    "bolt_serve_stop": (_I, [_VP]),
 }
 
 _lib = None

@@ -497,3 +497,68 @@ def plin(k, par, bg, ih, n_q=15, ℓᵧ=50, ℓ_ν=50, ℓ_mν=20, x=0.0, reltol
         check(lib().bolt_plin(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), float(x), pd(pk),
                               pi(status), ctypes.byref(st)))
     return pk if np.ndim(k) else float(pk[0])
+
+
+class Server:
+    """Continuous batching of spectrum requests on one context.
+
+    Each request carries its own cosmology; the library builds its tables on
+    the host and runs it on a free slot (one HIP stream with preallocated HBM).
+    Requests on different slots run concurrently on the device, and each
+    result is the same as the computation run alone (plin / source_grid +
+    cltt_on_grid on the same k grid)."""
+
+    def __init__(self, slots=4, hbm_gib=8.0, ctx=None):
+        self.ctx = ctx or default_context()
+        check(lib().bolt_serve_start(self.ctx.handle, int(slots), float(hbm_gib)))
+        self.slots = lib().bolt_serve_slots(self.ctx.handle)
+        self._out = {}
+
+    def _submit(self, par, kind, nk, kmin, kmax, k, ell_max, nkd, cfg, nout):
+        out = np.zeros(nout)
+        kk = None if k is None else np.ascontiguousarray(k, dtype=np.float64)
+        if kk is not None:
+            nk = len(kk)
+        req = _abi.RequestC(par.c_struct(), kind, int(nk), float(kmin), float(kmax),
+                            None if kk is None else pd(kk), int(ell_max), int(nkd), cfg, pd(out))
+        rid = ctypes.c_uint64()
+        check(lib().bolt_submit(self.ctx.handle, ctypes.byref(req), ctypes.byref(rid)))
+        self._out[rid.value] = (out, kk)
+        return rid.value
+
+    def submit_pk(self, par, nk=200, kmin=1e-4, kmax=0.5, n_q=15, ℓᵧ=50, ℓ_ν=50, ℓ_mν=20, reltol=1e-5,
+                  abstol=1e-6, k=None):
+        """plin at x = 0 (spectra.jl:163-198 defaults) on nk log-spaced k, or on the given k."""
+        return self._submit(par, _abi.REQ_PK, nk, kmin, kmax, k, 0, 0, hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, n_q, reltol, abstol),
+                            nk if k is None else len(k))
+
+    def submit_cltt(self, par, nk=800, kmin=1e-4, kmax=0.5, ell_max=2500, nkd=5000, ℓᵧ=50, ℓ_ν=50, ℓ_mν=10, nq=15,
+                    reltol=1e-11, abstol=1e-6, k=None):
+        """source_grid on nk log-spaced k (or the given k), then cltt for ℓ = 2..ell_max."""
+        return self._submit(par, _abi.REQ_CLTT, nk, kmin, kmax, k, ell_max, nkd,
+                            hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol), ell_max - 1)
+
+    @staticmethod
+    def _info(r):
+        return dict(state=r.state, status=r.status, queue_ms=r.queue_ms, host_ms=r.host_ms, device_ms=r.device_ms,
+                    latency_ms=r.latency_ms)
+
+    def poll(self, rid):
+        r = _abi.ResultC()
+        check(lib().bolt_poll(self.ctx.handle, rid, ctypes.byref(r)))
+        return self._info(r)
+
+    def wait(self, rid):
+        """(values, info) once the request is done; raises if it failed."""
+        r = _abi.ResultC()
+        check(lib().bolt_wait(self.ctx.handle, rid, ctypes.byref(r)))
+        info = self._info(r)
+        out = self._out.pop(rid)[0]
+        if r.status != 0:
+            raise _abi.BoltError(r.status, f"request {rid} failed")
+        return out, info
+
+    def close(self):
+        if self.ctx is not None and self.ctx.handle:
+            lib().bolt_serve_stop(self.ctx.handle)
+        self._out.clear()

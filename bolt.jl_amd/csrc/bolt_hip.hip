@@ -18,14 +18,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bolt_hip.h"
+#include "host_tables.h"
 
 namespace bolt {
 int set_error(int code, const std::string& msg);
@@ -1499,6 +1505,9 @@ struct CosmoSlot {
     CosmoDev* d_self = nullptr;
 };
 
+struct Server;
+static void server_stop(Server* sv);
+
 struct bolt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1506,6 +1515,8 @@ struct bolt_ctx {
     int lds_max = 64 * 1024;
     std::vector<CosmoSlot> cosmos;
     std::mutex mu;
+    Server* server = nullptr;  // request server (bolt_serve_start), if running
+    std::mutex server_mu;
 };
 
 static int check_cfg(const bolt_hier_cfg* c) {
@@ -1575,6 +1586,11 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
 
 int bolt_ctx_destroy(bolt_ctx* ctx) {
     if (!ctx) return BOLT_OK;
+    {
+        std::lock_guard<std::mutex> lk(ctx->server_mu);
+        if (ctx->server) server_stop(ctx->server);
+        ctx->server = nullptr;
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& s : ctx->cosmos) {
@@ -1594,6 +1610,50 @@ int bolt_ctx_sync(bolt_ctx* ctx) {
     return BOLT_OK;
 }
 
+}  // extern "C"
+
+// the device-side description of one cosmology (parameters, scalars, q nodes
+// and weights); the table pointers are bound by bind_cosmo_device
+static void make_cosmo_host(const bolt_cosmo_params& par, const bolt_tables& t, CosmoDev& C,
+                            std::vector<double>& qbuf) {
+    C.h = par.h; C.Or = par.Omega_r; C.Ob = par.Omega_b; C.Oc = par.Omega_c; C.A = par.A; C.ns = par.n;
+    C.Yp = par.Y_p; C.Nnu = par.N_nu; C.mnu = par.Sum_m_nu;
+    C.H0 = t.scal[BOLT_S_H0]; C.eta0 = t.scal[BOLT_S_ETA0]; C.rho_crit = t.scal[BOLT_S_RHO_CRIT];
+    C.Onu = 7.0 * (2.0 / 3.0) * par.N_nu / 8.0 * std::pow(4.0 / 11.0, 4.0 / 3.0) * par.Omega_r;
+    C.x0 = t.x0; C.x1 = t.x1; C.dx = (t.x1 - t.x0) / (t.nx - 1); C.nx = t.nx; C.nq = t.nq;
+    C.Pprim_A = par.A; C.Pprim_n = par.n;
+    {   // plin's Ω_ν (spectra.jl:182-186)
+        double Tg = std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par.Omega_r, 0.25);
+        double nufac = (90.0 * 1.2020569 / (11.0 * std::pow(M_PI, 4))) * (par.Omega_r * par.h * par.h / Tg) *
+                       std::pow(par.N_nu / 3.0, 0.75);
+        C.Om_nu_nr = par.Sum_m_nu * nufac / (par.h * par.h);
+    }
+    // q nodes and weights (perturbations.jl:127-158 with util.jl:24-27)
+    int nq = t.nq;
+    qbuf.assign(4 * (size_t)nq, 0.0);
+    double Tnu = std::pow(par.N_nu / 3.0, 0.25) * std::pow(4.0 / 11.0, 1.0 / 3.0) *
+                 std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par.Omega_r, 0.25);
+    double lo = std::log10(Tnu / 30.0), hi = std::log10(Tnu * 30.0);
+    for (int i = 0; i < nq; ++i) {
+        double q = std::pow(10.0, lo + (hi - lo) / 2.0 * (t.xq[i] + 1.0));
+        double tu = -1.0 + 2.0 / (hi - lo) * (1.0 + lo - lo);
+        double dxdq = (1.0 + tu) / (q * std::log(10.0));
+        double f0 = 2.0 / std::pow(2.0 * M_PI, 3) / (std::exp(q / Tnu) + 1.0);
+        qbuf[i] = q;
+        qbuf[nq + i] = -q / Tnu / (1.0 + std::exp(-q / Tnu));
+        qbuf[2 * nq + i] = 4.0 * M_PI * q * q * f0 / dxdq * t.wq[i];
+        qbuf[3 * nq + i] = 4.0 * M_PI * q * q * q * f0 / dxdq * t.wq[i];
+    }
+}
+
+// point C at device copies of the tables (coef) and the q arrays (qbuf)
+static void bind_cosmo_device(CosmoDev& C, double* d_coef, double* d_q) {
+    const int nq = C.nq;
+    C.coef = d_coef; C.qv = d_q; C.df0 = d_q + nq; C.base = d_q + 2 * nq; C.thbase = d_q + 3 * nq;
+}
+
+extern "C" {
+
 int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_tables* t, int* cosmo_id) {
     if (!ctx || !par || !t || !cosmo_id || !t->coef || !t->xq || !t->wq)
         return set_error(BOLT_E_ARG, "bolt_cosmo_upload: NULL argument");
@@ -1603,35 +1663,8 @@ int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_ta
     HIPCHK(hipSetDevice(ctx->device));
     CosmoSlot s;
     s.used = true;
-    CosmoDev& C = s.host;
-    C.h = par->h; C.Or = par->Omega_r; C.Ob = par->Omega_b; C.Oc = par->Omega_c; C.A = par->A; C.ns = par->n;
-    C.Yp = par->Y_p; C.Nnu = par->N_nu; C.mnu = par->Sum_m_nu;
-    C.H0 = t->scal[BOLT_S_H0]; C.eta0 = t->scal[BOLT_S_ETA0]; C.rho_crit = t->scal[BOLT_S_RHO_CRIT];
-    C.Onu = 7.0 * (2.0 / 3.0) * par->N_nu / 8.0 * std::pow(4.0 / 11.0, 4.0 / 3.0) * par->Omega_r;
-    C.x0 = t->x0; C.x1 = t->x1; C.dx = (t->x1 - t->x0) / (t->nx - 1); C.nx = t->nx; C.nq = t->nq;
-    C.Pprim_A = par->A; C.Pprim_n = par->n;
-    {   // plin's Ω_ν (spectra.jl:182-186)
-        double Tg = std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par->Omega_r, 0.25);
-        double nufac = (90.0 * 1.2020569 / (11.0 * std::pow(M_PI, 4))) * (par->Omega_r * par->h * par->h / Tg) *
-                       std::pow(par->N_nu / 3.0, 0.75);
-        C.Om_nu_nr = par->Sum_m_nu * nufac / (par->h * par->h);
-    }
-    // q nodes and weights (perturbations.jl:127-158 with util.jl:24-27)
-    int nq = t->nq;
-    std::vector<double> qbuf(4 * nq);
-    double Tnu = std::pow(par->N_nu / 3.0, 0.25) * std::pow(4.0 / 11.0, 1.0 / 3.0) *
-                 std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par->Omega_r, 0.25);
-    double lo = std::log10(Tnu / 30.0), hi = std::log10(Tnu * 30.0);
-    for (int i = 0; i < nq; ++i) {
-        double q = std::pow(10.0, lo + (hi - lo) / 2.0 * (t->xq[i] + 1.0));
-        double tu = -1.0 + 2.0 / (hi - lo) * (1.0 + lo - lo);
-        double dxdq = (1.0 + tu) / (q * std::log(10.0));
-        double f0 = 2.0 / std::pow(2.0 * M_PI, 3) / (std::exp(q / Tnu) + 1.0);
-        qbuf[i] = q;
-        qbuf[nq + i] = -q / Tnu / (1.0 + std::exp(-q / Tnu));
-        qbuf[2 * nq + i] = 4.0 * M_PI * q * q * f0 / dxdq * t->wq[i];
-        qbuf[3 * nq + i] = 4.0 * M_PI * q * q * q * f0 / dxdq * t->wq[i];
-    }
+    std::vector<double> qbuf;
+    make_cosmo_host(*par, *t, s.host, qbuf);
     size_t ncoef = (size_t)BOLT_NTAB * (t->nx + 2);
     if (hipMalloc(&s.d_coef, ncoef * sizeof(double)) != hipSuccess ||
         hipMalloc(&s.d_q, qbuf.size() * sizeof(double)) != hipSuccess ||
@@ -1642,8 +1675,8 @@ int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_ta
     }
     HIPCHK(hipMemcpy(s.d_coef, t->coef, ncoef * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s.d_q, qbuf.data(), qbuf.size() * sizeof(double), hipMemcpyHostToDevice));
-    C.coef = s.d_coef; C.qv = s.d_q; C.df0 = s.d_q + nq; C.base = s.d_q + 2 * nq; C.thbase = s.d_q + 3 * nq;
-    HIPCHK(hipMemcpy(s.d_self, &C, sizeof(CosmoDev), hipMemcpyHostToDevice));
+    bind_cosmo_device(s.host, s.d_coef, s.d_q);
+    HIPCHK(hipMemcpy(s.d_self, &s.host, sizeof(CosmoDev), hipMemcpyHostToDevice));
     int id = -1;
     for (size_t i = 0; i < ctx->cosmos.size(); ++i) if (!ctx->cosmos[i].used) { id = (int)i; break; }
     if (id < 0) { id = (int)ctx->cosmos.size(); ctx->cosmos.push_back(s); } else ctx->cosmos[id] = s;
@@ -1665,9 +1698,23 @@ int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id) {
 
 }  // extern "C"
 
-// the persistent integrator over a mode list whose modes may differ in
-// cosmology and hierarchy; modes are sorted by k, largest (most steps) first
-static int launch_modes(bolt_ctx* ctx, const std::vector<CosmoDev>& cosmos, const std::vector<CfgDev>& cfgs,
+// device scratch of one solver launch; a caller that must not allocate per
+// call (the serving slots) passes its own
+struct SolveBufs {
+    ModeDesc* modes = nullptr;
+    CfgDev* cfgs = nullptr;
+    CosmoDev* cosmos = nullptr;
+    int* counter = nullptr;
+    int cap_modes = 0, cap_cfgs = 0, cap_cosmos = 0;
+};
+
+// the integrator over a mode list whose modes may differ in cosmology and
+// hierarchy; modes are sorted by k, largest (most steps) first.  persistent:
+// a grid of at most one resident workgroup per slot pulls modes from the work
+// counter; otherwise one workgroup per mode, so that launches from several
+// streams interleave on the CUs.
+static int launch_modes(bolt_ctx* ctx, hipStream_t stream, const SolveBufs* pre, bool persistent,
+                        const std::vector<CosmoDev>& cosmos, const std::vector<CfgDev>& cfgs,
                         std::vector<ModeDesc>& md, int out_mode, const double* d_xsave, int nsave, double* d_out,
                         double* d_out2, int* d_status, long long* d_counters, int n_stride, double* kernel_ms) {
     const int B = (int)md.size();
@@ -1676,21 +1723,29 @@ static int launch_modes(bolt_ctx* ctx, const std::vector<CosmoDev>& cosmos, cons
     for (const CfgDev& c : cfgs) lds = std::max(lds, ws_bytes(c.n, c.nq));
     if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
     DBuf dmodes, dcfg, dcos, dctr;
-    HIPCHK(dmodes.alloc(sizeof(ModeDesc) * B));
-    HIPCHK(dcfg.alloc(sizeof(CfgDev) * cfgs.size()));
-    HIPCHK(dcos.alloc(sizeof(CosmoDev) * cosmos.size()));
-    HIPCHK(dctr.alloc(sizeof(int)));
-    HIPCHK(hipMemcpyAsync(dmodes.p, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dcfg.p, cfgs.data(), sizeof(CfgDev) * cfgs.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dcos.p, cosmos.data(), sizeof(CosmoDev) * cosmos.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-    HIPCHK(hipMemsetAsync(dctr.p, 0, sizeof(int), ctx->stream));
+    SolveBufs b;
+    if (pre) {
+        if (B > pre->cap_modes || (int)cfgs.size() > pre->cap_cfgs || (int)cosmos.size() > pre->cap_cosmos)
+            return set_error(BOLT_E_ARG, "launch exceeds the preallocated solver buffers");
+        b = *pre;
+    } else {
+        HIPCHK(dmodes.alloc(sizeof(ModeDesc) * B));
+        HIPCHK(dcfg.alloc(sizeof(CfgDev) * cfgs.size()));
+        HIPCHK(dcos.alloc(sizeof(CosmoDev) * cosmos.size()));
+        HIPCHK(dctr.alloc(sizeof(int)));
+        b.modes = dmodes.as<ModeDesc>(); b.cfgs = dcfg.as<CfgDev>(); b.cosmos = dcos.as<CosmoDev>();
+        b.counter = dctr.as<int>();
+    }
+    HIPCHK(hipMemcpyAsync(b.modes, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(b.cfgs, cfgs.data(), sizeof(CfgDev) * cfgs.size(), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(b.cosmos, cosmos.data(), sizeof(CosmoDev) * cosmos.size(), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemsetAsync(b.counter, 0, sizeof(int), stream));
     SolveArgs A;
-    A.cosmos = dcos.as<CosmoDev>();
-    A.cfgs = dcfg.as<CfgDev>();
-    A.modes = dmodes.as<ModeDesc>();
+    A.cosmos = b.cosmos;
+    A.cfgs = b.cfgs;
+    A.modes = b.modes;
     A.nmodes = B;
-    A.work_counter = dctr.as<int>();
+    A.work_counter = b.counter;
     A.out_mode = out_mode;
     A.x_save = d_xsave;
     A.nsave = nsave;
@@ -1699,17 +1754,20 @@ static int launch_modes(bolt_ctx* ctx, const std::vector<CosmoDev>& cosmos, cons
     A.status = d_status;
     A.counters = d_counters;
     A.n_state_stride = n_stride;
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, NTHR, lds));
-    if (per_cu < 1) per_cu = 1;
-    int grid = std::min(B, per_cu * ctx->n_cu);
+    int grid = B;
+    if (persistent) {
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, NTHR, lds));
+        if (per_cu < 1) per_cu = 1;
+        grid = std::min(B, per_cu * ctx->n_cu);
+    }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(NTHR), lds, ctx->stream, A);
+    HIPCHK(hipEventRecord(e0, stream));
+    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(NTHR), lds, stream, A);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventRecord(e1, stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
@@ -1742,8 +1800,8 @@ static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, 
     if (r) return r;
     std::vector<ModeDesc> md(B);
     for (int i = 0; i < B; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
-    return launch_modes(ctx, {cs->host}, {to_cfgdev(cfg)}, md, out_mode, d_xsave, nsave, d_out, d_out2, d_status,
-                        d_counters, n_stride, kernel_ms);
+    return launch_modes(ctx, ctx->stream, nullptr, true, {cs->host}, {to_cfgdev(cfg)}, md, out_mode, d_xsave, nsave,
+                        d_out, d_out2, d_status, d_counters, n_stride, kernel_ms);
 }
 
 static void fill_stats(bolt_stats* st, const std::vector<long long>& ctr, int B, double ms) {
@@ -2005,7 +2063,8 @@ static int source_grid_batch_core(bolt_ctx* ctx, int ncosmo, const int* cosmo_id
             m.k = kh[(size_t)c * nk + i]; m.cosmo = c; m.cfg = c; m.out = c * nk + i; m.pad = 0;
         }
     const int mode = (d_ST ? 1 : 0) | (d_SP ? 2 : 0);
-    return launch_modes(ctx, cos, cf, md, mode, nullptr, 0, d_ST, d_SP, d_status, d_counters, 0, ms);
+    return launch_modes(ctx, ctx->stream, nullptr, true, cos, cf, md, mode, nullptr, 0, d_ST, d_SP, d_status,
+                        d_counters, 0, ms);
 }
 
 extern "C" {
@@ -2103,9 +2162,18 @@ int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const doubl
 // C_ℓ core on device buffers; ells given on the host (validated there).
 // d_S: temperature source or NULL, d_SP: polarization source or NULL; outputs
 // d_tt (needs d_S), d_te (needs both), d_ee (needs d_SP), each may be NULL.
-static int cl_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double* d_SP, const double* d_k_src,
-                   int nk_src, const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode,
-                   double* d_tt, double* d_te, double* d_ee, long long* d_steps, float* ms_out) {
+struct LosBufs {
+    int* lidx = nullptr;
+    double* etaj = nullptr;
+    double* dxj = nullptr;
+    double* contrib = nullptr;
+    size_t cap_lidx = 0, cap_nxi = 0, cap_contrib = 0;
+};
+
+static int cl_core(bolt_ctx* ctx, hipStream_t stream, const LosBufs* pre, const CosmoDev& host,
+                   const CosmoDev* d_cosmo, const double* d_S, const double* d_SP, const double* d_k_src, int nk_src,
+                   const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode, double* d_tt,
+                   double* d_te, double* d_ee, long long* d_steps, float* ms_out) {
     if (bessel_mode != 0) return set_error(BOLT_E_ARG, "bessel_mode 1 (spline table) is not implemented on device");
     if ((d_tt && !d_S) || (d_te && (!d_S || !d_SP)) || (d_ee && !d_SP))
         return set_error(BOLT_E_ARG, "a requested spectrum lacks its source grid");
@@ -2118,10 +2186,10 @@ static int cl_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double
         if (l < 0 || l > 20000) return set_error(BOLT_E_ARG, "ℓ out of range [0, 20000]");
         lmax = std::max(lmax, l);
     }
-    int nx = cs->host.nx;
+    int nx = host.nx;
     int i0 = -1;
     for (int j = 0; j < nx; ++j) {
-        double xj = (cs->host.x0 * (nx - 1 - j) + cs->host.x1 * j) / (nx - 1);
+        double xj = (host.x0 * (nx - 1 - j) + host.x1 * j) / (nx - 1);
         if (xj > -8.0) { i0 = j; break; }
     }
     if (i0 < 0 || i0 >= nx - 1) return set_error(BOLT_E_ARG, "x grid has no node after x = -8");
@@ -2129,38 +2197,45 @@ static int cl_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double
     if (nxi > LOS_SLOTS * WAVE) return set_error(BOLT_E_ARG, "too many integration nodes for the LOS kernel");
     std::vector<int> lidx(lmax + 1, -1);
     for (int e = 0; e < nell; ++e) lidx[ells[e]] = e;
-    DBuf dl, deta, ddx, dcon;
-    HIPCHK(dl.alloc(sizeof(int) * (lmax + 1)));
-    HIPCHK(deta.alloc(sizeof(double) * nxi));
-    HIPCHK(ddx.alloc(sizeof(double) * nxi));
     const int nout = two ? 3 : 1;
-    HIPCHK(dcon.alloc(sizeof(double) * (size_t)nout * nell * (nkd - 1)));
-    HIPCHK(hipMemcpyAsync(dl.p, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, ctx->stream, cs->d_self, i0, nxi,
-                       deta.as<double>(), ddx.as<double>());
+    DBuf dl, deta, ddx, dcon;
+    LosBufs lb;
+    const size_t ncon = (size_t)nout * nell * (nkd - 1);
+    if (pre) {
+        if ((size_t)(lmax + 1) > pre->cap_lidx || (size_t)nxi > pre->cap_nxi || ncon > pre->cap_contrib)
+            return set_error(BOLT_E_ARG, "C_l request exceeds the preallocated LOS buffers");
+        lb = *pre;
+    } else {
+        HIPCHK(dl.alloc(sizeof(int) * (lmax + 1)));
+        HIPCHK(deta.alloc(sizeof(double) * nxi));
+        HIPCHK(ddx.alloc(sizeof(double) * nxi));
+        HIPCHK(dcon.alloc(sizeof(double) * ncon));
+        lb.lidx = dl.as<int>(); lb.etaj = deta.as<double>(); lb.dxj = ddx.as<double>(); lb.contrib = dcon.as<double>();
+    }
+    HIPCHK(hipMemcpyAsync(lb.lidx, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, stream, d_cosmo, i0, nxi, lb.etaj, lb.dxj);
     HIPCHK(hipGetLastError());
     LosArgs A;
-    A.cosmo = cs->d_self; A.k_src = d_k_src; A.nk_src = nk_src; A.kgrid = d_kgrid; A.nkd = nkd;
+    A.cosmo = d_cosmo; A.k_src = d_k_src; A.nk_src = nk_src; A.kgrid = d_kgrid; A.nkd = nkd;
     A.S = two ? d_S : (d_tt ? d_S : d_SP);
     A.SP = two ? d_SP : nullptr;
     A.spin = (!two && !d_tt) ? 1 : 0;
-    A.lidx = dl.as<int>(); A.lmax = lmax; A.nell = nell; A.etaj = deta.as<double>(); A.dxj = ddx.as<double>();
-    A.i0 = i0; A.nxi = nxi; A.contrib = dcon.as<double>(); A.steps = d_steps;
+    A.lidx = lb.lidx; A.lmax = lmax; A.nell = nell; A.etaj = lb.etaj; A.dxj = lb.dxj;
+    A.i0 = i0; A.nxi = nxi; A.contrib = lb.contrib; A.steps = d_steps;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     int nkm = nkd - 1;
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    if (two) hipLaunchKernelGGL(k_los<2>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
-    else hipLaunchKernelGGL(k_los<1>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, ctx->stream, A);
+    HIPCHK(hipEventRecord(e0, stream));
+    if (two) hipLaunchKernelGGL(k_los<2>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, stream, A);
+    else hipLaunchKernelGGL(k_los<1>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, stream, A);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventRecord(e1, stream));
     if (two)
-        hipLaunchKernelGGL(k_los_reduce, dim3(3 * nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell, d_tt,
-                           d_te, d_ee);
+        hipLaunchKernelGGL(k_los_reduce, dim3(3 * nell), dim3(256), 0, stream, lb.contrib, nkm, nell, d_tt, d_te, d_ee);
     else
-        hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, ctx->stream, dcon.as<double>(), nkm, nell,
-                           d_tt ? d_tt : d_ee, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, stream, lb.contrib, nkm, nell, d_tt ? d_tt : d_ee,
+                           (double*)nullptr, (double*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -2168,7 +2243,7 @@ static int cl_core(bolt_ctx* ctx, CosmoSlot* cs, const double* d_S, const double
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms_out) *ms_out = ms;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(stream));
     return BOLT_OK;
 }
 
@@ -2195,7 +2270,8 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
     HIPCHK(hipMemcpyAsync(dks.p, k_src, sizeof(double) * nk_src, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
     std::vector<int> ev(ells, ells + nell);
-    if ((r = cl_core(ctx, cs, dS.as<double>(), nullptr, dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
+    if ((r = cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, dS.as<double>(), nullptr, dks.as<double>(),
+                     nk_src, dkg.as<double>(), nkd, ev,
                      bessel_mode, dcl.as<double>(), nullptr, nullptr, nullptr, nullptr)))
         return r;
     HIPCHK(hipMemcpy(cl, dcl.p, sizeof(double) * nell, hipMemcpyDeviceToHost));
@@ -2215,7 +2291,8 @@ int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* 
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<int> ev(ells, ells + nell);
-    return cl_core(ctx, cs, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_cl, nullptr, nullptr,
+    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd, ev,
+                   0, d_cl, nullptr, nullptr,
                    (long long*)d_steps, ms);
 }
 
@@ -2243,7 +2320,8 @@ int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, c
     HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
     std::vector<int> ev(ells, ells + nell);
     double* d = dcl.as<double>();
-    if ((r = cl_core(ctx, cs, dS.as<double>(), dP.as<double>(), dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
+    if ((r = cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, dS.as<double>(), dP.as<double>(),
+                     dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
                      bessel_mode, cltt ? d : nullptr, clte ? d + nell : nullptr, clee ? d + 2 * nell : nullptr,
                      nullptr, nullptr)))
         return r;
@@ -2264,8 +2342,355 @@ int bolt_cl_dev(bolt_ctx* ctx, int cosmo_id, const double* d_ST, const double* d
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<int> ev(ells, ells + nell);
-    return cl_core(ctx, cs, d_ST, d_SP, d_k_src, nk_src, d_kgrid, nkd, ev, 0, d_tt, d_te, d_ee, (long long*)d_steps,
+    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_ST, d_SP, d_k_src, nk_src, d_kgrid, nkd, ev, 0,
+                   d_tt, d_te, d_ee, (long long*)d_steps,
                    ms);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving
+// Continuous batching of spectrum requests.  Each slot owns a HIP stream and
+// preallocated device buffers sized for the largest request (no allocation
+// per request: hipMalloc/hipFree synchronise the device and would serialise
+// the slots).  A worker thread per slot takes the next queued request, builds
+// its background/RECFAST tables on the host, uploads them into the slot and
+// runs the solver with one workgroup per k-mode, so that the kernels of
+// requests on different slots interleave on the CUs as workgroups retire.
+// Admission: a request starts as soon as a slot is free; the number of slots
+// is bounded by the HBM budget given at start.
+namespace {
+constexpr int SV_NX = 2001;            // x grid -20:0.01:0 (background.jl:86)
+constexpr int SV_MAX_NK = 800;
+constexpr int SV_MAX_ELL = 2500;
+constexpr int SV_MAX_NKD = 5000;
+constexpr int SV_MAX_N = 1024;         // state length
+constexpr int SV_MAX_NQ = 48;
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+struct ServeSlot {
+    hipStream_t stream = nullptr;
+    double *d_coef = nullptr, *d_q = nullptr, *d_k = nullptr, *d_S = nullptr, *d_dense = nullptr, *d_cl = nullptr;
+    double *d_x = nullptr, *d_states = nullptr, *d_pk = nullptr;
+    CosmoDev* d_self = nullptr;
+    int* d_status = nullptr;
+    long long* d_ctr = nullptr;
+    SolveBufs sb;
+    LosBufs lb;
+    std::vector<void*> allocs;
+};
+
+static size_t slot_bytes() {
+    size_t b = 0;
+    b += sizeof(double) * ((size_t)BOLT_NTAB * (SV_NX + 2) + 4 * SV_MAX_NQ);             // tables, q arrays
+    b += sizeof(double) * ((size_t)SV_MAX_NK * SV_NX + SV_MAX_NK + SV_MAX_NKD + SV_MAX_ELL + 1 + SV_MAX_NK);
+    b += sizeof(double) * (size_t)SV_MAX_NK * SV_MAX_N;                                  // plin states
+    b += sizeof(double) * (size_t)(SV_MAX_ELL - 1) * (SV_MAX_NKD - 1);                   // LOS contributions
+    b += sizeof(double) * 2 * (size_t)LOS_SLOTS * WAVE + sizeof(int) * (SV_MAX_ELL + 1);
+    b += (sizeof(int) + 4 * sizeof(long long) + sizeof(ModeDesc)) * SV_MAX_NK;
+    return b;
+}
+
+struct ServeJob {
+    uint64_t id;
+    bolt_request req;
+    double t_submit;
+};
+
+struct Server {
+    bolt_ctx* ctx = nullptr;
+    std::vector<ServeSlot> slots;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_job, cv_done;
+    std::deque<ServeJob> queue;
+    std::unordered_map<uint64_t, bolt_result> results;
+    uint64_t next_id = 1;
+    bool stopping = false;
+};
+
+template <class T>
+static hipError_t slot_alloc(ServeSlot& s, T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) s.allocs.push_back((void*)*p);
+    return e;
+}
+
+static int slot_create(ServeSlot& s) {
+    HIPCHK(hipStreamCreate(&s.stream));
+    HIPCHK(slot_alloc(s, &s.d_coef, (size_t)BOLT_NTAB * (SV_NX + 2)));
+    HIPCHK(slot_alloc(s, &s.d_q, 4 * SV_MAX_NQ));
+    HIPCHK(slot_alloc(s, &s.d_self, 1));
+    HIPCHK(slot_alloc(s, &s.d_k, SV_MAX_NK));
+    HIPCHK(slot_alloc(s, &s.d_S, (size_t)SV_MAX_NK * SV_NX));
+    HIPCHK(slot_alloc(s, &s.d_dense, SV_MAX_NKD));
+    HIPCHK(slot_alloc(s, &s.d_cl, SV_MAX_ELL));
+    HIPCHK(slot_alloc(s, &s.d_x, 1));
+    HIPCHK(slot_alloc(s, &s.d_states, (size_t)SV_MAX_NK * SV_MAX_N));
+    HIPCHK(slot_alloc(s, &s.d_pk, SV_MAX_NK));
+    HIPCHK(slot_alloc(s, &s.d_status, SV_MAX_NK));
+    HIPCHK(slot_alloc(s, &s.d_ctr, 4 * SV_MAX_NK));
+    HIPCHK(slot_alloc(s, &s.sb.modes, SV_MAX_NK));
+    HIPCHK(slot_alloc(s, &s.sb.cfgs, 1));
+    HIPCHK(slot_alloc(s, &s.sb.cosmos, 1));
+    HIPCHK(slot_alloc(s, &s.sb.counter, 1));
+    s.sb.cap_modes = SV_MAX_NK; s.sb.cap_cfgs = 1; s.sb.cap_cosmos = 1;
+    HIPCHK(slot_alloc(s, &s.lb.lidx, SV_MAX_ELL + 1));
+    HIPCHK(slot_alloc(s, &s.lb.etaj, LOS_SLOTS * WAVE));
+    HIPCHK(slot_alloc(s, &s.lb.dxj, LOS_SLOTS * WAVE));
+    HIPCHK(slot_alloc(s, &s.lb.contrib, (size_t)(SV_MAX_ELL - 1) * (SV_MAX_NKD - 1)));
+    s.lb.cap_lidx = SV_MAX_ELL + 1; s.lb.cap_nxi = LOS_SLOTS * WAVE;
+    s.lb.cap_contrib = (size_t)(SV_MAX_ELL - 1) * (SV_MAX_NKD - 1);
+    return BOLT_OK;
+}
+
+static void slot_free(ServeSlot& s) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (void* p : s.allocs) (void)hipFree(p);
+    s.allocs.clear();
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+}
+
+static int validate_request(const bolt_request* r) {
+    if (!r || !r->out) return set_error(BOLT_E_ARG, "bolt_submit: NULL request or output");
+    if (r->kind != BOLT_REQ_PK && r->kind != BOLT_REQ_CLTT) return set_error(BOLT_E_ARG, "bolt_submit: unknown kind");
+    if (r->nk < 2 || r->nk > SV_MAX_NK) return set_error(BOLT_E_ARG, "bolt_submit: nk must be in [2, 800]");
+    if (r->k) {
+        for (int i = 0; i < r->nk; ++i)
+            if (!(r->k[i] > 0) || !std::isfinite(r->k[i]) || (i > 0 && !(r->k[i] > r->k[i - 1])))
+                return set_error(BOLT_E_ARG, "bolt_submit: k must be positive, finite and increasing");
+    } else if (!(r->kmin > 0) || !(r->kmax > r->kmin) || !std::isfinite(r->kmax)) {
+        return set_error(BOLT_E_ARG, "bolt_submit: need 0 < kmin < kmax");
+    }
+    if (r->kind == BOLT_REQ_CLTT && (r->ell_max < 2 || r->ell_max > SV_MAX_ELL || r->nkd < 2 || r->nkd > SV_MAX_NKD))
+        return set_error(BOLT_E_ARG, "bolt_submit: need 2 <= ell_max <= 2500 and 2 <= nkd <= 5000");
+    int rc = check_cfg(&r->cfg);
+    if (rc) return rc;
+    if (r->cfg.nq > SV_MAX_NQ || bolt_state_length(&r->cfg) > SV_MAX_N)
+        return set_error(BOLT_E_ARG, "bolt_submit: hierarchy larger than a serving slot holds");
+    if (!(r->par.h > 0) || !(r->par.Omega_r > 0) || !(r->par.N_nu > 0))
+        return set_error(BOLT_E_ARG, "bolt_submit: Omega_r, N_nu and h must be positive");
+    return BOLT_OK;
+}
+
+// one request on one slot; returns its status, *host_ms / *device_ms the
+// host-table and device times
+static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, double* host_ms, double* device_ms) {
+    const double t0 = now_ms();
+    std::vector<double> coef, xq, wq, scal(BOLT_NSCAL);
+    try {
+        bolt::Background bg(rq.par, -20.0, 0.0, SV_NX, rq.cfg.nq);
+        bg.ionization(-1.0);
+        coef = bg.coef;
+        scal[BOLT_S_H0] = bg.H0; scal[BOLT_S_ETA0] = bg.eta0; scal[BOLT_S_RHO_CRIT] = bg.rho_crit;
+        scal[BOLT_S_OMEGA_L] = bg.Omega_L; scal[BOLT_S_TAU0] = bg.tau0;
+        xq.assign(bg.cosmo.xq.begin(), bg.cosmo.xq.end());
+        wq.assign(bg.cosmo.wq.begin(), bg.cosmo.wq.end());
+    } catch (const std::exception& e) {
+        return set_error(BOLT_E_NUMERIC, std::string("host tables: ") + e.what());
+    }
+    for (double v : coef) if (!std::isfinite(v)) return set_error(BOLT_E_NUMERIC, "host tables: non-finite entry");
+    bolt_tables t;
+    t.nx = SV_NX; t.nq = rq.cfg.nq; t.x0 = -20.0; t.x1 = 0.0;
+    for (int i = 0; i < BOLT_NSCAL; ++i) t.scal[i] = scal[i];
+    t.xq = xq.data(); t.wq = wq.data(); t.coef = coef.data();
+    CosmoDev C;
+    std::vector<double> qbuf;
+    make_cosmo_host(rq.par, t, C, qbuf);
+    bind_cosmo_device(C, sl.d_coef, sl.d_q);
+    const double t1 = now_ms();
+    *host_ms = t1 - t0;
+    hipStream_t st = sl.stream;
+    HIPCHK(hipMemcpyAsync(sl.d_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl.d_q, qbuf.data(), sizeof(double) * qbuf.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl.d_self, &C, sizeof(CosmoDev), hipMemcpyHostToDevice, st));
+    const int nk = rq.nk;
+    std::vector<double> kh(nk);  // given, or log-spaced as the k grids of the full-CMB workload
+    if (rq.k) {
+        std::memcpy(kh.data(), rq.k, sizeof(double) * nk);
+    } else {
+        const double l0 = std::log10(rq.kmin), l1 = std::log10(rq.kmax);
+        for (int i = 0; i < nk; ++i) kh[i] = std::pow(10.0, l0 + (l1 - l0) * i / (nk - 1));
+    }
+    HIPCHK(hipMemcpyAsync(sl.d_k, kh.data(), sizeof(double) * nk, hipMemcpyHostToDevice, st));
+    std::vector<ModeDesc> md(nk);
+    for (int i = 0; i < nk; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
+    const CfgDev cd = to_cfgdev(&rq.cfg);
+    double ms = 0;
+    int r;
+    std::vector<int> status(nk);
+    if (rq.kind == BOLT_REQ_PK) {
+        const double x = 0.0;
+        HIPCHK(hipMemcpyAsync(sl.d_x, &x, sizeof(double), hipMemcpyHostToDevice, st));
+        if ((r = launch_modes(ctx, st, &sl.sb, false, {C}, {cd}, md, 0, sl.d_x, 1, sl.d_states, nullptr, sl.d_status,
+                              sl.d_ctr, cd.n, &ms)))
+            return r;
+        hipLaunchKernelGGL(k_plin_post, dim3((nk + 127) / 128), dim3(128), 0, st, sl.sb.cosmos, sl.sb.cfgs,
+                           sl.sb.modes, nk, x, sl.d_states, sl.d_pk);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(rq.out, sl.d_pk, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
+    } else {
+        if ((r = launch_modes(ctx, st, &sl.sb, false, {C}, {cd}, md, 1, nullptr, 0, sl.d_S, nullptr, sl.d_status,
+                              sl.d_ctr, 0, &ms)))
+            return r;
+        const int nkd = rq.nkd;
+        std::vector<double> dense(nkd);  // quadratic_k(0.01H₀, 1000H₀, nkd) (spectra.jl:60-63, 133)
+        const double kmin = 0.01 * C.H0, kmax = 1000.0 * C.H0;
+        for (int i = 1; i <= nkd; ++i) dense[i - 1] = kmin + (kmax - kmin) * ((double)i / nkd) * ((double)i / nkd);
+        HIPCHK(hipMemcpyAsync(sl.d_dense, dense.data(), sizeof(double) * nkd, hipMemcpyHostToDevice, st));
+        std::vector<int> ells(rq.ell_max - 1);
+        for (int l = 2; l <= rq.ell_max; ++l) ells[l - 2] = l;
+        float ms2 = 0;
+        if ((r = cl_core(ctx, st, &sl.lb, C, sl.d_self, sl.d_S, nullptr, sl.d_k, nk, sl.d_dense, nkd, ells, 0, sl.d_cl,
+                         nullptr, nullptr, nullptr, &ms2)))
+            return r;
+        ms += ms2;
+        HIPCHK(hipMemcpyAsync(rq.out, sl.d_cl, sizeof(double) * ells.size(), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipMemcpyAsync(status.data(), sl.d_status, sizeof(int) * nk, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *device_ms = ms;
+    for (int i = 0; i < nk; ++i) if (status[i] != BOLT_MODE_OK) return set_error(BOLT_E_NUMERIC, "a mode failed");
+    return BOLT_OK;
+}
+
+static void worker_loop(Server* sv, int w) {
+    (void)hipSetDevice(sv->ctx->device);
+    for (;;) {
+        ServeJob job;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv_job.wait(lk, [&] { return sv->stopping || !sv->queue.empty(); });
+            if (sv->queue.empty()) return;
+            job = sv->queue.front();
+            sv->queue.pop_front();
+            sv->results[job.id].state = BOLT_REQ_RUNNING;
+        }
+        const double t0 = now_ms();
+        double host_ms = 0, dev_ms = 0;
+        int st = run_request(sv->ctx, sv->slots[w], job.req, &host_ms, &dev_ms);
+        const double t1 = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            bolt_result& r = sv->results[job.id];
+            r.state = BOLT_REQ_DONE;
+            r.status = st;
+            r.queue_ms = t0 - job.t_submit;
+            r.host_ms = host_ms;
+            r.device_ms = dev_ms;
+            r.latency_ms = t1 - job.t_submit;
+        }
+        sv->cv_done.notify_all();
+    }
+}
+
+static void server_stop(Server* sv) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stopping = true;
+    }
+    sv->cv_job.notify_all();
+    for (auto& t : sv->workers) t.join();
+    for (auto& s : sv->slots) slot_free(s);
+    delete sv;
+}
+
+extern "C" {
+
+int bolt_serve_start(bolt_ctx* ctx, int slots, double hbm_gib) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    if (slots < 1 || !(hbm_gib > 0)) return set_error(BOLT_E_ARG, "bolt_serve_start: need slots >= 1, hbm_gib > 0");
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    if (ctx->server) return set_error(BOLT_E_STATE, "bolt_serve_start: already serving");
+    const int by_hbm = (int)(hbm_gib * 1073741824.0 / (double)slot_bytes());
+    const int n = std::min(slots, by_hbm);
+    if (n < 1) return set_error(BOLT_E_NOMEM, "bolt_serve_start: the HBM budget holds no slot");
+    HIPCHK(hipSetDevice(ctx->device));
+    Server* sv = new Server;
+    sv->ctx = ctx;
+    sv->slots.resize(n);
+    for (auto& s : sv->slots) {
+        int r = slot_create(s);
+        if (r) {
+            for (auto& q : sv->slots) slot_free(q);
+            delete sv;
+            return r;
+        }
+    }
+    for (int w = 0; w < n; ++w) sv->workers.emplace_back(worker_loop, sv, w);
+    ctx->server = sv;
+    return BOLT_OK;
+}
+
+int bolt_serve_slots(bolt_ctx* ctx) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    return ctx->server ? (int)ctx->server->slots.size() : 0;
+}
+
+int bolt_submit(bolt_ctx* ctx, const bolt_request* req, uint64_t* id) {
+    if (!ctx || !id) return set_error(BOLT_E_ARG, "bolt_submit: NULL argument");
+    int r = validate_request(req);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_submit: call bolt_serve_start first");
+    Server* sv = ctx->server;
+    {
+        std::lock_guard<std::mutex> lq(sv->mu);
+        ServeJob j;
+        j.id = sv->next_id++;
+        j.req = *req;
+        j.t_submit = now_ms();
+        bolt_result res;
+        std::memset(&res, 0, sizeof(res));
+        res.state = BOLT_REQ_QUEUED;
+        sv->results[j.id] = res;
+        sv->queue.push_back(j);
+        *id = j.id;
+    }
+    sv->cv_job.notify_one();
+    return BOLT_OK;
+}
+
+int bolt_poll(bolt_ctx* ctx, uint64_t id, bolt_result* res) {
+    if (!ctx || !res) return set_error(BOLT_E_ARG, "bolt_poll: NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_poll: not serving");
+    std::lock_guard<std::mutex> lq(ctx->server->mu);
+    auto it = ctx->server->results.find(id);
+    if (it == ctx->server->results.end()) return set_error(BOLT_E_ARG, "bolt_poll: unknown request id");
+    *res = it->second;
+    return BOLT_OK;
+}
+
+int bolt_wait(bolt_ctx* ctx, uint64_t id, bolt_result* res) {
+    if (!ctx || !res) return set_error(BOLT_E_ARG, "bolt_wait: NULL argument");
+    Server* sv;
+    {
+        std::lock_guard<std::mutex> lk(ctx->server_mu);
+        if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_wait: not serving");
+        sv = ctx->server;
+    }
+    std::unique_lock<std::mutex> lq(sv->mu);
+    auto it = sv->results.find(id);
+    if (it == sv->results.end()) return set_error(BOLT_E_ARG, "bolt_wait: unknown request id");
+    sv->cv_done.wait(lq, [&] { return sv->results[id].state == BOLT_REQ_DONE; });
+    *res = sv->results[id];
+    return BOLT_OK;
+}
+
+int bolt_serve_stop(bolt_ctx* ctx) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    if (!ctx->server) return BOLT_OK;
+    server_stop(ctx->server);
+    ctx->server = nullptr;
+    return BOLT_OK;
 }
 
 }  // extern "C"

@@ -222,6 +222,56 @@ int bolt_cl_dev(bolt_ctx* ctx, int cosmo_id, const double* d_ST, const double* d
 int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk,
               double x, double* pk, int* status, bolt_stats* stats);
 
+/* ------------------------------------------------------------------ serving
+ * Continuous batching of spectrum requests on one context (no Bolt.jl
+ * counterpart: its callers run source_grid/cltt/plin per request).  Each
+ * request brings a cosmology; the library builds its background and RECFAST
+ * tables on the host, solves its k-modes and, for C_ℓ, runs the LOS on a free
+ * slot (one stream and its own preallocated HBM); requests on different slots
+ * run concurrently and their kernels interleave on the device.  A request's
+ * result equals that of the same computation run alone. */
+enum { BOLT_REQ_PK = 0, BOLT_REQ_CLTT = 1 };
+enum { BOLT_REQ_QUEUED = 0, BOLT_REQ_RUNNING = 1, BOLT_REQ_DONE = 2 };
+
+typedef struct {
+    bolt_cosmo_params par;
+    int kind;            /* BOLT_REQ_PK: plin at x = 0 (spectra.jl:163-198) on nk log-spaced k in
+                            [kmin, kmax]; BOLT_REQ_CLTT: source_grid on those k, then cltt for
+                            ℓ = 2..ell_max on quadratic_k(0.01H₀, 1000H₀, nkd) (spectra.jl:84-96) */
+    int nk;              /* 2..800 */
+    double kmin, kmax;   /* Mpc⁻¹ */
+    const double* k;     /* optional explicit k grid (nk positive values, increasing); NULL: log-spaced
+                            in [kmin, kmax]; caller-owned until the request is done */
+    int ell_max;         /* CLTT: 2..2500 */
+    int nkd;             /* CLTT: dense k nodes, 2..5000 (Bolt uses 5000) */
+    bolt_hier_cfg cfg;   /* hierarchy and tolerances (state length <= 1024, nq <= 48) */
+    double* out;         /* caller-owned, kept alive until the request is done: nk values
+                            (PK) or ell_max - 1 values (CLTT) */
+} bolt_request;
+
+typedef struct {
+    int state;           /* BOLT_REQ_QUEUED / RUNNING / DONE */
+    int status;          /* when DONE: BOLT_OK or the error code */
+    double queue_ms;     /* submit -> start on a slot */
+    double host_ms;      /* background + RECFAST tables on the host */
+    double device_ms;    /* solver + LOS kernel time */
+    double latency_ms;   /* submit -> done */
+} bolt_result;
+
+/* Start serving with up to `slots` concurrent requests, bounded by hbm_gib of
+ * device memory (each slot preallocates ~0.13 GiB). */
+int bolt_serve_start(bolt_ctx* ctx, int slots, double hbm_gib);
+/* Number of slots of the running server (0 if none). */
+int bolt_serve_slots(bolt_ctx* ctx);
+/* Queue a request; *id identifies it.  The request struct is copied. */
+int bolt_submit(bolt_ctx* ctx, const bolt_request* req, uint64_t* id);
+/* Non-blocking state of a request. */
+int bolt_poll(bolt_ctx* ctx, uint64_t id, bolt_result* res);
+/* Block until the request is done. */
+int bolt_wait(bolt_ctx* ctx, uint64_t id, bolt_result* res);
+/* Finish the queued and running requests, then free the slots. */
+int bolt_serve_stop(bolt_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif
