@@ -1245,13 +1245,43 @@ __global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const Mo
 // recurrence's growing solution has taken over).
 // One block of four wavefronts per dense-k midpoint; lanes own x nodes (slots
 // of 64 consecutive nodes); ℓ runs in chunks reduced across lanes by a
-// butterfly reduce-scatter and across the four wavefronts through LDS.
+// butterfly reduce-scatter and across the wavefronts through LDS.
 // NS = 1: one source (C_ℓ^TT, or C_ℓ^EE with the spin factor); NS = 2: the
 // temperature and polarization sources share each j_ℓ and give TT, TE, EE.
 #define LOS_SLOTS 16   // x slots of 64 nodes (up to 1024 nodes after x = -8)
 #define LOS_WAVES 4
 #define LOS_SPW (LOS_SLOTS / LOS_WAVES)
 #define LOS_TAU 1e-7
+// lanes with z < LOS_ZS take j_ℓ from its ascending series
+//   j_ℓ(z) = z^ℓ Σ_m c_{ℓ,m} z^{2m},  c_{ℓ,m} = (-1/2)^m / (m! (2ℓ+2m+1)!!),
+// which has no cancellation there (the upward recurrence loses digits for
+// z < ℓ), and run until |j_ℓ| < LOS_TAU_SMALL max|j| or ℓ = JSER_L: there the
+// polarization source (∝ 1/z²) makes terms far below the cut above count.
+#define LOS_ZS 2.0
+#define LOS_TAU_SMALL 1e-30
+#define JSER_L 48
+#define JSER_M 14
+__constant__ double c_jser[JSER_L][JSER_M];
+
+static void fill_jser(double* c) {
+    for (int l = 0; l < JSER_L; ++l)
+        for (int m = 0; m < JSER_M; ++m) {
+            long double v = 1.0L;
+            for (int i = 1; i <= m; ++i) v /= -2.0L * i;           // (-1/2)^m / m!
+            for (int i = 1; i <= 2 * l + 2 * m + 1; i += 2) v /= i;  // / (2ℓ+2m+1)!!
+            c[l * JSER_M + m] = (double)v;
+        }
+}
+
+// out of line: inlined into the unrolled ℓ loop its scalar coefficient loads
+// are hoisted together and exhaust the scalar registers
+__device__ __attribute__((noinline)) double jser(int l, double w, double zl) {
+    const double* c = c_jser[l];
+    double F = c[JSER_M - 1];
+#pragma unroll
+    for (int m = JSER_M - 2; m >= 0; --m) F = fma(F, w, c[m]);
+    return F * zl;
+}
 
 struct LosArgs {
     const CosmoDev* cosmo;
@@ -1314,6 +1344,39 @@ __device__ inline void los_slot(double& jp, double& jc, double iz, double s, dou
     jp = a_jp; jc = a_jc; M = MM; al = live;
 }
 
+// los_slot for a wavefront holding live small-z lanes: those lanes take the
+// series (zl carries z^ℓ), the others the recurrence as in los_slot
+template <int NS, int CH, int I0>
+__device__ inline void los_slot_series(double& jp, double& jc, double iz, double z, double& zl, bool small, double s,
+                                       double sp, double& M, bool& al, int L0, double* acc, double* accp,
+                                       long long& nstep) {
+    double a_jp = jp, a_jc = jc, MM = M, zz = zl;
+    bool live = al;
+    const double base = (double)(2 * L0 - 1), w = z * z;
+#pragma unroll
+    for (int i = I0; i < CH; ++i) {
+        const int l = L0 + i;
+        double t = (base + 2.0 * i) * iz;
+        double jn;
+        if (small) {
+            zz *= z;
+            jn = (live && l < JSER_L) ? jser(l, w, zz) : 0.0;
+            if (!(fabs(jn) >= LOS_TAU_SMALL * MM) || l >= JSER_L) live = false;
+        } else {
+            jn = t * a_jc - a_jp;
+            if (t > 2.0 && (fabs(jn) < LOS_TAU * MM || fabs(jn) > fabs(a_jc))) live = false;
+        }
+        a_jp = a_jc;
+        a_jc = jn;
+        MM = fmax(MM, fabs(jn));
+        double v = live ? jn : 0.0;
+        nstep += live ? 1 : 0;
+        acc[i] += v * s;
+        if constexpr (NS == 2) accp[i] += v * sp;
+    }
+    jp = a_jp; jc = a_jc; M = MM; al = live; zl = zz;
+}
+
 // butterfly reduce-scatter of acc[CH] over the wavefront: lane l returns Σ_lanes acc[l % CH]
 template <int CH>
 __device__ inline double los_reduce(double* acc, int lane) {
@@ -1334,9 +1397,9 @@ __device__ inline double los_reduce(double* acc, int lane) {
 }
 
 template <int NS>
-__global__ void __launch_bounds__(256) k_los(LosArgs A) {
+__global__ void __launch_bounds__(LOS_WAVES * WAVE) k_los(LosArgs A) {
     constexpr int CH = 16;  // ℓ per chunk (NS·CH accumulators per lane)
-    // one block (4 wavefronts) per dense-k midpoint; wave w owns x slots w, w+4, w+8, w+12
+    // one block (LOS_WAVES wavefronts) per dense-k midpoint; wave w owns x slots w, w + LOS_WAVES, ...
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
     const int ik = blockIdx.x;
@@ -1360,8 +1423,8 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
     const double* Pa = NS == 2 ? A.SP + (size_t)ia * nx + A.i0 : nullptr;
     const double* Pb = NS == 2 ? A.SP + (size_t)(ia + 1) * nx + A.i0 : nullptr;
 
-    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], spv[LOS_SPW], mx[LOS_SPW];
-    bool alive[LOS_SPW];
+    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], spv[LOS_SPW], mx[LOS_SPW], zv[LOS_SPW], zl[LOS_SPW];
+    bool alive[LOS_SPW], small[LOS_SPW];
 #pragma unroll
     for (int sl = 0; sl < LOS_SPW; ++sl) {
         int j = (wv + LOS_WAVES * sl) * WAVE + lane;
@@ -1369,15 +1432,19 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
             double z = k * (eta0 - A.etaj[j]);
             double s = ((1.0 - tt) * Sa[j] + tt * Sb[j]) * A.dxj[j];
             double sp = NS == 2 ? ((1.0 - tt) * Pa[j] + tt * Pb[j]) * A.dxj[j] : 0.0;
-            double sn, cs;
-            sincos(z, &sn, &cs);
-            double j0 = sn / z, j1;
-            if (z < 0.1) {
-                double z2 = z * z;
-                j1 = z / 3.0 * (1.0 - z2 / 10.0 * (1.0 - z2 / 28.0 * (1.0 - z2 / 54.0 * (1.0 - z2 / 88.0))));
+            double j0, j1;
+            small[sl] = z < LOS_ZS;
+            if (small[sl]) {
+                j0 = jser(0, z * z, 1.0);
+                j1 = jser(1, z * z, z);
             } else {
+                double sn, cs;
+                sincos(z, &sn, &cs);
+                j0 = sn / z;
                 j1 = (j0 - cs) / z;
             }
+            zv[sl] = z;
+            zl[sl] = z;
             jp[sl] = j0; jc[sl] = j1;
             iz[sl] = 1.0 / z;
             sv[sl] = s;
@@ -1386,7 +1453,9 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
             alive[sl] = true;
         } else {
             jp[sl] = jc[sl] = iz[sl] = sv[sl] = spv[sl] = mx[sl] = 0.0;
+            zv[sl] = zl[sl] = 0.0;
             alive[sl] = false;
+            small[sl] = false;
         }
     }
     long long nstep = 0;
@@ -1404,7 +1473,12 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
                 acc[0] += jp[sl] * sv[sl];
                 acc[1] += jc[sl] * sv[sl];
                 if constexpr (NS == 2) { accp[0] += jp[sl] * spv[sl]; accp[1] += jc[sl] * spv[sl]; }
-                los_slot<NS, CH, 2>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], 0, acc, accp, nstep);
+                if (__any(alive[sl] && small[sl]))
+                    los_slot_series<NS, CH, 2>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
+                                               mx[sl], alive[sl], 0, acc, accp, nstep);
+                else
+                    los_slot<NS, CH, 2>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], 0, acc, accp,
+                                        nstep);
             }
             any_live = true;
         } else {
@@ -1412,7 +1486,12 @@ __global__ void __launch_bounds__(256) k_los(LosArgs A) {
             for (int sl = 0; sl < LOS_SPW; ++sl) {
                 if (!__any(alive[sl])) continue;
                 any_live = true;
-                los_slot<NS, CH, 0>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], L0, acc, accp, nstep);
+                if (__any(alive[sl] && small[sl]))
+                    los_slot_series<NS, CH, 0>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
+                                               mx[sl], alive[sl], L0, acc, accp, nstep);
+                else
+                    los_slot<NS, CH, 0>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], L0, acc, accp,
+                                        nstep);
             }
         }
         double Tv = 0.0, Ev = 0.0;
@@ -1579,6 +1658,15 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
         (void)hipFuncSetAttribute((const void*)k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipFuncSetAttribute((const void*)k_ics, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipGetLastError();  // an unsupported opt-in must not surface at the next launch check
+    }
+    {
+        std::vector<double> cj(JSER_L * JSER_M);
+        fill_jser(cj.data());
+        if (hipMemcpyToSymbol(HIP_SYMBOL(c_jser), cj.data(), sizeof(double) * cj.size()) != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return set_error(BOLT_E_DEVICE, "hipMemcpyToSymbol(series table) failed");
+        }
     }
     *out = c;
     return BOLT_OK;
@@ -2421,7 +2509,7 @@ static hipError_t slot_alloc(ServeSlot& s, T** p, size_t count) {
 }
 
 static int slot_create(ServeSlot& s) {
-    HIPCHK(hipStreamCreate(&s.stream));
+    HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     HIPCHK(slot_alloc(s, &s.d_coef, (size_t)BOLT_NTAB * (SV_NX + 2)));
     HIPCHK(slot_alloc(s, &s.d_q, 4 * SV_MAX_NQ));
     HIPCHK(slot_alloc(s, &s.d_self, 1));
@@ -2541,7 +2629,10 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
         const int nkd = rq.nkd;
         std::vector<double> dense(nkd);  // quadratic_k(0.01H₀, 1000H₀, nkd) (spectra.jl:60-63, 133)
         const double kmin = 0.01 * C.H0, kmax = 1000.0 * C.H0;
-        for (int i = 1; i <= nkd; ++i) dense[i - 1] = kmin + (kmax - kmin) * ((double)i / nkd) * ((double)i / nkd);
+        for (int i = 1; i <= nkd; ++i) {
+            const double f = (double)i / nkd;
+            dense[i - 1] = kmin + (kmax - kmin) * (f * f);
+        }
         HIPCHK(hipMemcpyAsync(sl.d_dense, dense.data(), sizeof(double) * nkd, hipMemcpyHostToDevice, st));
         std::vector<int> ells(rq.ell_max - 1);
         for (int l = 2; l <= rq.ell_max; ++l) ells[l - 2] = l;
