@@ -508,6 +508,26 @@ __device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v
 // Every later block is therefore tridiagonal, and there u1 is left unchanged
 // (u1 -= m1·u2 with u2 = 0), which halves the work of those rows.
 #define CB 8
+// one tridiagonal factor block of rows jb..jb+CB-1; GUARD: store only rows < m
+template <bool GUARD>
+__device__ inline void factor_tri_block(double* l1, double* d, const double* u1, int jb, int m, double& ip1,
+                                        double& u1p1) {
+    double L1[CB], D[CB], U1[CB];
+#pragma unroll
+    for (int i = 0; i < CB; ++i) { L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; }
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+        double m1 = L1[i] * ip1;
+        double inv = rcp(D[i] - m1 * u1p1);
+        L1[i] = m1; D[i] = inv;
+        ip1 = inv;
+        u1p1 = U1[i];
+    }
+#pragma unroll
+    for (int i = 0; i < CB; ++i)
+        if (!GUARD || jb + i < m) { l1[jb + i] = L1[i]; d[jb + i] = D[i]; }
+}
+
 __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
     if (T.tid >= W.misc()[0]) return;
     const int c0 = W.seg()[2 * T.tid], m = W.seg()[2 * T.tid + 1];
@@ -534,22 +554,45 @@ __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
         for (int i = 0; i < CB; ++i)
             if (i < m) { l2[i] = L2[i]; l1[i] = L1[i]; d[i] = D[i]; u1[i] = U1[i]; }
     }
-    for (int jb = CB; jb < m; jb += CB) {  // tridiagonal blocks
-        double L1[CB], D[CB], U1[CB];
+    // tridiagonal blocks: full ones unguarded (lanes with shorter segments
+    // leave the loop early), then the partial tail
+    int jb = CB;
+    for (; jb + CB <= m; jb += CB) factor_tri_block<false>(l1, d, u1, jb, m, ip1, u1p1);
+    if (jb < m) factor_tri_block<true>(l1, d, u1, jb, m, ip1, u1p1);
+}
+
+template <bool GUARD>
+__device__ inline void fwd_tri_block(const double* l1, double* b, int jb, int m, double& y1) {
+    double B[CB], L1[CB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) { L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; }
+    for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; }
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            double m1 = L1[i] * ip1;
-            double inv = rcp(D[i] - m1 * u1p1);
-            L1[i] = m1; D[i] = inv;
-            ip1 = inv;
-            u1p1 = U1[i];
-        }
-#pragma unroll
-        for (int i = 0; i < CB; ++i)
-            if (jb + i < m) { l1[jb + i] = L1[i]; d[jb + i] = D[i]; }
+    for (int i = 0; i < CB; ++i) {
+        double y = B[i] - L1[i] * y1;
+        B[i] = y;
+        y1 = y;
     }
+#pragma unroll
+    for (int i = 0; i < CB; ++i) if (!GUARD || jb + i < m) b[jb + i] = B[i];
+}
+
+template <bool GUARD>
+__device__ inline void bwd_tri_block(const double* u1, const double* d, double* b, int jb, int m, double& x1,
+                                     double& x2) {
+    double B[CB], U1[CB], D[CB];
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+        bool in = !GUARD || jb + i < m;
+        B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; D[i] = in ? d[jb + i] : 0.0;
+    }
+#pragma unroll
+    for (int i = CB - 1; i >= 0; --i) {
+        double x = (B[i] - U1[i] * x1) * D[i];
+        B[i] = x;
+        x2 = x1; x1 = x;
+    }
+#pragma unroll
+    for (int i = 0; i < CB; ++i) if (!GUARD || jb + i < m) b[jb + i] = B[i];
 }
 
 // forward + back substitution of one segment for one right-hand side, in
@@ -572,38 +615,14 @@ __device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
 #pragma unroll
         for (int i = 0; i < CB; ++i) if (i < m) b[i] = B[i];
     }
-    for (int jb = CB; jb < m; jb += CB) {
-        double B[CB], L1[CB];
-#pragma unroll
-        for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            double y = B[i] - L1[i] * y1;
-            B[i] = y;
-            y1 = y;
-        }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
-    }
-    // backward, blocks from the top; rows >= m read as zero contributions
+    int jb = CB;
+    for (; jb + CB <= m; jb += CB) fwd_tri_block<false>(l1, b, jb, m, y1);
+    const bool tail = jb < m;
+    if (tail) fwd_tri_block<true>(l1, b, jb, m, y1);
+    // backward: the tail, the full blocks downward, then the first block
     double x1 = 0.0, x2 = 0.0;
-    const int top = ((m + CB - 1) / CB) * CB;
-    for (int jb = top - CB; jb >= CB; jb -= CB) {
-        double B[CB], U1[CB], D[CB];
-#pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            bool in = jb + i < m;
-            B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; D[i] = in ? d[jb + i] : 0.0;
-        }
-#pragma unroll
-        for (int i = CB - 1; i >= 0; --i) {
-            double x = (B[i] - U1[i] * x1) * D[i];
-            B[i] = x;
-            x2 = x1; x1 = x;
-        }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) if (jb + i < m) b[jb + i] = B[i];
-    }
+    if (tail) bwd_tri_block<true>(u1, d, b, jb, m, x1, x2);
+    for (jb -= CB; jb >= CB; jb -= CB) bwd_tri_block<false>(u1, d, b, jb, m, x1, x2);
     {
         double B[CB], U1[CB], U2[CB], D[CB];
 #pragma unroll
@@ -676,9 +695,10 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
 }
 
 // After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
-// matrix and finish z = w + hγ Y M⁻¹ D w in place
+// matrix K and y = K⁻¹ D w; the stage value is z = w + hγ (c_S y₀ + c_P y₁ + c_Π y₂).
+// Every wavefront forms the same K and y from its own lanes; nothing is written.
 __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double hg,
-                                         const Thr& T) {
+                                   const Thr& T, double* y) {
     const int lane = T.lane;
     double a[8];
 #pragma unroll
@@ -701,11 +721,8 @@ __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X,
     K.m[0][0] = 1.0 - hg * Fs.S; K.m[0][1] = -hg * Fp.S; K.m[0][2] = -hg * Fq.S;
     K.m[1][0] = -hg * Fs.P; K.m[1][1] = 1.0 - hg * Fp.P; K.m[1][2] = -hg * Fq.P;
     K.m[2][0] = -hg * Fs.Pi; K.m[2][1] = -hg * Fp.Pi; K.m[2][2] = 1.0 - hg * Fq.Pi;
-    double r[3] = {Fw.S, Fw.P, Fw.Pi}, y[3];
+    double r[3] = {Fw.S, Fw.P, Fw.Pi};
     solve3(K, r, y);
-    __syncthreads();
-    for (int c = T.tid; c < G.n; c += T.nt)
-        W.w()[c] += hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
     return K;
 }
 
@@ -1013,15 +1030,15 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             bool last = false;
             if (h >= xend - x) { h = xend - x; last = true; }
             const double hg = h * KC_G;
-            for (int c = T.tid; c < G.n; c += T.nt) W.Z(0)[c] = h * W.f()[c];
+            // Z₀ = h f(u) and the first stage's argument
+            for (int c = T.tid; c < G.n; c += T.nt) {
+                double z0 = h * W.f()[c];
+                W.Z(0)[c] = z0;
+                W.arg()[c] = W.u()[c] + KC_A[1][0] * z0;
+            }
             Cap K;
             XC Xs;
             for (int s = 1; s < 6; ++s) {
-                for (int c = T.tid; c < G.n; c += T.nt) {
-                    double acc = W.u()[c];
-                    for (int j = 0; j < s; ++j) acc += KC_A[s][j] * W.Z(j)[c];
-                    W.arg()[c] = acc;
-                }
                 Xs = make_xc(C, k, x + KC_C[s] * h);
                 qfill(C, G, Xs, W, T);
                 __syncthreads();
@@ -1034,29 +1051,36 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 chain_solve4(G, W, T);
                 ++nsol;
                 __syncthreads();
-                K = stage_border(C, G, Xs, W, hg, T);
-                for (int c = T.tid; c < G.n; c += T.nt) W.Z(s)[c] = W.w()[c];
+                double y[3];
+                K = stage_border(C, G, Xs, W, hg, T, y);
+                // one pass: Z_s, then the next stage's argument, or after the last
+                // stage u_{n+1} (stiffly accurate) and the embedded error (into ψ,
+                // which no wavefront reads any more; w still is)
+                for (int c = T.tid; c < G.n; c += T.nt) {
+                    const double z = W.w()[c] + hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
+                    W.Z(s)[c] = z;
+                    if (s < 5) {
+                        double acc = W.u()[c];
+                        for (int j = 0; j < s; ++j) acc += KC_A[s + 1][j] * W.Z(j)[c];
+                        W.arg()[c] = acc + KC_A[s + 1][s] * z;
+                    } else {
+                        double acc = W.u()[c], e = 0.0;
+                        for (int j = 0; j < 5; ++j) {
+                            acc += KC_A[5][j] * W.Z(j)[c];
+                            e += (KC_A[5][j] - KC_BH[j]) * W.Z(j)[c];
+                        }
+                        W.un()[c] = acc + KC_G * z;
+                        W.arg()[c] = e + (KC_G - KC_BH[5]) * z;
+                    }
+                }
                 __syncthreads();
             }
-            // u_{n+1} (stiffly accurate: last stage) and the embedded error
-            for (int c = T.tid; c < G.n; c += T.nt) {
-                double acc = W.u()[c], e = 0.0;
-                for (int j = 0; j < 5; ++j) {
-                    acc += KC_A[5][j] * W.Z(j)[c];
-                    e += (KC_A[5][j] - KC_BH[j]) * W.Z(j)[c];
-                }
-                acc += KC_G * W.Z(5)[c];
-                e += (KC_G - KC_BH[5]) * W.Z(5)[c];
-                W.un()[c] = acc;
-                W.w()[c] = e;
-            }
-            __syncthreads();
             // Shampine's smoothed estimate: W⁻¹ e with the last stage's W
-            chain_solve1(G, W, W.w(), T);
+            chain_solve1(G, W, W.arg(), T);
             ++nsol;
             __syncthreads();
-            border_solve(C, G, Xs, W, K, hg, W.w(), T);
-            double en = wrms(G, W, W.w(), W.u(), W.un(), rt, at, T);
+            border_solve(C, G, Xs, W, K, hg, W.arg(), T);
+            double en = wrms(G, W, W.arg(), W.u(), W.un(), rt, at, T);
             __syncthreads();
             if (!isfinite(en)) {
                 ++nrej;
