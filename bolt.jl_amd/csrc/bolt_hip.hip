@@ -186,9 +186,13 @@ __device__ inline double rcp(double d) {
 // largest hierarchy cut-off accepted (row descriptors pack ℓ in 11 bits)
 #define LTAB 512
 
-// x-dependent coefficients of hierarchy! (perturbations.jl:163-172)
+// x-dependent coefficients of hierarchy! (perturbations.jl:163-172), with the
+// divisions by per-mode and per-x constants taken once here: three reciprocals
+// per evaluation point instead of one division per use
 struct XC {
     double x, a, Hc, eta, tp, tpp, csb2, kH, R, c12, gg, k23, iHeta;
+    double fs2;                         // Ψ:  c12 / (4 ρ_crit)        (multiplies Σ σ)
+    double fp0, fp1, fp2, fp3, fp4;     // Φ′: gg Ω_c/a, gg Ω_b/a, gg 4Ω_r/a², gg 4Ω_ν/a², gg/(ρ_crit a²)
     bool rsa;
 };
 
@@ -201,13 +205,20 @@ __device__ inline XC make_xc(const CosmoDev& C, double k, double x) {
     X.tp = spl(C, BOLT_T_TAU_P, x);
     X.tpp = spl(C, BOLT_T_TAU_PP, x);
     X.csb2 = spl(C, BOLT_T_CSB2, x);
-    X.kH = k / X.Hc;
-    X.R = 4.0 * C.Or / (3.0 * C.Ob * X.a);
-    double H02 = C.H0 * C.H0;
-    X.c12 = 12.0 * H02 / (k * k) / (X.a * X.a);
-    X.gg = H02 / (2.0 * X.Hc * X.Hc);
-    X.k23 = k * k / (3.0 * X.Hc * X.Hc);
-    X.iHeta = 1.0 / (X.Hc * X.eta);
+    const double ia = 1.0 / X.a, ia2 = ia * ia, iHc = 1.0 / X.Hc, iHc2 = iHc * iHc, ik2 = 1.0 / (k * k);
+    const double H02 = C.H0 * C.H0, irho = 1.0 / C.rho_crit;
+    X.kH = k * iHc;
+    X.R = (4.0 * C.Or / (3.0 * C.Ob)) * ia;
+    X.c12 = (12.0 * H02) * ik2 * ia2;
+    X.gg = (0.5 * H02) * iHc2;
+    X.k23 = (k * k / 3.0) * iHc2;
+    X.iHeta = iHc / X.eta;
+    X.fs2 = X.c12 * (0.25 * irho);
+    X.fp0 = X.gg * C.Oc * ia;
+    X.fp1 = X.gg * C.Ob * ia;
+    X.fp2 = X.gg * (4.0 * C.Or) * ia2;
+    X.fp3 = X.gg * (4.0 * C.Onu) * ia2;
+    X.fp4 = X.gg * irho * ia2;
     X.rsa = (k * X.eta > 240.0) && (-X.tp * X.Hc / X.eta > 100.0);  // perturbations.jl:216
     return X;
 }
@@ -439,11 +450,8 @@ struct Fun {
 __device__ inline Fun fun_finish(const CosmoDev& C, const Geo& G, const XC& X, const double* v, double s2,
                                  double s0) {
     Fun F;
-    double a = X.a;
-    F.S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2] + s2 / C.rho_crit / 4.0);
-    F.P = F.S - X.k23 * v[G.cE] +
-          X.gg * (C.Oc / a * v[G.cD] + C.Ob / a * v[0] + 4.0 * C.Or / (a * a) * v[2] +
-                  4.0 * C.Onu / (a * a) * v[G.cC] + s0 / C.rho_crit / (a * a));
+    F.S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2]) - X.fs2 * s2;
+    F.P = F.S - X.k23 * v[G.cE] + (X.fp0 * v[G.cD] + X.fp1 * v[0] + X.fp2 * v[2] + X.fp3 * v[G.cC] + X.fp4 * s0);
     F.Pi = v[4] + v[G.cB + 2] + v[G.cB];
     return F;
 }
@@ -682,16 +690,18 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
     double a2[4] = {K.m[2][0], K.m[2][1], K.m[2][2], r[2]};
     swap_rows_if(fabs(a1[0]) > fabs(a0[0]), a0, a1);
     swap_rows_if(fabs(a2[0]) > fabs(a0[0]), a0, a2);
-    double f1 = a1[0] / a0[0], f2 = a2[0] / a0[0];
+    const double p0 = rcp(a0[0]);
+    double f1 = a1[0] * p0, f2 = a2[0] * p0;
 #pragma unroll
     for (int j = 1; j < 4; ++j) { a1[j] -= f1 * a0[j]; a2[j] -= f2 * a0[j]; }
     swap_rows_if(fabs(a2[1]) > fabs(a1[1]), a1, a2);
-    double g2 = a2[1] / a1[1];
+    const double p1 = rcp(a1[1]);
+    double g2 = a2[1] * p1;
 #pragma unroll
     for (int j = 2; j < 4; ++j) a2[j] -= g2 * a1[j];
-    y[2] = a2[3] / a2[2];
-    y[1] = (a1[3] - a1[2] * y[2]) / a1[1];
-    y[0] = (a0[3] - a0[1] * y[1] - a0[2] * y[2]) / a0[0];
+    y[2] = a2[3] * rcp(a2[2]);
+    y[1] = (a1[3] - a1[2] * y[2]) * p1;
+    y[0] = (a0[3] - a0[1] * y[1] - a0[2] * y[2]) * p0;
 }
 
 // After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
