@@ -44,6 +44,17 @@ def workload():
     return par, bg, ih, ks, cfg, ells, dense
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_boltsolve_cmb_full.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+([0-9.eE+-]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def ode_flops(n, nq, steps, accepted, nmodes, nodes):
     """Algorithmic FP64 flops of the integrator as implemented (per step: five
     stage RHS + banded factor/solve + border, the smoothed error solve; per
@@ -177,12 +188,12 @@ def main():
     los_tf_exec = 4.0 * los_exec / (ms_los * 1e-3) / 1e12
     if ms_ode >= ms_los:
         roof = {"kernel": "k_boltsolve", "bound": "mfma", "achieved": ode_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                "frac": ode_tf / PEAK_FP64, "traffic": None,
+                "frac": ode_tf / PEAK_FP64, "traffic": measured_traffic("k_boltsolve"),
                 "note": "FP64 vector peak (equal to the FP64 MFMA figure); algorithmic flops of the structured "
                         "ESDIRK solve per step/RHS/node, from the kernel's own step counters"}
     else:
         roof = {"kernel": "k_los", "bound": "mfma", "achieved": los_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                "frac": los_tf / PEAK_FP64, "traffic": None}
+                "frac": los_tf / PEAK_FP64, "traffic": measured_traffic("k_los")}
     line = {
         "metric": METRIC, "value": 1.0 / dt, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
