@@ -223,10 +223,35 @@ __device__ inline XC make_xc(const CosmoDev& C, double k, double x) {
     return X;
 }
 
-// LDS workspace of one k-mode: 20 vectors of n doubles, 7 per-q arrays, two
-// per-row constants, cross-wavefront partial sums, the per-row descriptors,
-// the row evaluation order and the segment table of the banded solve.  Every
-// pointer is base + offset arithmetic.
+// q-quantities at one x: (q/ϵ, ϵ/q·dlnf₀/dlnq, 4πq²f₀w·q²/ϵ, 4πq²f₀w·ϵ) per bin
+struct QSet {
+    const double* qe;
+    const double* eqdf;
+    const double* sw;
+    const double* rw;
+};
+
+// the LU of W = I - hγL at one stage point: l1 <- multipliers, d <- 1/pivot,
+// u1 <- upper entries; the second band lives only in the first block of the
+// first segment (rows 0..CB-1), kept in l2c/u2c
+struct Bands {
+    double* l1;
+    double* d;
+    double* u1;
+    double* l2c;
+    double* u2c;
+};
+
+#define CB 8      // rows per block of the banded sweeps
+#define NXC 24    // doubles per stored XC
+
+// LDS workspace of one k-mode.  Vectors of n doubles: u, f, u_{n+1}, f_{n+1}, ψ,
+// w, Z₀..Z₅, the border columns c_S, c_P, c_Π, the ℓ couplings ℓ/(2ℓ+1) and
+// (ℓ+1)/(2ℓ+1), and the LU of W at the five implicit stage points (3 vectors
+// each).  Then the q arrays (mode constants, the current x, the five stage
+// points), the compact second bands, the five stage points' coefficients,
+// cross-wavefront partial sums, and the integer tables (row descriptors, row
+// evaluation order, segments).  Every pointer is base + offset arithmetic.
 struct WS {
     double* b;
     int n, nq;
@@ -237,25 +262,35 @@ struct WS {
     __device__ double* arg() const { return b + 4 * n; }
     __device__ double* w() const { return b + 5 * n; }
     __device__ double* Z(int j) const { return b + (6 + j) * n; }
-    __device__ double* l2() const { return b + 12 * n; }
-    __device__ double* l1() const { return b + 13 * n; }
-    __device__ double* d() const { return b + 14 * n; }
-    __device__ double* u1() const { return b + 15 * n; }
-    __device__ double* u2() const { return b + 16 * n; }
-    __device__ double* cS() const { return b + 17 * n; }
-    __device__ double* cP() const { return b + 18 * n; }
-    __device__ double* cPi() const { return b + 19 * n; }
-    __device__ double* q() const { return b + 20 * n; }
-    __device__ double* df0() const { return b + 20 * n + nq; }
-    __device__ double* base() const { return b + 20 * n + 2 * nq; }
-    __device__ double* qe() const { return b + 20 * n + 3 * nq; }
-    __device__ double* eqdf() const { return b + 20 * n + 4 * nq; }
-    __device__ double* sw() const { return b + 20 * n + 5 * nq; }
-    __device__ double* rw() const { return b + 20 * n + 6 * nq; }
-    __device__ double* ra() const { return b + 20 * n + 7 * nq; }   // ℓ/(2ℓ+1)
-    __device__ double* rb() const { return b + 21 * n + 7 * nq; }   // (ℓ+1)/(2ℓ+1)
-    __device__ double* red() const { return b + 22 * n + 7 * nq; }  // [8]
-    __device__ int* desc() const { return (int*)(b + 22 * n + 7 * nq + 8); }
+    __device__ double* cS() const { return b + 12 * n; }
+    __device__ double* cP() const { return b + 13 * n; }
+    __device__ double* cPi() const { return b + 14 * n; }
+    __device__ double* ra() const { return b + 15 * n; }   // ℓ/(2ℓ+1)
+    __device__ double* rb() const { return b + 16 * n; }   // (ℓ+1)/(2ℓ+1)
+    __device__ Bands bands(int s) const {
+        Bands B;
+        B.l1 = b + (17 + 3 * s) * n; B.d = b + (18 + 3 * s) * n; B.u1 = b + (19 + 3 * s) * n;
+        B.l2c = b + 32 * n + 27 * nq + 2 * CB * s;
+        B.u2c = B.l2c + CB;
+        return B;
+    }
+    __device__ double* q() const { return b + 32 * n; }
+    __device__ double* df0() const { return b + 32 * n + nq; }
+    __device__ double* base() const { return b + 32 * n + 2 * nq; }
+    __device__ double* qe() const { return b + 32 * n + 3 * nq; }
+    __device__ double* eqdf() const { return b + 32 * n + 4 * nq; }
+    __device__ double* sw() const { return b + 32 * n + 5 * nq; }
+    __device__ double* rw() const { return b + 32 * n + 6 * nq; }
+    __device__ double* qst_base(int s) const { return b + 32 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
+    __device__ QSet qcur() const { QSet Q; Q.qe = qe(); Q.eqdf = eqdf(); Q.sw = sw(); Q.rw = rw(); return Q; }
+    __device__ QSet qst(int s) const {
+        const double* p = qst_base(s);
+        QSet Q; Q.qe = p; Q.eqdf = p + nq; Q.sw = p + 2 * nq; Q.rw = p + 3 * nq;
+        return Q;
+    }
+    __device__ double* xcs(int s) const { return b + 32 * n + 27 * nq + 10 * CB + NXC * s; }
+    __device__ double* red() const { return b + 32 * n + 27 * nq + 10 * CB + 5 * NXC; }  // [8]
+    __device__ int* desc() const { return (int*)(b + 32 * n + 27 * nq + 10 * CB + 5 * NXC + 8); }
     __device__ int* order() const { return desc() + n; }            // [n + NTHR]
     __device__ int* seg() const { return desc() + 2 * n + NTHR; }   // [MAXSEG][2]: c0, m
     __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg, nslot
@@ -267,7 +302,21 @@ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
     W.nq = nq;
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
-    return sizeof(double) * ((size_t)22 * n + 7 * nq + 8) + sizeof(int) * (size_t)(2 * n + NTHR + 2 * MAXSEG + 4);
+    return sizeof(double) * ((size_t)32 * n + 27 * nq + 10 * CB + 5 * NXC + 8) +
+           sizeof(int) * (size_t)(2 * n + NTHR + 2 * MAXSEG + 4);
+}
+
+__device__ inline void store_xc(double* p, const XC& X) {
+    p[0] = X.x; p[1] = X.a; p[2] = X.Hc; p[3] = X.eta; p[4] = X.tp; p[5] = X.tpp; p[6] = X.csb2; p[7] = X.kH;
+    p[8] = X.R; p[9] = X.c12; p[10] = X.gg; p[11] = X.k23; p[12] = X.iHeta; p[13] = X.fs2; p[14] = X.fp0;
+    p[15] = X.fp1; p[16] = X.fp2; p[17] = X.fp3; p[18] = X.fp4; p[19] = X.rsa ? 1.0 : 0.0;
+}
+__device__ inline XC load_xc(const double* p) {
+    XC X;
+    X.x = p[0]; X.a = p[1]; X.Hc = p[2]; X.eta = p[3]; X.tp = p[4]; X.tpp = p[5]; X.csb2 = p[6]; X.kH = p[7];
+    X.R = p[8]; X.c12 = p[9]; X.gg = p[10]; X.k23 = p[11]; X.iHeta = p[12]; X.fs2 = p[13]; X.fp0 = p[14];
+    X.fp1 = p[15]; X.fp2 = p[16]; X.fp3 = p[17]; X.fp4 = p[18]; X.rsa = p[19] != 0.0;
+    return X;
 }
 
 // row types of the chain-ordered state
@@ -365,19 +414,22 @@ __device__ inline void setup_rows(const Geo& G, const WS& W, const Thr& T) {
     }
 }
 
-// per-x q-quantities (perturbations.jl:203-213, 127-145); lanes < nq; caller syncs
+// per-x q-quantities of bin i at scale factor a (perturbations.jl:203-213,
+// 127-145) into the arrays at p (qe, eqdf, sw, rw, nq apart)
+__device__ inline void qfill_one(const CosmoDev& C, const WS& W, double a, double* p, int i) {
+    double q = W.q()[i];
+    double am = a * C.mnu;
+    double eps = sqrt(q * q + am * am);
+    double ie = 1.0 / eps;
+    p[i] = q * ie;
+    p[W.nq + i] = eps / q * W.df0()[i];
+    p[2 * W.nq + i] = W.base()[i] * (q * q * ie);
+    p[3 * W.nq + i] = W.base()[i] * eps;
+}
+
+// the current-x set; threads < nq; caller syncs
 __device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const Thr& T) {
-    if (T.tid < G.nq) {
-        const int lane = T.tid;
-        double q = W.q()[lane];
-        double am = X.a * C.mnu;
-        double eps = sqrt(q * q + am * am);
-        double ie = 1.0 / eps;
-        W.qe()[lane] = q * ie;
-        W.eqdf()[lane] = eps / q * W.df0()[lane];
-        W.sw()[lane] = W.base()[lane] * (q * q * ie);
-        W.rw()[lane] = W.base()[lane] * eps;
-    }
+    if (T.tid < G.nq) qfill_one(C, W, X.a, W.qe(), T.tid);
 }
 
 // One row of du = L u + c_S Ψ + c_P Φ′ + c_Π Π (perturbations.jl:196-265)
@@ -385,7 +437,7 @@ struct RowC {
     double l2, l1, d, u1, u2, cs, cp, cpi;
 };
 
-__device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, int desc, int c) {
+__device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, const QSet& Q, int desc, int c) {
     RowC r = {0, 0, 0, 0, 0, 0, 0, 0};
     const int type = desc & 31, l = (desc >> 5) & 2047, qi = desc >> 16;
     const double kH = X.kH, tp = X.tp;
@@ -395,7 +447,7 @@ __device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, int desc
     if (photon_nu && X.rsa) return r;  // RSA: Θ′ = Θᵖ′ = 𝒩′ = 0 (perturbations.jl:229-232)
     if (type == R_TL || type == R_PL || type == R_NL || type == R_ML) {
         // interior of a hierarchy: ℓ/(2ℓ+1) and (ℓ+1)/(2ℓ+1) couplings
-        double kf = (type == R_ML) ? kH * W.qe()[qi] : kH;
+        double kf = (type == R_ML) ? kH * Q.qe[qi] : kH;
         r.l1 = kf * W.ra()[c];
         r.u1 = -kf * W.rb()[c];
         r.d = photon ? tp : 0.0;
@@ -404,7 +456,7 @@ __device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, int desc
     }
     if (type == R_TN || type == R_PN || type == R_NN || type == R_MN) {
         // truncation rows
-        double kf = (type == R_MN) ? kH * W.qe()[qi] : kH;
+        double kf = (type == R_MN) ? kH * Q.qe[qi] : kH;
         // value selects (a select between struct fields would index the struct)
         const double Lg = (double)(G.ng + 1), Ln = (double)(G.nn + 1), Lm = (double)(G.nm + 1);
         double L1 = (type == R_NN) ? Ln : Lg;
@@ -420,10 +472,10 @@ __device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, int desc
         case R_T1: r.l2 = tp * third; r.l1 = kH * third; r.d = tp; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
         case R_P0: r.d = tp; r.u1 = -kH; r.cpi = -tp * 0.5; break;
         case R_N1: r.l1 = kH * third; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
-        case R_M0: r.u1 = -kH * W.qe()[qi]; r.cp = W.df0()[qi]; break;
+        case R_M0: r.u1 = -kH * Q.qe[qi]; r.cp = W.df0()[qi]; break;
         case R_M1: {
-            double qe = W.qe()[qi];
-            r.l1 = kH * qe * third; r.u1 = -2.0 * kH * qe * third; r.cs = -kH * third * W.eqdf()[qi];
+            double qe = Q.qe[qi];
+            r.l1 = kH * qe * third; r.u1 = -2.0 * kH * qe * third; r.cs = -kH * third * Q.eqdf[qi];
             break;
         }
         case R_DEL: r.u1 = kH; r.cp = -3.0; break;
@@ -457,38 +509,51 @@ __device__ inline Fun fun_finish(const CosmoDev& C, const Geo& G, const XC& X, c
 }
 
 // every wavefront forms the sums over its own lanes (identical in all of them)
-__device__ inline Fun functionals(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
+__device__ inline Fun functionals(const CosmoDev& C, const Geo& G, const XC& X, const QSet& Q, const double* v,
                                   const Thr& T) {
     double s2 = 0.0, s0 = 0.0;
     const int lane = T.lane;
     if (lane < G.nq) {
         int b = G.cM + lane * (G.nm + 1);
-        s2 = W.sw()[lane] * v[b + 2];
-        s0 = W.rw()[lane] * v[b];
+        s2 = Q.sw[lane] * v[b + 2];
+        s0 = Q.rw[lane] * v[b];
     }
     return fun_finish(C, G, X, v, wave_sum(s2), wave_sum(s0));
 }
 
-// du = f(v) on all rows (out), optionally writing W = I - hγL bands and the
-// border columns for the stage solve; returns the functionals of v
-__device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const double* v,
-                                     double scale, double* out, double hg, bool make_w, const Thr& T) {
-    Fun F = functionals(C, G, X, W, v, T);
+// du = f(v) on all rows (out = scale·du), optionally writing the border
+// columns c_S, c_P, c_Π; returns the functionals of v
+__device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
+                               const double* v, double scale, double* out, bool make_c, const Thr& T) {
+    Fun F = functionals(C, G, X, Q, v, T);
     const int* desc = W.desc();
     const int* order = W.order();
     const int nslot = W.misc()[1];
     for (int t = T.tid; t < nslot; t += T.nt) {
         const int c = order[t];
         if (c < 0) continue;
-        RowC r = row_coef(G, X, W, desc[c], c);
+        RowC r = row_coef(G, X, W, Q, desc[c], c);
         out[c] = scale * row_apply(r, v, c, G.n, F.S, F.P, F.Pi);
-        if (make_w) {
-            W.l2()[c] = -hg * r.l2; W.l1()[c] = -hg * r.l1; W.d()[c] = 1.0 - hg * r.d;
-            W.u1()[c] = -hg * r.u1; W.u2()[c] = -hg * r.u2;
-            W.cS()[c] = r.cs; W.cP()[c] = r.cp; W.cPi()[c] = r.cpi;
-        }
+        if (make_c) { W.cS()[c] = r.cs; W.cP()[c] = r.cp; W.cPi()[c] = r.cpi; }
     }
     return F;
+}
+
+// W = I - hγL at the five stage points (their XC and q-sets stored), every
+// (stage, row) pair spread over the workgroup; caller syncs
+__device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const Thr& T) {
+    const int nslot = W.misc()[1];
+    const int* desc = W.desc();
+    const int* order = W.order();
+    for (int t = T.tid; t < 5 * nslot; t += T.nt) {
+        const int s = t / nslot, c = order[t - s * nslot];
+        if (c < 0) continue;
+        const XC X = load_xc(W.xcs(s));
+        const RowC r = row_coef(G, X, W, W.qst(s), desc[c], c);
+        const Bands B = W.bands(s);
+        B.l1[c] = -hg * r.l1; B.d[c] = 1.0 - hg * r.d; B.u1[c] = -hg * r.u1;
+        if (c < CB) { B.l2c[c] = -hg * r.l2; B.u2c[c] = -hg * r.u2; }
+    }
 }
 
 // RSA overwrite of u (perturbations.jl:217-227); one lane; caller syncs around
@@ -515,7 +580,6 @@ __device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v
 // first segment: u2 on v_b (its Θ₁ coupling) and l2 on Θ₁ (its v_b coupling).
 // Every later block is therefore tridiagonal, and there u1 is left unchanged
 // (u1 -= m1·u2 with u2 = 0), which halves the work of those rows.
-#define CB 8
 // one tridiagonal factor block of rows jb..jb+CB-1; GUARD: store only rows < m
 template <bool GUARD>
 __device__ inline void factor_tri_block(double* l1, double* d, const double* u1, int jb, int m, double& ip1,
@@ -536,16 +600,17 @@ __device__ inline void factor_tri_block(double* l1, double* d, const double* u1,
         if (!GUARD || jb + i < m) { l1[jb + i] = L1[i]; d[jb + i] = D[i]; }
 }
 
-__device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
-    if (T.tid >= W.misc()[0]) return;
-    const int c0 = W.seg()[2 * T.tid], m = W.seg()[2 * T.tid + 1];
-    double* l2 = W.l2() + c0; double* l1 = W.l1() + c0; double* d = W.d() + c0;
-    double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
+__device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m) {
+    double* l1 = Bd.l1 + c0; double* d = Bd.d + c0; double* u1 = Bd.u1 + c0;
+    const bool first = c0 == 0;  // the only segment with a second band
     double ip1 = 0.0, ip2 = 0.0, u1p1 = 0.0, u1p2 = 0.0, u2p1 = 0.0, u2p2 = 0.0;
     {   // first block: the full bandwidth-2 recurrence
         double L2[CB], L1[CB], D[CB], U1[CB], U2[CB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) { L2[i] = l2[i]; L1[i] = l1[i]; D[i] = d[i]; U1[i] = u1[i]; U2[i] = u2[i]; }
+        for (int i = 0; i < CB; ++i) {
+            L2[i] = first ? Bd.l2c[i] : 0.0; U2[i] = first ? Bd.u2c[i] : 0.0;
+            L1[i] = l1[i]; D[i] = d[i]; U1[i] = u1[i];
+        }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
             double m2 = L2[i] * ip2;
@@ -560,13 +625,25 @@ __device__ inline void chain_factor(const Geo& G, const WS& W, const Thr& T) {
         }
 #pragma unroll
         for (int i = 0; i < CB; ++i)
-            if (i < m) { l2[i] = L2[i]; l1[i] = L1[i]; d[i] = D[i]; u1[i] = U1[i]; }
+            if (i < m) {
+                if (first) Bd.l2c[i] = L2[i];
+                l1[i] = L1[i]; d[i] = D[i]; u1[i] = U1[i];
+            }
     }
     // tridiagonal blocks: full ones unguarded (lanes with shorter segments
     // leave the loop early), then the partial tail
     int jb = CB;
     for (; jb + CB <= m; jb += CB) factor_tri_block<false>(l1, d, u1, jb, m, ip1, u1p1);
     if (jb < m) factor_tri_block<true>(l1, d, u1, jb, m, ip1, u1p1);
+}
+
+// the LU of all five stage points, one (stage, segment) unit per thread
+__device__ inline void chain_factor_all(const Geo& G, const WS& W, const Thr& T) {
+    const int nseg = W.misc()[0];
+    for (int u = T.tid; u < 5 * nseg; u += T.nt) {
+        const int st = u / nseg, sg = u - st * nseg;
+        chain_factor_seg(W.bands(st), W.seg()[2 * sg], W.seg()[2 * sg + 1]);
+    }
 }
 
 template <bool GUARD>
@@ -605,15 +682,15 @@ __device__ inline void bwd_tri_block(const double* u1, const double* d, double* 
 
 // forward + back substitution of one segment for one right-hand side, in
 // blocks (the first block with both bands, the others tridiagonal)
-__device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
-    const double* l2 = W.l2() + c0; const double* l1 = W.l1() + c0; const double* d = W.d() + c0;
-    const double* u1 = W.u1() + c0; const double* u2 = W.u2() + c0;
+__device__ inline void chain_subst(const Bands& Bd, int c0, int m, double* bvec) {
+    const double* l1 = Bd.l1 + c0; const double* d = Bd.d + c0; const double* u1 = Bd.u1 + c0;
+    const bool first = c0 == 0;
     double* b = bvec + c0;
     double y1 = 0.0, y2 = 0.0;
     {
         double B[CB], L1[CB], L2[CB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) { B[i] = b[i]; L1[i] = l1[i]; L2[i] = l2[i]; }
+        for (int i = 0; i < CB; ++i) { B[i] = b[i]; L1[i] = l1[i]; L2[i] = first ? Bd.l2c[i] : 0.0; }
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
             double y = (B[i] - L2[i] * y2) - L1[i] * y1;
@@ -636,7 +713,8 @@ __device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
             bool in = i < m;
-            B[i] = in ? b[i] : 0.0; U1[i] = in ? u1[i] : 0.0; U2[i] = in ? u2[i] : 0.0; D[i] = in ? d[i] : 0.0;
+            B[i] = in ? b[i] : 0.0; U1[i] = in ? u1[i] : 0.0; U2[i] = (in && first) ? Bd.u2c[i] : 0.0;
+            D[i] = in ? d[i] : 0.0;
         }
 #pragma unroll
         for (int i = CB - 1; i >= 0; --i) {
@@ -652,20 +730,21 @@ __device__ inline void chain_subst(const WS& W, int c0, int m, double* bvec) {
 // the four right-hand sides w, c_S, c_P, c_Π after chain_factor: one
 // (segment, rhs) unit per lane; with four wavefronts, wavefront r takes
 // right-hand side r
-__device__ inline void chain_solve4(const Geo& G, const WS& W, const Thr& T) {
+__device__ inline void chain_solve4(const Geo& G, const WS& W, int st, const Thr& T) {
     const int nseg = W.misc()[0];
+    const Bands Bd = W.bands(st);
     for (int u = T.tid; u < 4 * WAVE; u += T.nt) {
         const int r = u / WAVE, sg = u % WAVE;
         if (sg >= nseg) continue;
         double* bv = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
-        chain_subst(W, W.seg()[2 * sg], W.seg()[2 * sg + 1], bv);
+        chain_subst(Bd, W.seg()[2 * sg], W.seg()[2 * sg + 1], bv);
     }
 }
 
 // one right-hand side with the stored factors, one lane per segment
-__device__ inline void chain_solve1(const Geo& G, const WS& W, double* bvec, const Thr& T) {
+__device__ inline void chain_solve1(const Geo& G, const WS& W, int st, double* bvec, const Thr& T) {
     if (T.tid >= W.misc()[0]) return;
-    chain_subst(W, W.seg()[2 * T.tid], W.seg()[2 * T.tid + 1], bvec);
+    chain_subst(W.bands(st), W.seg()[2 * T.tid], W.seg()[2 * T.tid + 1], bvec);
 }
 
 // 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
@@ -707,15 +786,15 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
 // After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
 // matrix K and y = K⁻¹ D w; the stage value is z = w + hγ (c_S y₀ + c_P y₁ + c_Π y₂).
 // Every wavefront forms the same K and y from its own lanes; nothing is written.
-__device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double hg,
-                                   const Thr& T, double* y) {
+__device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
+                                   double hg, const Thr& T, double* y) {
     const int lane = T.lane;
     double a[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) a[i] = 0.0;
     if (lane < G.nq) {
         int b = G.cM + lane * (G.nm + 1);
-        double sw = W.sw()[lane], rw = W.rw()[lane];
+        double sw = Q.sw[lane], rw = Q.rw[lane];
         a[0] = sw * W.w()[b + 2];   a[1] = rw * W.w()[b];
         a[2] = sw * W.cS()[b + 2];  a[3] = rw * W.cS()[b];
         a[4] = sw * W.cP()[b + 2];  a[5] = rw * W.cP()[b];
@@ -737,9 +816,9 @@ __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X,
 }
 
 // b = (I - hγA)⁻¹ b with stored chain factors, Y columns and capacitance K
-__device__ inline void border_solve(const CosmoDev& C, const Geo& G, const XC& X, const WS& W,
+__device__ inline void border_solve(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
                                           const Cap& K, double hg, double* b, const Thr& T) {
-    Fun F = functionals(C, G, X, W, b, T);
+    Fun F = functionals(C, G, X, Q, b, T);
     double r[3] = {F.S, F.P, F.Pi}, y[3];
     solve3(K, r, y);
     __syncthreads();
@@ -946,11 +1025,11 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
     XC X = make_xc(C, k, xv);
     qfill(C, G, X, W, T);
     __syncthreads();
-    Fun F = functionals(C, G, X, W, W.arg(), T);
+    Fun F = functionals(C, G, X, W.qcur(), W.arg(), T);
     if (want_t && T.wv == 0) {
         int c = source_row(G, T.lane);
         if (c >= 0) {
-            RowC r = row_coef(G, X, W, W.desc()[c], c);
+            RowC r = row_coef(G, X, W, W.qcur(), W.desc()[c], c);
             W.w()[c] = row_apply(r, W.arg(), c, G.n, F.S, F.P, F.Pi);
         }
     }
@@ -1006,7 +1085,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             XC X = make_xc(C, k, x);
             qfill(C, G, X, W, T);
             __syncthreads();
-            Fun F = rhs_rows(C, G, X, W, W.u(), 1.0, W.f(), 0.0, false, T);
+            Fun F = rhs_rows(C, G, X, W, W.qcur(), W.u(), 1.0, W.f(), false, T);
             ++nrhs;
             __syncthreads();
             if (X.rsa && T.tid == 0) rsa_mutate(G, X, k, W.u(), F.P);
@@ -1023,7 +1102,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             XC X1 = make_xc(C, k, x + h0);
             qfill(C, G, X1, W, T);
             __syncthreads();
-            rhs_rows(C, G, X1, W, W.arg(), 1.0, W.w(), 0.0, false, T);
+            rhs_rows(C, G, X1, W, W.qcur(), W.arg(), 1.0, W.w(), false, T);
             ++nrhs;
             __syncthreads();
             for (int c = T.tid; c < G.n; c += T.nt) W.w()[c] -= W.f()[c];
@@ -1046,23 +1125,34 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 W.Z(0)[c] = z0;
                 W.arg()[c] = W.u()[c] + KC_A[1][0] * z0;
             }
+            // the five implicit stage points depend on x and h only: their
+            // coefficients, q-quantities, W bands and LU come first, all stages
+            // at once
+            if (T.tid < 5) store_xc(W.xcs(T.tid), make_xc(C, k, x + KC_C[T.tid + 1] * h));
+            __syncthreads();
+            for (int t = T.tid; t < 5 * G.nq; t += T.nt) {
+                const int st = t / G.nq;
+                qfill_one(C, W, W.xcs(st)[1], W.qst_base(st), t - st * G.nq);
+            }
+            __syncthreads();
+            stage_bands(G, W, hg, T);
+            __syncthreads();
+            chain_factor_all(G, W, T);
+            __syncthreads();
             Cap K;
             XC Xs;
             for (int s = 1; s < 6; ++s) {
-                Xs = make_xc(C, k, x + KC_C[s] * h);
-                qfill(C, G, Xs, W, T);
-                __syncthreads();
-                // rhs = h f(ψ) and W = I - hγL, border columns
-                rhs_rows(C, G, Xs, W, W.arg(), h, W.w(), hg, true, T);
+                Xs = load_xc(W.xcs(s - 1));
+                const QSet Qs = W.qst(s - 1);
+                // rhs = h f(ψ) and the border columns
+                rhs_rows(C, G, Xs, W, Qs, W.arg(), h, W.w(), true, T);
                 ++nrhs;
                 __syncthreads();
-                chain_factor(G, W, T);
-                __syncthreads();
-                chain_solve4(G, W, T);
+                chain_solve4(G, W, s - 1, T);
                 ++nsol;
                 __syncthreads();
                 double y[3];
-                K = stage_border(C, G, Xs, W, hg, T, y);
+                K = stage_border(C, G, Xs, W, Qs, hg, T, y);
                 // one pass: Z_s, then the next stage's argument, or after the last
                 // stage u_{n+1} (stiffly accurate) and the embedded error (into ψ,
                 // which no wavefront reads any more; w still is)
@@ -1086,10 +1176,10 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 __syncthreads();
             }
             // Shampine's smoothed estimate: W⁻¹ e with the last stage's W
-            chain_solve1(G, W, W.arg(), T);
+            chain_solve1(G, W, 4, W.arg(), T);
             ++nsol;
             __syncthreads();
-            border_solve(C, G, Xs, W, K, hg, W.arg(), T);
+            border_solve(C, G, Xs, W, W.qst(4), K, hg, W.arg(), T);
             double en = wrms(G, W, W.arg(), W.u(), W.un(), rt, at, T);
             __syncthreads();
             if (!isfinite(en)) {
@@ -1104,7 +1194,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 XC Xn = make_xc(C, k, xn);
                 qfill(C, G, Xn, W, T);
                 __syncthreads();
-                Fun Fn = rhs_rows(C, G, Xn, W, W.un(), 1.0, W.fn(), 0.0, false, T);
+                Fun Fn = rhs_rows(C, G, Xn, W, W.qcur(), W.un(), 1.0, W.fn(), false, T);
                 ++nrhs;
                 __syncthreads();
                 if (Xn.rsa && T.tid == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
@@ -1158,7 +1248,7 @@ __global__ void __launch_bounds__(WAVE) k_rhs(const CosmoDev* cosmos, const CfgD
     XC X = make_xc(C, md.k, x);
     qfill(C, G, X, W, T);
     __syncthreads();
-    Fun F = rhs_rows(C, G, X, W, W.arg(), 1.0, W.w(), 0.0, false, T);
+    Fun F = rhs_rows(C, G, X, W, W.qcur(), W.arg(), 1.0, W.w(), false, T);
     __syncthreads();
     if (mutate && X.rsa && T.tid == 0) rsa_mutate(G, X, md.k, W.arg(), F.P);
     __syncthreads();
