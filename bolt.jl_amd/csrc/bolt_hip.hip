@@ -539,20 +539,23 @@ __device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, con
     return F;
 }
 
-// W = I - hγL at the five stage points (their XC and q-sets stored), every
-// (stage, row) pair spread over the workgroup; caller syncs
+// W = I - hγL at the five stage points (their XC and q-sets stored): each
+// thread takes its rows (in the evaluation order) through all five stages;
+// caller syncs
 __device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const Thr& T) {
     const int nslot = W.misc()[1];
-    const int* desc = W.desc();
-    const int* order = W.order();
-    for (int t = T.tid; t < 5 * nslot; t += T.nt) {
-        const int s = t / nslot, c = order[t - s * nslot];
+    for (int t = T.tid; t < nslot; t += T.nt) {
+        const int c = W.order()[t];
         if (c < 0) continue;
-        const XC X = load_xc(W.xcs(s));
-        const RowC r = row_coef(G, X, W, W.qst(s), desc[c], c);
-        const Bands B = W.bands(s);
-        B.l1[c] = -hg * r.l1; B.d[c] = 1.0 - hg * r.d; B.u1[c] = -hg * r.u1;
-        if (c < CB) { B.l2c[c] = -hg * r.l2; B.u2c[c] = -hg * r.u2; }
+        const int dsc = W.desc()[c];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            const XC X = load_xc(W.xcs(s));
+            const RowC r = row_coef(G, X, W, W.qst(s), dsc, c);
+            const Bands B = W.bands(s);
+            B.l1[c] = -hg * r.l1; B.d[c] = 1.0 - hg * r.d; B.u1[c] = -hg * r.u1;
+            if (c < CB) { B.l2c[c] = -hg * r.l2; B.u2c[c] = -hg * r.u2; }
+        }
     }
 }
 
