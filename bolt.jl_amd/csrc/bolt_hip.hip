@@ -649,105 +649,128 @@ __device__ inline void chain_factor_all(const Geo& G, const WS& W, const Thr& T)
     }
 }
 
-template <bool GUARD>
-__device__ inline void fwd_tri_block(const double* l1, double* b, int jb, int m, double& y1) {
-    double B[CB], L1[CB];
-#pragma unroll
-    for (int i = 0; i < CB; ++i) { B[i] = b[jb + i]; L1[i] = l1[jb + i]; }
-#pragma unroll
-    for (int i = 0; i < CB; ++i) {
-        double y = B[i] - L1[i] * y1;
-        B[i] = y;
-        y1 = y;
-    }
-#pragma unroll
-    for (int i = 0; i < CB; ++i) if (!GUARD || jb + i < m) b[jb + i] = B[i];
+// ---- substitution by wavefront scans
+// With every chain's first row at l1 = 0 and last row at u1 = 0, the forward
+// and back substitutions over all n rows are each one first-order affine
+// recurrence, y_j = b_j - l1_j·y_{j-1} and x_j = d_j·y_j - d_j·u1_j·x_{j+1}
+// (the chain boundaries cut it by themselves).  A wavefront takes 64·SR rows
+// at a time, SR consecutive rows per lane: each lane composes the maps
+// f_j(t) = a_j·t + b_j of its rows in registers, an inclusive scan across the
+// lanes composes the lane maps (Hillis-Steele in rows of 16 with row_shr 1, 2,
+// 4, 8, then row_bcast 15 and 31 across rows), and each lane takes its
+// incoming value from the previous lane (wave_shr 1).  The two second-band
+// entries (l2 on row 3, u2 on row 1) are added as exact corrections in the
+// lane holding rows 0..3.
+
+// 64-bit DPP move; lanes whose source is invalid or whose row is masked off
+// keep `old`
+template <int CTRL, int RMASK>
+__device__ inline double dpp64_keep(double v, double old) {
+    int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, RMASK, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, RMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// (a, b) <- (a, b) ∘ (a', b') with (a', b') from the lane CTRL names; missing
+// sources act as the identity map (1, 0)
+template <int CTRL, int RMASK>
+__device__ inline void affine_step(double& a, double& b) {
+    const double ap = dpp64_keep<CTRL, RMASK>(a, 1.0);
+    const double bp = dpp64_keep<CTRL, RMASK>(b, 0.0);
+    b = fma(a, bp, b);
+    a = a * ap;
+}
+// lane L ends with f_L ∘ f_{L-1} ∘ ... ∘ f_0; every lane of the wavefront
+// must take part
+__device__ inline void affine_scan(double& a, double& b) {
+    affine_step<0x111, 0xF>(a, b);
+    affine_step<0x112, 0xF>(a, b);
+    affine_step<0x114, 0xF>(a, b);
+    affine_step<0x118, 0xF>(a, b);
+    affine_step<0x142, 0xA>(a, b);
+    affine_step<0x143, 0xC>(a, b);
 }
 
-template <bool GUARD>
-__device__ inline void bwd_tri_block(const double* u1, const double* d, double* b, int jb, int m, double& x1,
-                                     double& x2) {
-    double B[CB], U1[CB], D[CB];
+constexpr int SR = 6;  // rows per lane: 384 rows per pass
+static_assert(SR >= 4, "rows 0..3 must sit in one lane");
+
+// the lane maps of one pass: al/be hold f_i ∘ ... ∘ f_0 of the lane's rows;
+// returns the lane's incoming value and advances carry past the pass
+__device__ inline double pass_scan(double (&al)[SR], double (&be)[SR], double& carry) {
 #pragma unroll
-    for (int i = 0; i < CB; ++i) {
-        bool in = !GUARD || jb + i < m;
-        B[i] = in ? b[jb + i] : 0.0; U1[i] = in ? u1[jb + i] : 0.0; D[i] = in ? d[jb + i] : 0.0;
-    }
-#pragma unroll
-    for (int i = CB - 1; i >= 0; --i) {
-        double x = (B[i] - U1[i] * x1) * D[i];
-        B[i] = x;
-        x2 = x1; x1 = x;
-    }
-#pragma unroll
-    for (int i = 0; i < CB; ++i) if (!GUARD || jb + i < m) b[jb + i] = B[i];
+    for (int i = 1; i < SR; ++i) { be[i] = fma(al[i], be[i - 1], be[i]); al[i] = al[i] * al[i - 1]; }
+    double A = al[SR - 1], B = be[SR - 1];
+    affine_scan(A, B);
+    const double last = fma(A, carry, B);
+    const double tin = dpp64_keep<0x138, 0xF>(last, carry);  // wave_shr 1; lane 0 keeps carry
+    carry = readlane64(last, WAVE - 1);
+    return tin;
 }
 
-// forward + back substitution of one segment for one right-hand side, in
-// blocks (the first block with both bands, the others tridiagonal)
-__device__ inline void chain_subst(const Bands& Bd, int c0, int m, double* bvec) {
-    const double* l1 = Bd.l1 + c0; const double* d = Bd.d + c0; const double* u1 = Bd.u1 + c0;
-    const bool first = c0 == 0;
-    double* b = bvec + c0;
-    double y1 = 0.0, y2 = 0.0;
-    {
-        double B[CB], L1[CB], L2[CB];
+// forward + back substitution of one right-hand side over all n rows by one
+// wavefront (all 64 lanes, uniform control flow)
+__device__ inline void wave_subst(const Bands& Bd, int n, double* bv, int lane) {
+    constexpr int span = WAVE * SR;
+    double carry = 0.0;
+    for (int p0 = 0; p0 < n; p0 += span) {
+        double al[SR], be[SR];
+        const int j0 = p0 + SR * lane;
 #pragma unroll
-        for (int i = 0; i < CB; ++i) { B[i] = b[i]; L1[i] = l1[i]; L2[i] = first ? Bd.l2c[i] : 0.0; }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            double y = (B[i] - L2[i] * y2) - L1[i] * y1;
-            B[i] = y;
-            y2 = y1; y1 = y;
+        for (int i = 0; i < SR; ++i) {
+            const bool in = j0 + i < n;
+            al[i] = in ? -Bd.l1[j0 + i] : 0.0;
+            be[i] = in ? bv[j0 + i] : 0.0;
         }
+        // row 3 (Θ₁) also takes -l2·y_1 with y_1 = b_1 - l1_1·b_0
+        if (p0 == 0 && lane == 0) be[3] -= Bd.l2c[3] * fma(al[1], be[0], be[1]);
+        const double tin = pass_scan(al, be, carry);
 #pragma unroll
-        for (int i = 0; i < CB; ++i) if (i < m) b[i] = B[i];
+        for (int i = 0; i < SR; ++i)
+            if (j0 + i < n) bv[j0 + i] = fma(al[i], tin, be[i]);
     }
-    int jb = CB;
-    for (; jb + CB <= m; jb += CB) fwd_tri_block<false>(l1, b, jb, m, y1);
-    const bool tail = jb < m;
-    if (tail) fwd_tri_block<true>(l1, b, jb, m, y1);
-    // backward: the tail, the full blocks downward, then the first block
-    double x1 = 0.0, x2 = 0.0;
-    if (tail) bwd_tri_block<true>(u1, d, b, jb, m, x1, x2);
-    for (jb -= CB; jb >= CB; jb -= CB) bwd_tri_block<false>(u1, d, b, jb, m, x1, x2);
-    {
-        double B[CB], U1[CB], U2[CB], D[CB];
+    // backward: pass p covers rows p·span .. p·span + span - 1, lane order
+    // running down the rows
+    carry = 0.0;
+    for (int p = (n - 1) / span; p >= 0; --p) {
+        double al[SR], be[SR];
+        const int jt = p * span + span - 1 - SR * lane;  // the lane's highest row
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
-            bool in = i < m;
-            B[i] = in ? b[i] : 0.0; U1[i] = in ? u1[i] : 0.0; U2[i] = (in && first) ? Bd.u2c[i] : 0.0;
-            D[i] = in ? d[i] : 0.0;
+        for (int i = 0; i < SR; ++i) {
+            const bool in = jt - i < n;
+            const double dj = in ? Bd.d[jt - i] : 0.0;
+            al[i] = in ? -dj * Bd.u1[jt - i] : 0.0;
+            be[i] = in ? dj * bv[jt - i] : 0.0;
+        }
+        const bool low = p == 0 && lane == WAVE - 1;  // rows SR-1 .. 0
+        const double y0 = low ? bv[0] : 0.0, y1 = low ? bv[1] : 0.0;
+        const double tin = pass_scan(al, be, carry);
+        double x[SR];
+#pragma unroll
+        for (int i = 0; i < SR; ++i) x[i] = fma(al[i], tin, be[i]);
+        if (low) {
+            // row 1 (v_b) also takes -u2·x_3; row 0 then follows from row 1
+            x[SR - 2] = ((y1 - Bd.u2c[1] * x[SR - 4]) - Bd.u1[1] * x[SR - 3]) * Bd.d[1];
+            x[SR - 1] = (y0 - Bd.u1[0] * x[SR - 2]) * Bd.d[0];
         }
 #pragma unroll
-        for (int i = CB - 1; i >= 0; --i) {
-            double x = ((B[i] - U2[i] * x2) - U1[i] * x1) * D[i];
-            B[i] = x;
-            x2 = x1; x1 = x;
-        }
-#pragma unroll
-        for (int i = 0; i < CB; ++i) if (i < m) b[i] = B[i];
+        for (int i = 0; i < SR; ++i)
+            if (jt - i < n) bv[jt - i] = x[i];
     }
 }
 
-// the four right-hand sides w, c_S, c_P, c_Π after chain_factor: one
-// (segment, rhs) unit per lane; with four wavefronts, wavefront r takes
-// right-hand side r
+// the four right-hand sides w, c_S, c_P, c_Π after chain_factor: wavefront r
+// takes right-hand side r
 __device__ inline void chain_solve4(const Geo& G, const WS& W, int st, const Thr& T) {
-    const int nseg = W.misc()[0];
     const Bands Bd = W.bands(st);
-    for (int u = T.tid; u < 4 * WAVE; u += T.nt) {
-        const int r = u / WAVE, sg = u % WAVE;
-        if (sg >= nseg) continue;
+    for (int r = T.wv; r < 4; r += T.nt / WAVE) {
         double* bv = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
-        chain_subst(Bd, W.seg()[2 * sg], W.seg()[2 * sg + 1], bv);
+        wave_subst(Bd, G.n, bv, T.lane);
     }
 }
 
-// one right-hand side with the stored factors, one lane per segment
+// one right-hand side with the stored factors, by the first wavefront
 __device__ inline void chain_solve1(const Geo& G, const WS& W, int st, double* bvec, const Thr& T) {
-    if (T.tid >= W.misc()[0]) return;
-    chain_subst(W.bands(st), W.seg()[2 * T.tid], W.seg()[2 * T.tid + 1], bvec);
+    if (T.wv != 0) return;
+    wave_subst(W.bands(st), G.n, bvec, T.lane);
 }
 
 // 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
