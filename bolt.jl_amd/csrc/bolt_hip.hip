@@ -231,6 +231,12 @@ struct QSet {
     const double* rw;
 };
 
+// The x-dependent scalars every row coefficient is built from, one table per
+// x point: kH = k/ℋ, τ′, 1/(ℋη), τ′R, kH·c_s², 1, the RSA factor of the
+// photon/ν rows (0 when RSA is on), then kH·q_i/ε_i and
+// kH·(ε_i/q_i)·dlnf₀/dlnq (nq each)
+enum { K_KH = 0, K_TP, K_IHETA, K_TPR, K_KHCS, K_ONE, K_RSA, K_Q0 };
+
 // the LU of W = I - hγL at one stage point: l1 <- multipliers, d <- 1/pivot,
 // u1 <- upper entries; the second band lives only in the first block of the
 // first segment (rows 0..CB-1), kept in l2c/u2c
@@ -246,12 +252,13 @@ struct Bands {
 #define NXC 24    // doubles per stored XC
 
 // LDS workspace of one k-mode.  Vectors of n doubles: u, f, u_{n+1}, f_{n+1}, ψ,
-// w, Z₀..Z₅, the border columns c_S, c_P, c_Π, the ℓ couplings ℓ/(2ℓ+1) and
-// (ℓ+1)/(2ℓ+1), and the LU of W at the five implicit stage points (3 vectors
-// each).  Then the q arrays (mode constants, the current x, the five stage
-// points), the compact second bands, the five stage points' coefficients,
-// cross-wavefront partial sums, and the integer tables (row descriptors, row
-// evaluation order, segments).  Every pointer is base + offset arithmetic.
+// w, Z₀..Z₅, the border columns c_S, c_P, c_Π, the row recipes (nine
+// coefficient vectors), and the LU of W at the five implicit stage points (3
+// vectors each).  Then the q arrays (mode constants, the current x, the five
+// stage points), the compact second bands, the five stage points' coefficients,
+// the six scalar tables (five stage points and the current x), cross-wavefront
+// partial sums, and the integer tables (row descriptors, recipe kinds,
+// segments).  Every pointer is base + offset arithmetic.
 struct WS {
     double* b;
     int n, nq;
@@ -265,35 +272,37 @@ struct WS {
     __device__ double* cS() const { return b + 12 * n; }
     __device__ double* cP() const { return b + 13 * n; }
     __device__ double* cPi() const { return b + 14 * n; }
-    __device__ double* ra() const { return b + 15 * n; }   // ℓ/(2ℓ+1)
-    __device__ double* rb() const { return b + 16 * n; }   // (ℓ+1)/(2ℓ+1)
+    __device__ double* rcp_(int j) const { return b + (15 + j) * n; }  // recipe vector j (RC_*)
     __device__ Bands bands(int s) const {
         Bands B;
-        B.l1 = b + (17 + 3 * s) * n; B.d = b + (18 + 3 * s) * n; B.u1 = b + (19 + 3 * s) * n;
-        B.l2c = b + 32 * n + 27 * nq + 2 * CB * s;
+        B.l1 = b + (24 + 3 * s) * n; B.d = b + (25 + 3 * s) * n; B.u1 = b + (26 + 3 * s) * n;
+        B.l2c = b + 39 * n + 27 * nq + 2 * CB * s;
         B.u2c = B.l2c + CB;
         return B;
     }
-    __device__ double* q() const { return b + 32 * n; }
-    __device__ double* df0() const { return b + 32 * n + nq; }
-    __device__ double* base() const { return b + 32 * n + 2 * nq; }
-    __device__ double* qe() const { return b + 32 * n + 3 * nq; }
-    __device__ double* eqdf() const { return b + 32 * n + 4 * nq; }
-    __device__ double* sw() const { return b + 32 * n + 5 * nq; }
-    __device__ double* rw() const { return b + 32 * n + 6 * nq; }
-    __device__ double* qst_base(int s) const { return b + 32 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
+    __device__ double* q() const { return b + 39 * n; }
+    __device__ double* df0() const { return b + 39 * n + nq; }
+    __device__ double* base() const { return b + 39 * n + 2 * nq; }
+    __device__ double* qe() const { return b + 39 * n + 3 * nq; }
+    __device__ double* eqdf() const { return b + 39 * n + 4 * nq; }
+    __device__ double* sw() const { return b + 39 * n + 5 * nq; }
+    __device__ double* rw() const { return b + 39 * n + 6 * nq; }
+    __device__ double* qst_base(int s) const { return b + 39 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
     __device__ QSet qcur() const { QSet Q; Q.qe = qe(); Q.eqdf = eqdf(); Q.sw = sw(); Q.rw = rw(); return Q; }
     __device__ QSet qst(int s) const {
         const double* p = qst_base(s);
         QSet Q; Q.qe = p; Q.eqdf = p + nq; Q.sw = p + 2 * nq; Q.rw = p + 3 * nq;
         return Q;
     }
-    __device__ double* xcs(int s) const { return b + 32 * n + 27 * nq + 10 * CB + NXC * s; }
-    __device__ double* red() const { return b + 32 * n + 27 * nq + 10 * CB + 5 * NXC; }  // [8]
-    __device__ int* desc() const { return (int*)(b + 32 * n + 27 * nq + 10 * CB + 5 * NXC + 8); }
-    __device__ int* order() const { return desc() + n; }            // [n + NTHR]
-    __device__ int* seg() const { return desc() + 2 * n + NTHR; }   // [MAXSEG][2]: c0, m
-    __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg, nslot
+    __device__ double* xcs(int s) const { return b + 39 * n + 27 * nq + 10 * CB + NXC * s; }
+    // scalar table of stage point s (s < 5) or of the current x (s = 5)
+    __device__ double* ktab(int s) const { return b + 39 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
+    __device__ double* kcur() const { return ktab(5); }
+    __device__ double* red() const { return ktab(6); }  // [8]
+    __device__ int* desc() const { return (int*)(red() + 8); }
+    __device__ int* kinds() const { return desc() + n; }
+    __device__ int* seg() const { return desc() + 2 * n; }          // [MAXSEG][2]: c0, m
+    __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg
 };
 
 __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
@@ -302,8 +311,8 @@ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
     W.nq = nq;
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
-    return sizeof(double) * ((size_t)32 * n + 27 * nq + 10 * CB + 5 * NXC + 8) +
-           sizeof(int) * (size_t)(2 * n + NTHR + 2 * MAXSEG + 4);
+    return sizeof(double) * ((size_t)39 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 8) +
+           sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4);
 }
 
 __device__ inline void store_xc(double* p, const XC& X) {
@@ -350,47 +359,70 @@ __device__ __host__ inline int row_desc(const Geo& G, int c) {
     return pack_desc(R_PHI, 0, 0);
 }
 
-__device__ inline bool interior_type(int type) {
-    return type == R_TL || type == R_PL || type == R_NL || type == R_ML;
+// Row recipes (perturbations.jl:196-265).  Every coefficient of row c is a
+// row constant times one table scalar: l1 = al·K[kl], u1 = au·K[ku],
+// d = dc + dt·τ′ + di/(ℋη) + dr·τ′R, c_S = cs·K[kcs], c_P = cp, c_Π = cpi·τ′,
+// all times K[km] (the RSA factor on photon/ν rows, 1 elsewhere).  The
+// second band is v_b's u2 = 3τ′R (row 1) and Θ₁'s l2 = τ′/3 (row 3).
+enum { RC_AL = 0, RC_AU, RC_DC, RC_DT, RC_DI, RC_DR, RC_CS, RC_CP, RC_CPI, RC_N };
+
+__device__ inline void row_recipe(const CosmoDev& C, const Geo& G, const WS& W, int c) {
+    const int dsc = W.desc()[c];
+    const int type = dsc & 31, l = (dsc >> 5) & 2047, qi = dsc >> 16;
+    const double third = 1.0 / 3.0;
+    const bool photon = type >= R_T0 && type <= R_PN;
+    const bool photon_nu = type >= R_T0 && type <= R_NN;
+    const bool massive = type >= R_M0 && type <= R_MN;
+    const int kq = massive ? K_Q0 + qi : K_KH;
+    double r[RC_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int kl = K_KH, ku = K_KH, kcs = K_KH;
+    if (type == R_TL || type == R_PL || type == R_NL || type == R_ML) {
+        // interior of a hierarchy: ℓ/(2ℓ+1) and (ℓ+1)/(2ℓ+1) couplings
+        kl = ku = kq;
+        r[RC_AL] = (double)l / (double)(2 * l + 1);
+        r[RC_AU] = -((double)(l + 1) / (double)(2 * l + 1));
+        r[RC_DT] = photon ? 1.0 : 0.0;
+        r[RC_CPI] = (photon && l == 2) ? -0.1 : 0.0;
+    } else if (type == R_TN || type == R_PN || type == R_NN || type == R_MN) {
+        // truncation rows
+        kl = kq;
+        r[RC_AL] = 1.0;
+        r[RC_DI] = -(double)((type == R_NN ? G.nn : type == R_MN ? G.nm : G.ng) + 1);
+        r[RC_DT] = photon ? 1.0 : 0.0;
+    } else {
+        switch (type) {
+            case R_DB: r[RC_AU] = 1.0; r[RC_CP] = -3.0; break;
+            case R_VB: kl = K_KHCS; r[RC_AL] = -1.0; r[RC_DC] = -1.0; r[RC_DR] = 1.0; r[RC_CS] = -1.0; break;
+            case R_T0: case R_N0: r[RC_AU] = -1.0; r[RC_CP] = -1.0; break;
+            case R_T1: r[RC_AL] = third; r[RC_DT] = 1.0; r[RC_AU] = -2.0 * third; r[RC_CS] = third; break;
+            case R_P0: r[RC_DT] = 1.0; r[RC_AU] = -1.0; r[RC_CPI] = -0.5; break;
+            case R_N1: r[RC_AL] = third; r[RC_AU] = -2.0 * third; r[RC_CS] = third; break;
+            case R_M0: ku = kq; r[RC_AU] = -1.0; r[RC_CP] = C.df0[qi]; break;
+            case R_M1:
+                kl = ku = kq; kcs = K_Q0 + G.nq + qi;
+                r[RC_AL] = third; r[RC_AU] = -2.0 * third; r[RC_CS] = -third;
+                break;
+            case R_DEL: r[RC_AU] = 1.0; r[RC_CP] = -3.0; break;
+            case R_V: r[RC_DC] = -1.0; r[RC_CS] = -1.0; break;
+            default: r[RC_CP] = 1.0; break;  // R_PHI
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RC_N; ++j) W.rcp_(j)[c] = r[j];
+    W.kinds()[c] = kl | (ku << 8) | (kcs << 16) | ((photon_nu ? K_RSA : K_ONE) << 24);
 }
 
-// per-mode row tables (caller syncs):
-//  * descriptors and the ℓ couplings ℓ/(2ℓ+1), (ℓ+1)/(2ℓ+1) of every row;
-//  * the evaluation order of the row pass: interior hierarchy rows (one
-//    uniform formula) in the full passes, boundary rows grouped by type in the
-//    last pass, spread evenly over the wavefronts (slot -1 = idle lane);
-//  * the segments of the banded solve: consecutive chains packed greedily up
-//    to the longest chain's length.  Chains never couple to each other (the
-//    first row of a chain has l1 = l2 = 0, the last u1 = u2 = 0), so the LU of
-//    a segment is that of its chains and one lane can sweep a whole segment.
-__device__ inline void setup_rows(const Geo& G, const WS& W, const Thr& T) {
+// per-mode row tables (caller syncs): descriptors and recipes of every row,
+// and the segments of the banded LU: consecutive chains packed greedily up to
+// the longest chain's length.  Chains never couple to each other (the first
+// row of a chain has l1 = l2 = 0, the last u1 = u2 = 0), so the LU of a
+// segment is that of its chains and one lane can sweep a whole segment.
+__device__ inline void setup_rows(const CosmoDev& C, const Geo& G, const WS& W, const Thr& T) {
     for (int c = T.tid; c < G.n; c += T.nt) {
-        int dsc = row_desc(G, c), l = (dsc >> 5) & 2047;
-        W.desc()[c] = dsc;
-        W.ra()[c] = (double)l / (double)(2 * l + 1);
-        W.rb()[c] = (double)(l + 1) / (double)(2 * l + 1);
+        W.desc()[c] = row_desc(G, c);
+        row_recipe(C, G, W, c);
     }
     if (T.tid == 0) {
-        int* ord = W.order();
-        const int nfull = G.n / T.nt, r = G.n - nfull * T.nt, nw = T.nt / WAVE;
-        const int nslot = nfull * T.nt + (r > 0 ? T.nt : 0);
-        for (int t = nfull * T.nt; t < nslot; ++t) ord[t] = -1;
-        // rows in list order: interior rows, then boundary rows by type
-        int t = 0, w = 0, wi = 0;
-        for (int pass = 0; pass <= R_PHI + 1; ++pass)
-            for (int c = 0; c < G.n; ++c) {
-                int type = row_desc(G, c) & 31;
-                bool take = pass == 0 ? interior_type(type) : (!interior_type(type) && type == pass - 1);
-                if (!take) continue;
-                if (t < nfull * T.nt) {
-                    ord[t++] = c;
-                } else {
-                    int cnt = r / nw + (w < r % nw ? 1 : 0);
-                    while (wi >= cnt) { ++w; wi = 0; cnt = r / nw + (w < r % nw ? 1 : 0); }
-                    ord[nfull * T.nt + w * WAVE + wi] = c;
-                    ++wi;
-                }
-            }
         int lmax = 0;
         for (int ch = 0; ch < G.nch; ++ch) {
             int c0, m;
@@ -410,88 +442,58 @@ __device__ inline void setup_rows(const Geo& G, const WS& W, const Thr& T) {
         }
         W.seg()[2 * ns] = cur0; W.seg()[2 * ns + 1] = curm; ++ns;
         W.misc()[0] = ns;
-        W.misc()[1] = nslot;
     }
 }
 
 // per-x q-quantities of bin i at scale factor a (perturbations.jl:203-213,
-// 127-145) into the arrays at p (qe, eqdf, sw, rw, nq apart)
-__device__ inline void qfill_one(const CosmoDev& C, const WS& W, double a, double* p, int i) {
+// 127-145) into the arrays at p (qe, eqdf, sw, rw, nq apart), and their
+// entries of the scalar table K (kH = k/ℋ at the same x)
+__device__ inline void qfill_one(const CosmoDev& C, const WS& W, double a, double kH, double* p, double* K, int i) {
     double q = W.q()[i];
     double am = a * C.mnu;
     double eps = sqrt(q * q + am * am);
     double ie = 1.0 / eps;
-    p[i] = q * ie;
-    p[W.nq + i] = eps / q * W.df0()[i];
+    const double qe = q * ie, eqdf = eps / q * W.df0()[i];
+    p[i] = qe;
+    p[W.nq + i] = eqdf;
     p[2 * W.nq + i] = W.base()[i] * (q * q * ie);
     p[3 * W.nq + i] = W.base()[i] * eps;
+    K[K_Q0 + i] = kH * qe;
+    K[K_Q0 + W.nq + i] = kH * eqdf;
 }
 
-// the current-x set; threads < nq; caller syncs
+// the x-only entries of a scalar table
+__device__ inline void ktab_head(double* K, const XC& X) {
+    K[K_KH] = X.kH;
+    K[K_TP] = X.tp;
+    K[K_IHETA] = X.iHeta;
+    K[K_TPR] = X.tp * X.R;
+    K[K_KHCS] = X.kH * X.csb2;
+    K[K_ONE] = 1.0;
+    K[K_RSA] = X.rsa ? 0.0 : 1.0;
+}
+
+// the current-x set and scalar table (every thread holds the same X); caller
+// syncs
 __device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const Thr& T) {
-    if (T.tid < G.nq) qfill_one(C, W, X.a, W.qe(), T.tid);
+    if (T.tid < G.nq) qfill_one(C, W, X.a, X.kH, W.qe(), W.kcur(), T.tid);
+    if (T.tid == T.nt - 1) ktab_head(W.kcur(), X);
 }
 
 // One row of du = L u + c_S Ψ + c_P Φ′ + c_Π Π (perturbations.jl:196-265)
-struct RowC {
-    double l2, l1, d, u1, u2, cs, cp, cpi;
-};
-
-__device__ inline RowC row_coef(const Geo& G, const XC& X, const WS& W, const QSet& Q, int desc, int c) {
-    RowC r = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int type = desc & 31, l = (desc >> 5) & 2047, qi = desc >> 16;
-    const double kH = X.kH, tp = X.tp;
-    const double third = 1.0 / 3.0;
-    const bool photon = type >= R_T0 && type <= R_PN;
-    const bool photon_nu = type >= R_T0 && type <= R_NN;
-    if (photon_nu && X.rsa) return r;  // RSA: Θ′ = Θᵖ′ = 𝒩′ = 0 (perturbations.jl:229-232)
-    if (type == R_TL || type == R_PL || type == R_NL || type == R_ML) {
-        // interior of a hierarchy: ℓ/(2ℓ+1) and (ℓ+1)/(2ℓ+1) couplings
-        double kf = (type == R_ML) ? kH * Q.qe[qi] : kH;
-        r.l1 = kf * W.ra()[c];
-        r.u1 = -kf * W.rb()[c];
-        r.d = photon ? tp : 0.0;
-        r.cpi = (photon && l == 2) ? -tp * 0.1 : 0.0;
-        return r;
-    }
-    if (type == R_TN || type == R_PN || type == R_NN || type == R_MN) {
-        // truncation rows
-        double kf = (type == R_MN) ? kH * Q.qe[qi] : kH;
-        // value selects (a select between struct fields would index the struct)
-        const double Lg = (double)(G.ng + 1), Ln = (double)(G.nn + 1), Lm = (double)(G.nm + 1);
-        double L1 = (type == R_NN) ? Ln : Lg;
-        L1 = (type == R_MN) ? Lm : L1;
-        r.l1 = kf;
-        r.d = -L1 * X.iHeta + (photon ? tp : 0.0);
-        return r;
-    }
-    switch (type) {
-        case R_DB: r.u1 = kH; r.cp = -3.0; break;
-        case R_VB: r.l1 = -kH * X.csb2; r.d = -1.0 + tp * X.R; r.u2 = 3.0 * tp * X.R; r.cs = -kH; break;
-        case R_T0: case R_N0: r.u1 = -kH; r.cp = -1.0; break;
-        case R_T1: r.l2 = tp * third; r.l1 = kH * third; r.d = tp; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
-        case R_P0: r.d = tp; r.u1 = -kH; r.cpi = -tp * 0.5; break;
-        case R_N1: r.l1 = kH * third; r.u1 = -2.0 * kH * third; r.cs = kH * third; break;
-        case R_M0: r.u1 = -kH * Q.qe[qi]; r.cp = W.df0()[qi]; break;
-        case R_M1: {
-            double qe = Q.qe[qi];
-            r.l1 = kH * qe * third; r.u1 = -2.0 * kH * qe * third; r.cs = -kH * third * Q.eqdf[qi];
-            break;
-        }
-        case R_DEL: r.u1 = kH; r.cp = -3.0; break;
-        case R_V: r.d = -1.0; r.cs = -kH; break;
-        default: r.cp = 1.0; break;  // R_PHI
-    }
-    return r;
-}
-
-__device__ inline double row_apply(const RowC& r, const double* v, int c, int n, double S, double P, double Pi) {
-    double acc = r.d * v[c];
-    if (c >= 1) acc += r.l1 * v[c - 1];
-    if (c >= 2) acc += r.l2 * v[c - 2];
-    if (c + 1 < n) acc += r.u1 * v[c + 1];
-    if (c + 2 < n) acc += r.u2 * v[c + 2];
-    return acc + r.cs * S + r.cp * P + r.cpi * Pi;
+// from its recipe and the scalar table K of the x point
+__device__ inline double row_eval(const WS& W, const double* K, const double* v, int c, int n, double S, double P,
+                                  double Pi) {
+    const int kd = W.kinds()[c];
+    const double d = W.rcp_(RC_DC)[c] + W.rcp_(RC_DT)[c] * K[K_TP] + W.rcp_(RC_DI)[c] * K[K_IHETA] +
+                     W.rcp_(RC_DR)[c] * K[K_TPR];
+    double acc = d * v[c];
+    if (c >= 1) acc += (W.rcp_(RC_AL)[c] * K[kd & 255]) * v[c - 1];
+    if (c + 1 < n) acc += (W.rcp_(RC_AU)[c] * K[(kd >> 8) & 255]) * v[c + 1];
+    if (c == 1) acc += (3.0 * K[K_TPR]) * v[3];
+    if (c == 3) acc += (K[K_TP] * (1.0 / 3.0)) * v[1];
+    acc += (W.rcp_(RC_CS)[c] * K[(kd >> 16) & 255]) * S + W.rcp_(RC_CP)[c] * P + (W.rcp_(RC_CPI)[c] * K[K_TP]) * Pi;
+    return K[kd >> 24] * acc;
 }
 
 // Ψ, Φ′, Π of chain-ordered vectors (perturbations.jl:184-194, 247)
@@ -522,39 +524,45 @@ __device__ inline Fun functionals(const CosmoDev& C, const Geo& G, const XC& X, 
 }
 
 // du = f(v) on all rows (out = scale·du), optionally writing the border
-// columns c_S, c_P, c_Π; returns the functionals of v
+// columns c_S, c_P, c_Π; K is the scalar table of the same x as X and Q;
+// returns the functionals of v
 __device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
-                               const double* v, double scale, double* out, bool make_c, const Thr& T) {
+                               const double* K, const double* v, double scale, double* out, bool make_c,
+                               const Thr& T) {
     Fun F = functionals(C, G, X, Q, v, T);
-    const int* desc = W.desc();
-    const int* order = W.order();
-    const int nslot = W.misc()[1];
-    for (int t = T.tid; t < nslot; t += T.nt) {
-        const int c = order[t];
-        if (c < 0) continue;
-        RowC r = row_coef(G, X, W, Q, desc[c], c);
-        out[c] = scale * row_apply(r, v, c, G.n, F.S, F.P, F.Pi);
-        if (make_c) { W.cS()[c] = r.cs; W.cP()[c] = r.cp; W.cPi()[c] = r.cpi; }
+    for (int c = T.tid; c < G.n; c += T.nt) {
+        out[c] = scale * row_eval(W, K, v, c, G.n, F.S, F.P, F.Pi);
+        if (make_c) {
+            const int kd = W.kinds()[c];
+            const double m = K[kd >> 24];
+            W.cS()[c] = m * (W.rcp_(RC_CS)[c] * K[(kd >> 16) & 255]);
+            W.cP()[c] = m * W.rcp_(RC_CP)[c];
+            W.cPi()[c] = m * (W.rcp_(RC_CPI)[c] * K[K_TP]);
+        }
     }
     return F;
 }
 
-// W = I - hγL at the five stage points (their XC and q-sets stored): each
-// thread takes its rows (in the evaluation order) through all five stages;
-// caller syncs
+// W = I - hγL at the five stage points (their scalar tables stored): each
+// thread takes its rows through all five stages; caller syncs
 __device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const Thr& T) {
-    const int nslot = W.misc()[1];
-    for (int t = T.tid; t < nslot; t += T.nt) {
-        const int c = W.order()[t];
-        if (c < 0) continue;
-        const int dsc = W.desc()[c];
+    for (int c = T.tid; c < G.n; c += T.nt) {
+        const int kd = W.kinds()[c];
+        const int kl = kd & 255, ku = (kd >> 8) & 255, km = kd >> 24;
+        const double al = W.rcp_(RC_AL)[c], au = W.rcp_(RC_AU)[c], dc = W.rcp_(RC_DC)[c], dt = W.rcp_(RC_DT)[c],
+                     di = W.rcp_(RC_DI)[c], dr = W.rcp_(RC_DR)[c];
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            const XC X = load_xc(W.xcs(s));
-            const RowC r = row_coef(G, X, W, W.qst(s), dsc, c);
+            const double* K = W.ktab(s);
+            const double mh = K[km] * hg;
             const Bands B = W.bands(s);
-            B.l1[c] = -hg * r.l1; B.d[c] = 1.0 - hg * r.d; B.u1[c] = -hg * r.u1;
-            if (c < CB) { B.l2c[c] = -hg * r.l2; B.u2c[c] = -hg * r.u2; }
+            B.l1[c] = -mh * (al * K[kl]);
+            B.d[c] = 1.0 - mh * (dc + dt * K[K_TP] + di * K[K_IHETA] + dr * K[K_TPR]);
+            B.u1[c] = -mh * (au * K[ku]);
+            if (c < CB) {
+                B.l2c[c] = c == 3 ? -mh * (K[K_TP] * (1.0 / 3.0)) : 0.0;
+                B.u2c[c] = c == 1 ? -mh * (3.0 * K[K_TPR]) : 0.0;
+            }
         }
     }
 }
@@ -1055,8 +1063,7 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
     if (want_t && T.wv == 0) {
         int c = source_row(G, T.lane);
         if (c >= 0) {
-            RowC r = row_coef(G, X, W, W.qcur(), W.desc()[c], c);
-            W.w()[c] = row_apply(r, W.arg(), c, G.n, F.S, F.P, F.Pi);
+            W.w()[c] = row_eval(W, W.kcur(), W.arg(), c, G.n, F.S, F.P, F.Pi);
         }
     }
     __syncthreads();
@@ -1089,7 +1096,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
         const double k = md.k;
         if (T.tid < G.nq) { W.q()[T.tid] = C.qv[T.tid]; W.df0()[T.tid] = C.df0[T.tid]; W.base()[T.tid] = C.base[T.tid]; }
         for (int c = T.tid; c < G.n; c += T.nt) W.u()[c] = 0.0;
-        setup_rows(G, W, T);
+        setup_rows(C, G, W, T);
         __syncthreads();
         initial_conditions(C, G, W, k, W.u(), T);
         __syncthreads();
@@ -1111,7 +1118,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             XC X = make_xc(C, k, x);
             qfill(C, G, X, W, T);
             __syncthreads();
-            Fun F = rhs_rows(C, G, X, W, W.qcur(), W.u(), 1.0, W.f(), false, T);
+            Fun F = rhs_rows(C, G, X, W, W.qcur(), W.kcur(), W.u(), 1.0, W.f(), false, T);
             ++nrhs;
             __syncthreads();
             if (X.rsa && T.tid == 0) rsa_mutate(G, X, k, W.u(), F.P);
@@ -1128,7 +1135,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             XC X1 = make_xc(C, k, x + h0);
             qfill(C, G, X1, W, T);
             __syncthreads();
-            rhs_rows(C, G, X1, W, W.qcur(), W.arg(), 1.0, W.w(), false, T);
+            rhs_rows(C, G, X1, W, W.qcur(), W.kcur(), W.arg(), 1.0, W.w(), false, T);
             ++nrhs;
             __syncthreads();
             for (int c = T.tid; c < G.n; c += T.nt) W.w()[c] -= W.f()[c];
@@ -1154,11 +1161,15 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             // the five implicit stage points depend on x and h only: their
             // coefficients, q-quantities, W bands and LU come first, all stages
             // at once
-            if (T.tid < 5) store_xc(W.xcs(T.tid), make_xc(C, k, x + KC_C[T.tid + 1] * h));
+            if (T.tid < 5) {
+                const XC Xs = make_xc(C, k, x + KC_C[T.tid + 1] * h);
+                store_xc(W.xcs(T.tid), Xs);
+                ktab_head(W.ktab(T.tid), Xs);
+            }
             __syncthreads();
             for (int t = T.tid; t < 5 * G.nq; t += T.nt) {
                 const int st = t / G.nq;
-                qfill_one(C, W, W.xcs(st)[1], W.qst_base(st), t - st * G.nq);
+                qfill_one(C, W, W.xcs(st)[1], W.xcs(st)[7], W.qst_base(st), W.ktab(st), t - st * G.nq);
             }
             __syncthreads();
             stage_bands(G, W, hg, T);
@@ -1171,7 +1182,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 Xs = load_xc(W.xcs(s - 1));
                 const QSet Qs = W.qst(s - 1);
                 // rhs = h f(ψ) and the border columns
-                rhs_rows(C, G, Xs, W, Qs, W.arg(), h, W.w(), true, T);
+                rhs_rows(C, G, Xs, W, Qs, W.ktab(s - 1), W.arg(), h, W.w(), true, T);
                 ++nrhs;
                 __syncthreads();
                 chain_solve4(G, W, s - 1, T);
@@ -1220,7 +1231,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 XC Xn = make_xc(C, k, xn);
                 qfill(C, G, Xn, W, T);
                 __syncthreads();
-                Fun Fn = rhs_rows(C, G, Xn, W, W.qcur(), W.un(), 1.0, W.fn(), false, T);
+                Fun Fn = rhs_rows(C, G, Xn, W, W.qcur(), W.kcur(), W.un(), 1.0, W.fn(), false, T);
                 ++nrhs;
                 __syncthreads();
                 if (Xn.rsa && T.tid == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
@@ -1270,11 +1281,11 @@ __global__ void __launch_bounds__(WAVE) k_rhs(const CosmoDev* cosmos, const CfgD
     ws_bind(W, lds, G.n, G.nq);
     if (T.tid < G.nq) { W.q()[T.tid] = C.qv[T.tid]; W.df0()[T.tid] = C.df0[T.tid]; W.base()[T.tid] = C.base[T.tid]; }
     for (int c = T.tid; c < G.n; c += T.nt) W.arg()[c] = u[(size_t)chain_to_state(G, c) * B + mi];
-    setup_rows(G, W, T);
+    setup_rows(C, G, W, T);
     XC X = make_xc(C, md.k, x);
     qfill(C, G, X, W, T);
     __syncthreads();
-    Fun F = rhs_rows(C, G, X, W, W.qcur(), W.arg(), 1.0, W.w(), false, T);
+    Fun F = rhs_rows(C, G, X, W, W.qcur(), W.kcur(), W.arg(), 1.0, W.w(), false, T);
     __syncthreads();
     if (mutate && X.rsa && T.tid == 0) rsa_mutate(G, X, md.k, W.arg(), F.P);
     __syncthreads();
