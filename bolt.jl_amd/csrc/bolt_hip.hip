@@ -172,6 +172,22 @@ __device__ inline double wave_sum(double v) {
     v += dpp64<0x118>(v);
     return ((readlane64(v, 15) + readlane64(v, 31)) + readlane64(v, 47)) + readlane64(v, 63);
 }
+// N sums at once over values held by lanes 0..15 only (other lanes' values are
+// ignored): interleaved scans inside row 0, the totals read from lane 15; the
+// same value (same rounding) in every lane
+template <int N>
+__device__ inline void row0_sums(double (&a)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] += dpp64<0x111>(a[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] += dpp64<0x112>(a[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] += dpp64<0x114>(a[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] += dpp64<0x118>(a[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] = readlane64(a[i], 15);
+}
 
 // 1/d from v_rcp_f64 refined by two Newton steps (pivots here are O(1) to
 // O(1e12): no denormal or overflow range is involved)
@@ -484,14 +500,17 @@ __device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, const
 // from its recipe and the scalar table K of the x point
 __device__ inline double row_eval(const WS& W, const double* K, const double* v, int c, int n, double S, double P,
                                   double Pi) {
+    // branch-free: the first row has no l1 and the last no u1, so clamped
+    // neighbours contribute exact zeros
     const int kd = W.kinds()[c];
+    const int cm = c > 0 ? c - 1 : 0, cp = c + 1 < n ? c + 1 : c;
     const double d = W.rcp_(RC_DC)[c] + W.rcp_(RC_DT)[c] * K[K_TP] + W.rcp_(RC_DI)[c] * K[K_IHETA] +
                      W.rcp_(RC_DR)[c] * K[K_TPR];
     double acc = d * v[c];
-    if (c >= 1) acc += (W.rcp_(RC_AL)[c] * K[kd & 255]) * v[c - 1];
-    if (c + 1 < n) acc += (W.rcp_(RC_AU)[c] * K[(kd >> 8) & 255]) * v[c + 1];
-    if (c == 1) acc += (3.0 * K[K_TPR]) * v[3];
-    if (c == 3) acc += (K[K_TP] * (1.0 / 3.0)) * v[1];
+    acc += (W.rcp_(RC_AL)[c] * K[kd & 255]) * v[cm];
+    acc += (W.rcp_(RC_AU)[c] * K[(kd >> 8) & 255]) * v[cp];
+    const double e1 = (3.0 * K[K_TPR]) * v[3], e3 = (K[K_TP] * (1.0 / 3.0)) * v[1];
+    acc += c == 1 ? e1 : (c == 3 ? e3 : 0.0);
     acc += (W.rcp_(RC_CS)[c] * K[(kd >> 16) & 255]) * S + W.rcp_(RC_CP)[c] * P + (W.rcp_(RC_CPI)[c] * K[K_TP]) * Pi;
     return K[kd >> 24] * acc;
 }
@@ -501,26 +520,44 @@ struct Fun {
     double S, P, Pi;
 };
 
-__device__ inline Fun fun_finish(const CosmoDev& C, const Geo& G, const XC& X, const double* v, double s2,
-                                 double s0) {
+// the fixed rows of a chain-ordered vector the functionals read
+struct FunRows {
+    double e, t2, n2, dl, db, t0, n0, p2, p0;
+};
+__device__ inline FunRows fun_rows(const Geo& G, const double* v) {
+    FunRows r;
+    r.e = v[G.cE]; r.t2 = v[4]; r.n2 = v[G.cC + 2]; r.dl = v[G.cD]; r.db = v[0]; r.t0 = v[2]; r.n0 = v[G.cC];
+    r.p2 = v[G.cB + 2]; r.p0 = v[G.cB];
+    return r;
+}
+
+__device__ inline Fun fun_finish(const CosmoDev& C, const XC& X, const FunRows& v, double s2, double s0) {
     Fun F;
-    F.S = -v[G.cE] - X.c12 * (C.Or * v[4] + C.Onu * v[G.cC + 2]) - X.fs2 * s2;
-    F.P = F.S - X.k23 * v[G.cE] + (X.fp0 * v[G.cD] + X.fp1 * v[0] + X.fp2 * v[2] + X.fp3 * v[G.cC] + X.fp4 * s0);
-    F.Pi = v[4] + v[G.cB + 2] + v[G.cB];
+    F.S = -v.e - X.c12 * (C.Or * v.t2 + C.Onu * v.n2) - X.fs2 * s2;
+    F.P = F.S - X.k23 * v.e + (X.fp0 * v.dl + X.fp1 * v.db + X.fp2 * v.t0 + X.fp3 * v.n0 + X.fp4 * s0);
+    F.Pi = v.t2 + v.p2 + v.p0;
     return F;
+}
+
+// the q-bin sums Σ sw_i v[ℳ₂,ᵢ] and Σ rw_i v[ℳ₀,ᵢ] of a vector, folded onto
+// lanes 0..15 (bins i, i+16, i+32 on lane i)
+__device__ inline void qsum_fold(const Geo& G, const QSet& Q, const double* v, int lane, double& s2, double& s0) {
+    s2 = 0.0; s0 = 0.0;
+    for (int i = lane; i < G.nq && lane < 16; i += 16) {
+        const int b = G.cM + i * (G.nm + 1);
+        s2 += Q.sw[i] * v[b + 2];
+        s0 += Q.rw[i] * v[b];
+    }
 }
 
 // every wavefront forms the sums over its own lanes (identical in all of them)
 __device__ inline Fun functionals(const CosmoDev& C, const Geo& G, const XC& X, const QSet& Q, const double* v,
                                   const Thr& T) {
-    double s2 = 0.0, s0 = 0.0;
-    const int lane = T.lane;
-    if (lane < G.nq) {
-        int b = G.cM + lane * (G.nm + 1);
-        s2 = Q.sw[lane] * v[b + 2];
-        s0 = Q.rw[lane] * v[b];
-    }
-    return fun_finish(C, G, X, v, wave_sum(s2), wave_sum(s0));
+    const FunRows r = fun_rows(G, v);
+    double a[2];
+    qsum_fold(G, Q, v, T.lane, a[0], a[1]);
+    row0_sums<2>(a);
+    return fun_finish(C, X, r, a[0], a[1]);
 }
 
 // du = f(v) on all rows (out = scale·du), optionally writing the border
@@ -530,38 +567,80 @@ __device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, con
                                const double* K, const double* v, double scale, double* out, bool make_c,
                                const Thr& T) {
     Fun F = functionals(C, G, X, Q, v, T);
-    for (int c = T.tid; c < G.n; c += T.nt) {
-        out[c] = scale * row_eval(W, K, v, c, G.n, F.S, F.P, F.Pi);
-        if (make_c) {
-            const int kd = W.kinds()[c];
-            const double m = K[kd >> 24];
-            W.cS()[c] = m * (W.rcp_(RC_CS)[c] * K[(kd >> 16) & 255]);
-            W.cP()[c] = m * W.rcp_(RC_CP)[c];
-            W.cPi()[c] = m * (W.rcp_(RC_CPI)[c] * K[K_TP]);
+    // two rows per thread, every load before any store, so the two rows'
+    // LDS latencies overlap (a row past n repeats row c0 and is not stored)
+    for (int c0 = T.tid; c0 < G.n; c0 += 2 * T.nt) {
+        const int c1 = c0 + T.nt;
+        const bool in1 = c1 < G.n;
+        const int cc[2] = {c0, in1 ? c1 : c0};
+        double r[2], cs[2], cp[2], cpi[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = cc[i];
+            r[i] = scale * row_eval(W, K, v, c, G.n, F.S, F.P, F.Pi);
+            if (make_c) {
+                const int kd = W.kinds()[c];
+                const double m = K[kd >> 24];
+                cs[i] = m * (W.rcp_(RC_CS)[c] * K[(kd >> 16) & 255]);
+                cp[i] = m * W.rcp_(RC_CP)[c];
+                cpi[i] = m * (W.rcp_(RC_CPI)[c] * K[K_TP]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (i == 1 && !in1) break;
+            const int c = cc[i];
+            out[c] = r[i];
+            if (make_c) { W.cS()[c] = cs[i]; W.cP()[c] = cp[i]; W.cPi()[c] = cpi[i]; }
         }
     }
     return F;
 }
 
 // W = I - hγL at the five stage points (their scalar tables stored): each
-// thread takes its rows through all five stages; caller syncs
+// thread takes two rows through all five stages, loads before stores; caller
+// syncs
 __device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const Thr& T) {
-    for (int c = T.tid; c < G.n; c += T.nt) {
-        const int kd = W.kinds()[c];
-        const int kl = kd & 255, ku = (kd >> 8) & 255, km = kd >> 24;
-        const double al = W.rcp_(RC_AL)[c], au = W.rcp_(RC_AU)[c], dc = W.rcp_(RC_DC)[c], dt = W.rcp_(RC_DT)[c],
-                     di = W.rcp_(RC_DI)[c], dr = W.rcp_(RC_DR)[c];
+    for (int c0 = T.tid; c0 < G.n; c0 += 2 * T.nt) {
+        const int c1 = c0 + T.nt;
+        const bool in1 = c1 < G.n;
+        const int cc[2] = {c0, in1 ? c1 : c0};
+        double L[2][5], D[2][5], U[2][5];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = cc[i];
+            const int kd = W.kinds()[c];
+            const int kl = kd & 255, ku = (kd >> 8) & 255, km = kd >> 24;
+            const double al = W.rcp_(RC_AL)[c], au = W.rcp_(RC_AU)[c], dc = W.rcp_(RC_DC)[c], dt = W.rcp_(RC_DT)[c],
+                         di = W.rcp_(RC_DI)[c], dr = W.rcp_(RC_DR)[c];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) {
+                const double* K = W.ktab(s);
+                const double mh = K[km] * hg;
+                L[i][s] = -mh * (al * K[kl]);
+                D[i][s] = 1.0 - mh * (dc + dt * K[K_TP] + di * K[K_IHETA] + dr * K[K_TPR]);
+                U[i][s] = -mh * (au * K[ku]);
+            }
+        }
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            const double* K = W.ktab(s);
-            const double mh = K[km] * hg;
             const Bands B = W.bands(s);
-            B.l1[c] = -mh * (al * K[kl]);
-            B.d[c] = 1.0 - mh * (dc + dt * K[K_TP] + di * K[K_IHETA] + dr * K[K_TPR]);
-            B.u1[c] = -mh * (au * K[ku]);
-            if (c < CB) {
-                B.l2c[c] = c == 3 ? -mh * (K[K_TP] * (1.0 / 3.0)) : 0.0;
-                B.u2c[c] = c == 1 ? -mh * (3.0 * K[K_TPR]) : 0.0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (i == 1 && !in1) break;
+                const int c = cc[i];
+                B.l1[c] = L[i][s]; B.d[c] = D[i][s]; B.u1[c] = U[i][s];
+            }
+        }
+        if (c0 < CB) {
+            const int km = W.kinds()[c0] >> 24;
+#pragma unroll
+            for (int s = 0; s < 5; ++s) {
+                const double* K = W.ktab(s);
+                const double mh = K[km] * hg;
+                const Bands B = W.bands(s);
+                B.l2c[c0] = c0 == 3 ? -mh * (K[K_TP] * (1.0 / 3.0)) : 0.0;
+                B.u2c[c0] = c0 == 1 ? -mh * (3.0 * K[K_TPR]) : 0.0;
             }
         }
     }
@@ -822,24 +901,18 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
 // Every wavefront forms the same K and y from its own lanes; nothing is written.
 __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
                                    double hg, const Thr& T, double* y) {
-    const int lane = T.lane;
+    const FunRows rw_ = fun_rows(G, W.w()), rs_ = fun_rows(G, W.cS()), rp_ = fun_rows(G, W.cP()),
+                  rq_ = fun_rows(G, W.cPi());
     double a[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = 0.0;
-    if (lane < G.nq) {
-        int b = G.cM + lane * (G.nm + 1);
-        double sw = Q.sw[lane], rw = Q.rw[lane];
-        a[0] = sw * W.w()[b + 2];   a[1] = rw * W.w()[b];
-        a[2] = sw * W.cS()[b + 2];  a[3] = rw * W.cS()[b];
-        a[4] = sw * W.cP()[b + 2];  a[5] = rw * W.cP()[b];
-        a[6] = sw * W.cPi()[b + 2]; a[7] = rw * W.cPi()[b];
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = wave_sum(a[i]);
-    Fun Fw = fun_finish(C, G, X, W.w(), a[0], a[1]);
-    Fun Fs = fun_finish(C, G, X, W.cS(), a[2], a[3]);
-    Fun Fp = fun_finish(C, G, X, W.cP(), a[4], a[5]);
-    Fun Fq = fun_finish(C, G, X, W.cPi(), a[6], a[7]);
+    qsum_fold(G, Q, W.w(), T.lane, a[0], a[1]);
+    qsum_fold(G, Q, W.cS(), T.lane, a[2], a[3]);
+    qsum_fold(G, Q, W.cP(), T.lane, a[4], a[5]);
+    qsum_fold(G, Q, W.cPi(), T.lane, a[6], a[7]);
+    row0_sums<8>(a);
+    Fun Fw = fun_finish(C, X, rw_, a[0], a[1]);
+    Fun Fs = fun_finish(C, X, rs_, a[2], a[3]);
+    Fun Fp = fun_finish(C, X, rp_, a[4], a[5]);
+    Fun Fq = fun_finish(C, X, rq_, a[6], a[7]);
     Cap K;
     K.m[0][0] = 1.0 - hg * Fs.S; K.m[0][1] = -hg * Fp.S; K.m[0][2] = -hg * Fq.S;
     K.m[1][0] = -hg * Fs.P; K.m[1][1] = 1.0 - hg * Fp.P; K.m[1][2] = -hg * Fq.P;
