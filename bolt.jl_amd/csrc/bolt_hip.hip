@@ -903,11 +903,16 @@ __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X,
                                    double hg, const Thr& T, double* y) {
     const FunRows rw_ = fun_rows(G, W.w()), rs_ = fun_rows(G, W.cS()), rp_ = fun_rows(G, W.cP()),
                   rq_ = fun_rows(G, W.cPi());
-    double a[8];
-    qsum_fold(G, Q, W.w(), T.lane, a[0], a[1]);
-    qsum_fold(G, Q, W.cS(), T.lane, a[2], a[3]);
-    qsum_fold(G, Q, W.cP(), T.lane, a[4], a[5]);
-    qsum_fold(G, Q, W.cPi(), T.lane, a[6], a[7]);
+    // the q-bin sums of all four vectors in one pass, folded onto lanes 0..15
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = T.lane; i < G.nq && T.lane < 16; i += 16) {
+        const int b = G.cM + i * (G.nm + 1);
+        const double sw = Q.sw[i], rw = Q.rw[i];
+        a[0] += sw * W.w()[b + 2];   a[1] += rw * W.w()[b];
+        a[2] += sw * W.cS()[b + 2];  a[3] += rw * W.cS()[b];
+        a[4] += sw * W.cP()[b + 2];  a[5] += rw * W.cP()[b];
+        a[6] += sw * W.cPi()[b + 2]; a[7] += rw * W.cPi()[b];
+    }
     row0_sums<8>(a);
     Fun Fw = fun_finish(C, X, rw_, a[0], a[1]);
     Fun Fs = fun_finish(C, X, rs_, a[2], a[3]);
@@ -1300,11 +1305,22 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             }
             if (en <= 1.0) {
                 double xn = last ? xend : x + h;
-                // FSAL at x_{n+1}
-                XC Xn = make_xc(C, k, xn);
-                qfill(C, G, Xn, W, T);
-                __syncthreads();
-                Fun Fn = rhs_rows(C, G, Xn, W, W.qcur(), W.kcur(), W.un(), 1.0, W.fn(), false, T);
+                // FSAL at x_{n+1}: the last stage point is x + 1·h, so its
+                // coefficients, q-set and scalar table serve unless the step
+                // was cut to end on xend
+                XC Xn;
+                QSet Qn = W.qst(4);
+                const double* Kn = W.ktab(4);
+                if (last) {
+                    Xn = make_xc(C, k, xn);
+                    qfill(C, G, Xn, W, T);
+                    __syncthreads();
+                    Qn = W.qcur();
+                    Kn = W.kcur();
+                } else {
+                    Xn = load_xc(W.xcs(4));
+                }
+                Fun Fn = rhs_rows(C, G, Xn, W, Qn, Kn, W.un(), 1.0, W.fn(), false, T);
                 ++nrhs;
                 __syncthreads();
                 if (Xn.rsa && T.tid == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
