@@ -314,8 +314,8 @@ struct WS {
     // scalar table of stage point s (s < 5) or of the current x (s = 5)
     __device__ double* ktab(int s) const { return b + 39 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
     __device__ double* kcur() const { return ktab(5); }
-    __device__ double* red() const { return ktab(6); }  // [8]
-    __device__ int* desc() const { return (int*)(red() + 8); }
+    __device__ double* red() const { return ktab(6); }  // [16]
+    __device__ int* desc() const { return (int*)(red() + 16); }
     __device__ int* kinds() const { return desc() + n; }
     __device__ int* seg() const { return desc() + 2 * n; }          // [MAXSEG][2]: c0, m
     __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg
@@ -327,7 +327,7 @@ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
     W.nq = nq;
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
-    return sizeof(double) * ((size_t)39 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 8) +
+    return sizeof(double) * ((size_t)39 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 16) +
            sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4);
 }
 
@@ -898,32 +898,25 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
 
 // After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
 // matrix K and y = K⁻¹ D w; the stage value is z = w + hγ (c_S y₀ + c_P y₁ + c_Π y₂).
-// Every wavefront forms the same K and y from its own lanes; nothing is written.
+// Wavefront r evaluates the functionals of vector r (w, c_S, c_P, c_Π) into
+// red[3r..3r+2]; after a barrier every wavefront forms the same K and y.
+// Needs four wavefronts; red is not written again before the next barrier.
 __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
                                    double hg, const Thr& T, double* y) {
-    const FunRows rw_ = fun_rows(G, W.w()), rs_ = fun_rows(G, W.cS()), rp_ = fun_rows(G, W.cP()),
-                  rq_ = fun_rows(G, W.cPi());
-    // the q-bin sums of all four vectors in one pass, folded onto lanes 0..15
-    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = T.lane; i < G.nq && T.lane < 16; i += 16) {
-        const int b = G.cM + i * (G.nm + 1);
-        const double sw = Q.sw[i], rw = Q.rw[i];
-        a[0] += sw * W.w()[b + 2];   a[1] += rw * W.w()[b];
-        a[2] += sw * W.cS()[b + 2];  a[3] += rw * W.cS()[b];
-        a[4] += sw * W.cP()[b + 2];  a[5] += rw * W.cP()[b];
-        a[6] += sw * W.cPi()[b + 2]; a[7] += rw * W.cPi()[b];
+    const int r = T.wv;
+    if (r < 4) {
+        const double* v = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
+        const Fun F = functionals(C, G, X, Q, v, T);
+        if (T.lane == 0) { W.red()[3 * r] = F.S; W.red()[3 * r + 1] = F.P; W.red()[3 * r + 2] = F.Pi; }
     }
-    row0_sums<8>(a);
-    Fun Fw = fun_finish(C, X, rw_, a[0], a[1]);
-    Fun Fs = fun_finish(C, X, rs_, a[2], a[3]);
-    Fun Fp = fun_finish(C, X, rp_, a[4], a[5]);
-    Fun Fq = fun_finish(C, X, rq_, a[6], a[7]);
+    __syncthreads();
+    const double* f = W.red();
     Cap K;
-    K.m[0][0] = 1.0 - hg * Fs.S; K.m[0][1] = -hg * Fp.S; K.m[0][2] = -hg * Fq.S;
-    K.m[1][0] = -hg * Fs.P; K.m[1][1] = 1.0 - hg * Fp.P; K.m[1][2] = -hg * Fq.P;
-    K.m[2][0] = -hg * Fs.Pi; K.m[2][1] = -hg * Fp.Pi; K.m[2][2] = 1.0 - hg * Fq.Pi;
-    double r[3] = {Fw.S, Fw.P, Fw.Pi};
-    solve3(K, r, y);
+    K.m[0][0] = 1.0 - hg * f[3]; K.m[0][1] = -hg * f[6]; K.m[0][2] = -hg * f[9];
+    K.m[1][0] = -hg * f[4]; K.m[1][1] = 1.0 - hg * f[7]; K.m[1][2] = -hg * f[10];
+    K.m[2][0] = -hg * f[5]; K.m[2][1] = -hg * f[8]; K.m[2][2] = 1.0 - hg * f[11];
+    double rr[3] = {f[0], f[1], f[2]};
+    solve3(K, rr, y);
     return K;
 }
 
