@@ -265,6 +265,7 @@ struct Bands {
 };
 
 #define CB 8      // rows per block of the banded sweeps
+#define SR 6      // rows per lane of the substitution scans (64·SR rows per pass)
 #define NXC 24    // doubles per stored XC
 
 // LDS workspace of one k-mode.  Vectors of n doubles: u, f, u_{n+1}, f_{n+1}, ψ,
@@ -315,6 +316,7 @@ struct WS {
     __device__ double* ktab(int s) const { return b + 39 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
     __device__ double* kcur() const { return ktab(5); }
     __device__ double* red() const { return ktab(6); }  // [16]
+    __device__ double* dump() const { return red() + 15; }  // store slot nobody reads
     __device__ int* desc() const { return (int*)(red() + 16); }
     __device__ int* kinds() const { return desc() + n; }
     __device__ int* seg() const { return desc() + 2 * n; }          // [MAXSEG][2]: c0, m
@@ -328,7 +330,7 @@ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
     return sizeof(double) * ((size_t)39 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 16) +
-           sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4);
+           sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4) + sizeof(double) * WAVE * SR;  // scan read padding
 }
 
 __device__ inline void store_xc(double* p, const XC& X) {
@@ -586,9 +588,9 @@ __device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, con
                 cpi[i] = m * (W.rcp_(RC_CPI)[c] * K[K_TP]);
             }
         }
+        // past n, cc[1] = c0: the repeat is stored first, row c0's own value last
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            if (i == 1 && !in1) break;
+        for (int i = 1; i >= 0; --i) {
             const int c = cc[i];
             out[c] = r[i];
             if (make_c) { W.cS()[c] = cs[i]; W.cP()[c] = cp[i]; W.cPi()[c] = cpi[i]; }
@@ -626,8 +628,7 @@ __device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const T
         for (int s = 0; s < 5; ++s) {
             const Bands B = W.bands(s);
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                if (i == 1 && !in1) break;
+            for (int i = 1; i >= 0; --i) {  // past n, the repeat of row c0 first, its own values last
                 const int c = cc[i];
                 B.l1[c] = L[i][s]; B.d[c] = D[i][s]; B.u1[c] = U[i][s];
             }
@@ -777,7 +778,6 @@ __device__ inline void affine_scan(double& a, double& b) {
     affine_step<0x143, 0xC>(a, b);
 }
 
-constexpr int SR = 6;  // rows per lane: 384 rows per pass
 static_assert(SR >= 4, "rows 0..3 must sit in one lane");
 
 // the lane maps of one pass: al/be hold f_i ∘ ... ∘ f_0 of the lane's rows;
@@ -794,8 +794,11 @@ __device__ inline double pass_scan(double (&al)[SR], double (&be)[SR], double& c
 }
 
 // forward + back substitution of one right-hand side over all n rows by one
-// wavefront (all 64 lanes, uniform control flow)
-__device__ inline void wave_subst(const Bands& Bd, int n, double* bv, int lane) {
+// wavefront (all 64 lanes, uniform control flow).  Branch-free: loads past n
+// stay inside the workspace (it ends with one pass of padding) and are masked
+// to the identity, and stores from lanes whose rows lie past n go to the
+// never-read slot `dump`.
+__device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* dump, int lane) {
     constexpr int span = WAVE * SR;
     double carry = 0.0;
     for (int p0 = 0; p0 < n; p0 += span) {
@@ -804,15 +807,19 @@ __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, int lane) 
 #pragma unroll
         for (int i = 0; i < SR; ++i) {
             const bool in = j0 + i < n;
-            al[i] = in ? -Bd.l1[j0 + i] : 0.0;
-            be[i] = in ? bv[j0 + i] : 0.0;
+            const double l = Bd.l1[j0 + i], b = bv[j0 + i];
+            al[i] = in ? -l : 0.0;
+            be[i] = in ? b : 0.0;
         }
         // row 3 (Θ₁) also takes -l2·y_1 with y_1 = b_1 - l1_1·b_0
-        if (p0 == 0 && lane == 0) be[3] -= Bd.l2c[3] * fma(al[1], be[0], be[1]);
+        const double corr = Bd.l2c[3] * fma(al[1], be[0], be[1]);
+        be[3] -= (p0 == 0 && lane == 0) ? corr : 0.0;
         const double tin = pass_scan(al, be, carry);
 #pragma unroll
-        for (int i = 0; i < SR; ++i)
-            if (j0 + i < n) bv[j0 + i] = fma(al[i], tin, be[i]);
+        for (int i = 0; i < SR; ++i) {
+            double* dst = j0 + i < n ? bv + j0 + i : dump;
+            *dst = fma(al[i], tin, be[i]);
+        }
     }
     // backward: pass p covers rows p·span .. p·span + span - 1, lane order
     // running down the rows
@@ -823,24 +830,28 @@ __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, int lane) 
 #pragma unroll
         for (int i = 0; i < SR; ++i) {
             const bool in = jt - i < n;
-            const double dj = in ? Bd.d[jt - i] : 0.0;
-            al[i] = in ? -dj * Bd.u1[jt - i] : 0.0;
-            be[i] = in ? dj * bv[jt - i] : 0.0;
+            const double dj = Bd.d[jt - i], uj = Bd.u1[jt - i], yj = bv[jt - i];
+            al[i] = in ? -dj * uj : 0.0;
+            be[i] = in ? dj * yj : 0.0;
         }
-        const bool low = p == 0 && lane == WAVE - 1;  // rows SR-1 .. 0
-        const double y0 = low ? bv[0] : 0.0, y1 = low ? bv[1] : 0.0;
+        // rows 1 and 0 (the last lane of pass 0): row 1 (v_b) also takes
+        // -u2·x_3, row 0 then follows from row 1
+        const bool low = p == 0 && lane == WAVE - 1;
+        const double y0 = bv[0], y1 = bv[1], u2 = Bd.u2c[1], u11 = Bd.u1[1], d1 = Bd.d[1], u10 = Bd.u1[0],
+                     d0 = Bd.d[0];
         const double tin = pass_scan(al, be, carry);
         double x[SR];
 #pragma unroll
         for (int i = 0; i < SR; ++i) x[i] = fma(al[i], tin, be[i]);
-        if (low) {
-            // row 1 (v_b) also takes -u2·x_3; row 0 then follows from row 1
-            x[SR - 2] = ((y1 - Bd.u2c[1] * x[SR - 4]) - Bd.u1[1] * x[SR - 3]) * Bd.d[1];
-            x[SR - 1] = (y0 - Bd.u1[0] * x[SR - 2]) * Bd.d[0];
-        }
+        const double x1 = ((y1 - u2 * x[SR - 4]) - u11 * x[SR - 3]) * d1;
+        const double x0 = (y0 - u10 * x1) * d0;
+        x[SR - 2] = low ? x1 : x[SR - 2];
+        x[SR - 1] = low ? x0 : x[SR - 1];
 #pragma unroll
-        for (int i = 0; i < SR; ++i)
-            if (jt - i < n) bv[jt - i] = x[i];
+        for (int i = 0; i < SR; ++i) {
+            double* dst = jt - i < n ? bv + jt - i : dump;
+            *dst = x[i];
+        }
     }
 }
 
@@ -850,14 +861,14 @@ __device__ inline void chain_solve4(const Geo& G, const WS& W, int st, const Thr
     const Bands Bd = W.bands(st);
     for (int r = T.wv; r < 4; r += T.nt / WAVE) {
         double* bv = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
-        wave_subst(Bd, G.n, bv, T.lane);
+        wave_subst(Bd, G.n, bv, W.dump(), T.lane);
     }
 }
 
 // one right-hand side with the stored factors, by the first wavefront
 __device__ inline void chain_solve1(const Geo& G, const WS& W, int st, double* bvec, const Thr& T) {
     if (T.wv != 0) return;
-    wave_subst(W.bands(st), G.n, bvec, T.lane);
+    wave_subst(W.bands(st), G.n, bvec, W.dump(), T.lane);
 }
 
 // 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
