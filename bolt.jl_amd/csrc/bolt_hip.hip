@@ -1294,9 +1294,24 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 }
                 __syncthreads();
             }
-            // Shampine's smoothed estimate: W⁻¹ e with the last stage's W
-            chain_solve1(G, W, 4, W.arg(), T);
+            // Shampine's smoothed estimate: W⁻¹ e with the last stage's W, on
+            // wavefront 0.  Meanwhile the other wavefronts evaluate the FSAL
+            // derivative f(u_{n+1}) at x + h, which is the last stage point
+            // (abscissa 1): its coefficients, q-set and scalar table serve.
+            // It is used only if the step is accepted; a step cut to end on
+            // xend evaluates it at xend after acceptance instead.
+            const bool spec = !last;
+            Fun Fn;
+            if (T.wv == 0) {
+                chain_solve1(G, W, 4, W.arg(), T);
+            } else if (spec) {
+                Thr Tf = T;
+                Tf.tid -= WAVE;
+                Tf.nt -= WAVE;
+                Fn = rhs_rows(C, G, load_xc(W.xcs(4)), W, W.qst(4), W.ktab(4), W.un(), 1.0, W.fn(), false, Tf);
+            }
             ++nsol;
+            if (spec) ++nrhs;
             __syncthreads();
             border_solve(C, G, Xs, W, W.qst(4), K, hg, W.arg(), T);
             double en = wrms(G, W, W.arg(), W.u(), W.un(), rt, at, T);
@@ -1309,25 +1324,20 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             }
             if (en <= 1.0) {
                 double xn = last ? xend : x + h;
-                // FSAL at x_{n+1}: the last stage point is x + 1·h, so its
-                // coefficients, q-set and scalar table serve unless the step
-                // was cut to end on xend
+                // FSAL at x_{n+1} (already evaluated unless the step ends on xend)
                 XC Xn;
-                QSet Qn = W.qst(4);
-                const double* Kn = W.ktab(4);
-                if (last) {
+                if (spec) {
+                    Xn = load_xc(W.xcs(4));
+                } else {
                     Xn = make_xc(C, k, xn);
                     qfill(C, G, Xn, W, T);
                     __syncthreads();
-                    Qn = W.qcur();
-                    Kn = W.kcur();
-                } else {
-                    Xn = load_xc(W.xcs(4));
+                    Fn = rhs_rows(C, G, Xn, W, W.qcur(), W.kcur(), W.un(), 1.0, W.fn(), false, T);
+                    ++nrhs;
+                    __syncthreads();
                 }
-                Fun Fn = rhs_rows(C, G, Xn, W, Qn, Kn, W.un(), 1.0, W.fn(), false, T);
-                ++nrhs;
-                __syncthreads();
-                if (Xn.rsa && T.tid == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
+                // Fn is held by wavefronts 1.. in both cases
+                if (Xn.rsa && T.wv == 1 && T.lane == 0) rsa_mutate(G, Xn, k, W.un(), Fn.P);
                 __syncthreads();
                 ++nacc;
                 while (next < nout) {
