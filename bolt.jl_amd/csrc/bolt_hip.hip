@@ -189,14 +189,16 @@ __device__ inline void row0_sums(double (&a)[N]) {
     for (int i = 0; i < N; ++i) a[i] = readlane64(a[i], 15);
 }
 
-// 1/d from v_rcp_f64 refined by two Newton steps (pivots here are O(1) to
-// O(1e12): no denormal or overflow range is involved)
+// 1/d from v_rcp_f64 refined twice (pivots here are O(1) to O(1e12): no
+// denormal or overflow range is involved).  With e = 1 - d·r₀,
+// r₁ = r₀(1 + e) has error e² and r₂ = r₁(1 + e²) error e⁴; e² is formed
+// beside r₁, so the dependent chain is rcp, fma, fma, fma.
 __device__ inline double rcp(double d) {
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    return fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = fma(-d, r, 1.0);
+    const double r1 = fma(r, e, r);
+    const double e2 = e * e;
+    return fma(r1, e2, r1);
 }
 
 // largest hierarchy cut-off accepted (row descriptors pack ℓ in 11 bits)
@@ -674,24 +676,35 @@ __device__ inline void rsa_mutate(const Geo& G, const XC& X, double k, double* v
 // one tridiagonal factor block of rows jb..jb+CB-1; GUARD: store only rows < m
 template <bool GUARD>
 __device__ inline void factor_tri_block(double* l1, double* d, const double* u1, int jb, int m, double& ip1,
-                                        double& u1p1) {
+                                        double& u1p1, double* dump) {
     double L1[CB], D[CB], U1[CB];
 #pragma unroll
     for (int i = 0; i < CB; ++i) { L1[i] = l1[jb + i]; D[i] = d[jb + i]; U1[i] = u1[jb + i]; }
+    // the pivot chain is inv_i = 1/(D_i - (L1_i·U1_{i-1})·inv_{i-1}): the
+    // products and the multipliers L1_i·inv_{i-1} stay off it
+    double P[CB];
 #pragma unroll
     for (int i = 0; i < CB; ++i) {
-        double m1 = L1[i] * ip1;
-        double inv = rcp(D[i] - m1 * u1p1);
-        L1[i] = m1; D[i] = inv;
-        ip1 = inv;
+        P[i] = L1[i] * u1p1;
         u1p1 = U1[i];
     }
 #pragma unroll
-    for (int i = 0; i < CB; ++i)
-        if (!GUARD || jb + i < m) { l1[jb + i] = L1[i]; d[jb + i] = D[i]; }
+    for (int i = 0; i < CB; ++i) {
+        const double m1 = L1[i] * ip1;
+        const double inv = rcp(fma(-P[i], ip1, D[i]));
+        L1[i] = m1; D[i] = inv;
+        ip1 = inv;
+    }
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+        const bool in = !GUARD || jb + i < m;
+        *(in ? l1 + jb + i : dump) = L1[i];
+        *(in ? d + jb + i : dump) = D[i];
+    }
 }
 
-__device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m) {
+// (stores past the segment's end go to the never-read slot dump)
+__device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m, double* dump) {
     double* l1 = Bd.l1 + c0; double* d = Bd.d + c0; double* u1 = Bd.u1 + c0;
     const bool first = c0 == 0;  // the only segment with a second band
     double ip1 = 0.0, ip2 = 0.0, u1p1 = 0.0, u1p2 = 0.0, u2p1 = 0.0, u2p2 = 0.0;
@@ -699,7 +712,8 @@ __device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m) {
         double L2[CB], L1[CB], D[CB], U1[CB], U2[CB];
 #pragma unroll
         for (int i = 0; i < CB; ++i) {
-            L2[i] = first ? Bd.l2c[i] : 0.0; U2[i] = first ? Bd.u2c[i] : 0.0;
+            const double l2 = Bd.l2c[i], u2 = Bd.u2c[i];
+            L2[i] = first ? l2 : 0.0; U2[i] = first ? u2 : 0.0;
             L1[i] = l1[i]; D[i] = d[i]; U1[i] = u1[i];
         }
 #pragma unroll
@@ -715,25 +729,28 @@ __device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m) {
             u2p2 = u2p1; u2p1 = U2[i];
         }
 #pragma unroll
-        for (int i = 0; i < CB; ++i)
-            if (i < m) {
-                if (first) Bd.l2c[i] = L2[i];
-                l1[i] = L1[i]; d[i] = D[i]; u1[i] = U1[i];
-            }
+        for (int i = 0; i < CB; ++i) {
+            const bool in = i < m;
+            *(first && in ? Bd.l2c + i : dump) = L2[i];
+            *(in ? l1 + i : dump) = L1[i];
+            *(in ? d + i : dump) = D[i];
+            *(in ? u1 + i : dump) = U1[i];
+        }
     }
     // tridiagonal blocks: full ones unguarded (lanes with shorter segments
     // leave the loop early), then the partial tail
     int jb = CB;
-    for (; jb + CB <= m; jb += CB) factor_tri_block<false>(l1, d, u1, jb, m, ip1, u1p1);
-    if (jb < m) factor_tri_block<true>(l1, d, u1, jb, m, ip1, u1p1);
+    for (; jb + CB <= m; jb += CB) factor_tri_block<false>(l1, d, u1, jb, m, ip1, u1p1, dump);
+    if (jb < m) factor_tri_block<true>(l1, d, u1, jb, m, ip1, u1p1, dump);
 }
 
-// the LU of all five stage points, one (stage, segment) unit per thread
-__device__ inline void chain_factor_all(const Geo& G, const WS& W, const Thr& T) {
+// the LU of all five stage points, one (stage, segment) unit per lane of
+// one wavefront
+__device__ inline void chain_factor_all(const Geo& G, const WS& W, int lane) {
     const int nseg = W.misc()[0];
-    for (int u = T.tid; u < 5 * nseg; u += T.nt) {
+    for (int u = lane; u < 5 * nseg; u += WAVE) {
         const int st = u / nseg, sg = u - st * nseg;
-        chain_factor_seg(W.bands(st), W.seg()[2 * sg], W.seg()[2 * sg + 1]);
+        chain_factor_seg(W.bands(st), W.seg()[2 * sg], W.seg()[2 * sg + 1], W.dump());
     }
 }
 
@@ -1256,17 +1273,30 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             __syncthreads();
             stage_bands(G, W, hg, T);
             __syncthreads();
-            chain_factor_all(G, W, T);
+            // the LU of the five stage points on wavefront 0 while the others
+            // form stage 1's right-hand side and border columns (which need
+            // no factor)
+            if (T.wv == 0) {
+                chain_factor_all(G, W, T.lane);
+            } else {
+                Thr Tr = T;
+                Tr.tid -= WAVE;
+                Tr.nt -= WAVE;
+                rhs_rows(C, G, load_xc(W.xcs(0)), W, W.qst(0), W.ktab(0), W.arg(), h, W.w(), true, Tr);
+            }
+            ++nrhs;
             __syncthreads();
             Cap K;
             XC Xs;
             for (int s = 1; s < 6; ++s) {
                 Xs = load_xc(W.xcs(s - 1));
                 const QSet Qs = W.qst(s - 1);
-                // rhs = h f(ψ) and the border columns
-                rhs_rows(C, G, Xs, W, Qs, W.ktab(s - 1), W.arg(), h, W.w(), true, T);
-                ++nrhs;
-                __syncthreads();
+                // rhs = h f(ψ) and the border columns (stage 1's are formed above)
+                if (s > 1) {
+                    rhs_rows(C, G, Xs, W, Qs, W.ktab(s - 1), W.arg(), h, W.w(), true, T);
+                    ++nrhs;
+                    __syncthreads();
+                }
                 chain_solve4(G, W, s - 1, T);
                 ++nsol;
                 __syncthreads();
