@@ -926,9 +926,10 @@ __device__ inline void solve3(const Cap& K, const double* r, double* y) {
 
 // After the chain solves (w = W_L⁻¹ r, c_* = W_L⁻¹ c_*): build the capacitance
 // matrix K and y = K⁻¹ D w; the stage value is z = w + hγ (c_S y₀ + c_P y₁ + c_Π y₂).
-// Wavefront r evaluates the functionals of vector r (w, c_S, c_P, c_Π) into
-// red[3r..3r+2]; after a barrier every wavefront forms the same K and y.
-// Needs four wavefronts; red is not written again before the next barrier.
+// Wavefront r evaluates the functionals of vector r (w, c_S, c_P, c_Π), which
+// it has itself solved, into red[3r..3r+2]; after a barrier every wavefront
+// forms the same K and y.  Needs four wavefronts; red is not written again
+// before the next barrier.
 __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
                                    double hg, const Thr& T, double* y) {
     const int r = T.wv;
@@ -1299,7 +1300,9 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 }
                 chain_solve4(G, W, s - 1, T);
                 ++nsol;
-                __syncthreads();
+                // no barrier: wavefront r's border functionals read only the
+                // vector it has just solved (a wavefront's LDS accesses stay in
+                // program order)
                 double y[3];
                 K = stage_border(C, G, Xs, W, Qs, hg, T, y);
                 // one pass: Z_s, then the next stage's argument, or after the last
