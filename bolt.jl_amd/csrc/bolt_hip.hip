@@ -1070,17 +1070,68 @@ __device__ void initial_conditions(const CosmoDev& C, const Geo& G, const WS& W,
 
 // ------------------------------------------------------------- KenCarp4 data
 __constant__ double KC_C[6] = {0.0, 0.5, 83.0 / 250.0, 31.0 / 50.0, 17.0 / 20.0, 1.0};
-__constant__ double KC_A[6][6] = {
-    {0, 0, 0, 0, 0, 0},
-    {0.25, 0, 0, 0, 0, 0},
-    {8611.0 / 62500.0, -1743.0 / 31250.0, 0, 0, 0, 0},
-    {5012029.0 / 34652500.0, -654441.0 / 2922500.0, 174375.0 / 388108.0, 0, 0, 0},
-    {15267082809.0 / 155376265600.0, -71443401.0 / 120774400.0, 730878875.0 / 902184768.0,
-     2285395.0 / 8070912.0, 0, 0},
-    {82889.0 / 524892.0, 0.0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0}};
-__constant__ double KC_BH[6] = {4586570599.0 / 29645900160.0, 0.0, 178811875.0 / 945068544.0,
-                                814220225.0 / 1159782912.0, -3700637.0 / 11593932.0, 61727.0 / 225920.0};
 #define KC_G 0.25
+// KenCarp4's A (rows = stages) and embedded weights b̂ (compile-time indices)
+struct KCT {
+    static constexpr double A[6][6] = {
+        {0, 0, 0, 0, 0, 0},
+        {0.25, 0, 0, 0, 0, 0},
+        {8611.0 / 62500.0, -1743.0 / 31250.0, 0, 0, 0, 0},
+        {5012029.0 / 34652500.0, -654441.0 / 2922500.0, 174375.0 / 388108.0, 0, 0, 0},
+        {15267082809.0 / 155376265600.0, -71443401.0 / 120774400.0, 730878875.0 / 902184768.0,
+         2285395.0 / 8070912.0, 0, 0},
+        {82889.0 / 524892.0, 0.0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0}};
+    static constexpr double BH[6] = {4586570599.0 / 29645900160.0, 0.0, 178811875.0 / 945068544.0,
+                                     814220225.0 / 1159782912.0, -3700637.0 / 11593932.0, 61727.0 / 225920.0};
+};
+
+// One pass after stage S's border solve: Z_S = w + hγ(c_S y₀ + c_P y₁ + c_Π y₂),
+// then the next stage's argument u + Σ_j a_{S+1,j} Z_j, or after the last stage
+// u_{n+1} (stiffly accurate) and the embedded error (into ψ, which no
+// wavefront reads any more; w still is).  The tableau is compile-time; two
+// rows per thread, every load before any store (past n, cc[1] repeats row c0
+// and is stored first).  Caller syncs.
+template <int S>
+__device__ inline void stage_combine(const Geo& G, const WS& W, double hg, const double* y, const Thr& T) {
+    for (int c0 = T.tid; c0 < G.n; c0 += 2 * T.nt) {
+        const int c1 = c0 + T.nt;
+        const int cc[2] = {c0, c1 < G.n ? c1 : c0};
+        double z[2], a[2], e[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = cc[i];
+            z[i] = W.w()[c] + hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
+            double acc = W.u()[c], err = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const double zj = W.Z(j)[c];
+                if (S < 5) {
+                    acc += KCT::A[S + 1][j] * zj;
+                } else {
+                    acc += KCT::A[5][j] * zj;
+                    err += (KCT::A[5][j] - KCT::BH[j]) * zj;
+                }
+            }
+            if (S < 5) {
+                a[i] = acc + KCT::A[S + 1][S] * z[i];
+            } else {
+                a[i] = acc + KC_G * z[i];
+                e[i] = err + (KC_G - KCT::BH[5]) * z[i];
+            }
+        }
+#pragma unroll
+        for (int i = 1; i >= 0; --i) {
+            const int c = cc[i];
+            W.Z(S)[c] = z[i];
+            if (S < 5) {
+                W.arg()[c] = a[i];
+            } else {
+                W.un()[c] = a[i];
+                W.arg()[c] = e[i];
+            }
+        }
+    }
+}
 
 // ---------------------------------------------------------------- kernels
 struct SolveArgs {
@@ -1256,7 +1307,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             for (int c = T.tid; c < G.n; c += T.nt) {
                 double z0 = h * W.f()[c];
                 W.Z(0)[c] = z0;
-                W.arg()[c] = W.u()[c] + KC_A[1][0] * z0;
+                W.arg()[c] = W.u()[c] + KCT::A[1][0] * z0;
             }
             // the five implicit stage points depend on x and h only: their
             // coefficients, q-quantities, W bands and LU come first, all stages
@@ -1305,25 +1356,12 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 // program order)
                 double y[3];
                 K = stage_border(C, G, Xs, W, Qs, hg, T, y);
-                // one pass: Z_s, then the next stage's argument, or after the last
-                // stage u_{n+1} (stiffly accurate) and the embedded error (into ψ,
-                // which no wavefront reads any more; w still is)
-                for (int c = T.tid; c < G.n; c += T.nt) {
-                    const double z = W.w()[c] + hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
-                    W.Z(s)[c] = z;
-                    if (s < 5) {
-                        double acc = W.u()[c];
-                        for (int j = 0; j < s; ++j) acc += KC_A[s + 1][j] * W.Z(j)[c];
-                        W.arg()[c] = acc + KC_A[s + 1][s] * z;
-                    } else {
-                        double acc = W.u()[c], e = 0.0;
-                        for (int j = 0; j < 5; ++j) {
-                            acc += KC_A[5][j] * W.Z(j)[c];
-                            e += (KC_A[5][j] - KC_BH[j]) * W.Z(j)[c];
-                        }
-                        W.un()[c] = acc + KC_G * z;
-                        W.arg()[c] = e + (KC_G - KC_BH[5]) * z;
-                    }
+                switch (s) {
+                    case 1: stage_combine<1>(G, W, hg, y, T); break;
+                    case 2: stage_combine<2>(G, W, hg, y, T); break;
+                    case 3: stage_combine<3>(G, W, hg, y, T); break;
+                    case 4: stage_combine<4>(G, W, hg, y, T); break;
+                    default: stage_combine<5>(G, W, hg, y, T); break;
                 }
                 __syncthreads();
             }
