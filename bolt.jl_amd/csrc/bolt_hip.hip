@@ -949,17 +949,6 @@ __device__ inline Cap stage_border(const CosmoDev& C, const Geo& G, const XC& X,
     return K;
 }
 
-// b = (I - hγA)⁻¹ b with stored chain factors, Y columns and capacitance K
-__device__ inline void border_solve(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, const QSet& Q,
-                                          const Cap& K, double hg, double* b, const Thr& T) {
-    Fun F = functionals(C, G, X, Q, b, T);
-    double r[3] = {F.S, F.P, F.Pi}, y[3];
-    solve3(K, r, y);
-    __syncthreads();
-    for (int c = T.tid; c < G.n; c += T.nt)
-        b[c] += hg * (W.cS()[c] * y[0] + W.cP()[c] * y[1] + W.cPi()[c] * y[2]);
-}
-
 // source_function (perturbations.jl:343-383) from the chain-ordered (mutated)
 // u, the derivative rows du of Θ₁..₃, Θᵖ₀..₃, 𝒩₂, v_b, ℳ₂,ᵢ, and Φ′
 __device__ double source_fn(const CosmoDev& C, const Geo& G, const XC& X, const WS& W, double k, const double* u,
@@ -1298,17 +1287,28 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
         }
         const int maxsteps = cf.max_steps > 0 ? cf.max_steps : 100000;
         long long nsteps = 0;
+        bool take = false;  // the last attempt was accepted
         while (x < xend) {
             if (++nsteps > maxsteps) { status = BOLT_MODE_MAXSTEPS; break; }
             bool last = false;
             if (h >= xend - x) { h = xend - x; last = true; }
             const double hg = h * KC_G;
-            // Z₀ = h f(u) and the first stage's argument
+            // Z₀ = h f(u) and the first stage's argument; after an accepted
+            // step this pass also moves u_{n+1}, f_{n+1} into u, f (each thread
+            // copies and reads only its own rows)
             for (int c = T.tid; c < G.n; c += T.nt) {
-                double z0 = h * W.f()[c];
+                double uc, fc;
+                if (take) {
+                    uc = W.un()[c]; fc = W.fn()[c];
+                    W.u()[c] = uc; W.f()[c] = fc;
+                } else {
+                    uc = W.u()[c]; fc = W.f()[c];
+                }
+                const double z0 = h * fc;
                 W.Z(0)[c] = z0;
-                W.arg()[c] = W.u()[c] + KCT::A[1][0] * z0;
+                W.arg()[c] = uc + KCT::A[1][0] * z0;
             }
+            take = false;
             // the five implicit stage points depend on x and h only: their
             // coefficients, q-quantities, W bands and LU come first, all stages
             // at once
@@ -1384,9 +1384,23 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             ++nsol;
             if (spec) ++nrhs;
             __syncthreads();
-            border_solve(C, G, Xs, W, W.qst(4), K, hg, W.arg(), T);
-            double en = wrms(G, W, W.arg(), W.u(), W.un(), rt, at, T);
-            __syncthreads();
+            // the border step of the error solve folded into its norm:
+            // e_c = ψ_c + hγ(c_S y₀ + c_P y₁ + c_Π y₂) is formed and weighted
+            // row by row, never stored
+            double en;
+            {
+                const Fun Fe = functionals(C, G, Xs, W.qst(4), W.arg(), T);
+                double r[3] = {Fe.S, Fe.P, Fe.Pi}, ye[3];
+                solve3(K, r, ye);
+                double ss = 0.0;
+                for (int c = T.tid; c < G.n; c += T.nt) {
+                    const double e = W.arg()[c] + hg * (W.cS()[c] * ye[0] + W.cP()[c] * ye[1] + W.cPi()[c] * ye[2]);
+                    const double sc = at + rt * fmax(fabs(W.u()[c]), fabs(W.un()[c]));
+                    const double q = e / sc;
+                    ss += q * q;
+                }
+                en = sqrt(block_sum(ss, W, T) / G.n);
+            }
             if (!isfinite(en)) {
                 ++nrej;
                 h *= 0.2;
@@ -1417,8 +1431,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                     emit(C, G, W, A, md.out, k, x, W.u(), W.f(), xn, W.un(), W.fn(), xv, next, T);
                     ++next;
                 }
-                for (int c = T.tid; c < G.n; c += T.nt) { W.u()[c] = W.un()[c]; W.f()[c] = W.fn()[c]; }
-                __syncthreads();
+                take = true;  // u, f <- u_{n+1}, f_{n+1} in the next step's first pass
                 x = xn;
                 double fac = (en == 0.0) ? 10.0 : 0.9 * pow(en, -0.25);
                 fac = fmin(10.0, fmax(0.2, fac));
