@@ -60,7 +60,7 @@ __device__ inline Thr thr_self() {
 struct CosmoDev {
     double h, Or, Ob, Oc, A, ns, Yp, Nnu, mnu;
     double H0, eta0, rho_crit, Onu;  // Onu = 7*(2/3)*N_ν/8*(4/11)^(4/3)*Ω_r
-    double x0, x1, dx;
+    double x0, x1, dx, idx;  // idx = 1/dx
     int nx, nq;
     const double* coef;    // [16][nx+2]
     const double* qv;      // [nq] q_i
@@ -133,18 +133,19 @@ __device__ inline double node_x(const CosmoDev& C, int j) { return (C.x0 * (C.nx
 
 // uniform cubic B-spline (src/util.jl:11-13)
 __device__ inline double spl(const CosmoDev& C, int tab, double x, int deriv = 0) {
-    double t = (x - C.x0) / C.dx;
+    double t = (x - C.x0) * C.idx;
     int i = (int)floor(t);
     i = i < 0 ? 0 : (i > C.nx - 2 ? C.nx - 2 : i);
     double s = t - i, r = 1.0 - s;
     const double* q = C.coef + (size_t)tab * (C.nx + 2) + i;
+    constexpr double sixth = 1.0 / 6.0, two3 = 2.0 / 3.0;
     if (deriv == 0)
-        return r * r * r / 6.0 * q[0] + (2.0 / 3.0 - s * s + 0.5 * s * s * s) * q[1] +
-               (2.0 / 3.0 - r * r + 0.5 * r * r * r) * q[2] + s * s * s / 6.0 * q[3];
+        return (r * r * r) * sixth * q[0] + (two3 - s * s + 0.5 * s * s * s) * q[1] +
+               (two3 - r * r + 0.5 * r * r * r) * q[2] + (s * s * s) * sixth * q[3];
     if (deriv == 1)
         return (-0.5 * r * r * q[0] + (-2.0 * s + 1.5 * s * s) * q[1] + (2.0 * r - 1.5 * r * r) * q[2] +
-                0.5 * s * s * q[3]) / C.dx;
-    return (r * q[0] + (-2.0 + 3.0 * s) * q[1] + (-2.0 + 3.0 * r) * q[2] + s * q[3]) / (C.dx * C.dx);
+                0.5 * s * s * q[3]) * C.idx;
+    return (r * q[0] + (-2.0 + 3.0 * s) * q[1] + (-2.0 + 3.0 * r) * q[2] + s * q[3]) * (C.idx * C.idx);
 }
 
 // ------------------------------------------------------------ reductions
@@ -223,14 +224,15 @@ __device__ inline XC make_xc(const CosmoDev& C, double k, double x) {
     X.tp = spl(C, BOLT_T_TAU_P, x);
     X.tpp = spl(C, BOLT_T_TAU_PP, x);
     X.csb2 = spl(C, BOLT_T_CSB2, x);
-    const double ia = 1.0 / X.a, ia2 = ia * ia, iHc = 1.0 / X.Hc, iHc2 = iHc * iHc, ik2 = 1.0 / (k * k);
+    // a, ℋ, η are O(1e-9)..O(1e4): the refined reciprocal is within an ulp or two
+    const double ia = rcp(X.a), ia2 = ia * ia, iHc = rcp(X.Hc), iHc2 = iHc * iHc, ik2 = 1.0 / (k * k);
     const double H02 = C.H0 * C.H0, irho = 1.0 / C.rho_crit;
     X.kH = k * iHc;
     X.R = (4.0 * C.Or / (3.0 * C.Ob)) * ia;
     X.c12 = (12.0 * H02) * ik2 * ia2;
     X.gg = (0.5 * H02) * iHc2;
     X.k23 = (k * k / 3.0) * iHc2;
-    X.iHeta = iHc / X.eta;
+    X.iHeta = iHc * rcp(X.eta);
     X.fs2 = X.c12 * (0.25 * irho);
     X.fp0 = X.gg * C.Oc * ia;
     X.fp1 = X.gg * C.Ob * ia;
@@ -2056,7 +2058,7 @@ static void make_cosmo_host(const bolt_cosmo_params& par, const bolt_tables& t, 
     C.Yp = par.Y_p; C.Nnu = par.N_nu; C.mnu = par.Sum_m_nu;
     C.H0 = t.scal[BOLT_S_H0]; C.eta0 = t.scal[BOLT_S_ETA0]; C.rho_crit = t.scal[BOLT_S_RHO_CRIT];
     C.Onu = 7.0 * (2.0 / 3.0) * par.N_nu / 8.0 * std::pow(4.0 / 11.0, 4.0 / 3.0) * par.Omega_r;
-    C.x0 = t.x0; C.x1 = t.x1; C.dx = (t.x1 - t.x0) / (t.nx - 1); C.nx = t.nx; C.nq = t.nq;
+    C.x0 = t.x0; C.x1 = t.x1; C.dx = (t.x1 - t.x0) / (t.nx - 1); C.idx = 1.0 / C.dx; C.nx = t.nx; C.nq = t.nq;
     C.Pprim_A = par.A; C.Pprim_n = par.n;
     {   // plin's Ω_ν (spectra.jl:182-186)
         double Tg = std::pow(15.0 / (M_PI * M_PI) * C.rho_crit * par.Omega_r, 0.25);
