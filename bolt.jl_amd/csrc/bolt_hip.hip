@@ -786,13 +786,22 @@ __device__ inline void affine_step(double& a, double& b) {
     b = fma(a, bp, b);
     a = a * ap;
 }
+// the same for row_shr steps: a lane without a source keeps (a, b) since its
+// b' = a·0 + b with a zero-filled b'
+template <int CTRL>
+__device__ inline void affine_step_shr(double& a, double& b) {
+    const double ap = dpp64_keep<CTRL, 0xF>(a, 1.0);
+    const double bp = dpp64<CTRL>(b);
+    b = fma(a, bp, b);
+    a = a * ap;
+}
 // lane L ends with f_L ∘ f_{L-1} ∘ ... ∘ f_0; every lane of the wavefront
 // must take part
 __device__ inline void affine_scan(double& a, double& b) {
-    affine_step<0x111, 0xF>(a, b);
-    affine_step<0x112, 0xF>(a, b);
-    affine_step<0x114, 0xF>(a, b);
-    affine_step<0x118, 0xF>(a, b);
+    affine_step_shr<0x111>(a, b);
+    affine_step_shr<0x112>(a, b);
+    affine_step_shr<0x114>(a, b);
+    affine_step_shr<0x118>(a, b);
     affine_step<0x142, 0xA>(a, b);
     affine_step<0x143, 0xC>(a, b);
 }
@@ -814,9 +823,11 @@ __device__ inline double pass_scan(double (&al)[SR], double (&be)[SR], double& c
 
 // forward + back substitution of one right-hand side over all n rows by one
 // wavefront (all 64 lanes, uniform control flow).  Branch-free: loads past n
-// stay inside the workspace (it ends with one pass of padding) and are masked
-// to the identity, and stores from lanes whose rows lie past n go to the
-// never-read slot `dump`.
+// stay inside the workspace (it ends with one pass of padding), and stores
+// from lanes whose rows lie past n go to the never-read slot `dump`.  In the
+// forward pass rows past n come after every real row, and the scan carries
+// values only toward later rows, so they need no masking; in the backward
+// pass they come first and are masked to the identity map.
 __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* dump, int lane) {
     constexpr int span = WAVE * SR;
     double carry = 0.0;
@@ -825,10 +836,8 @@ __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* du
         const int j0 = p0 + SR * lane;
 #pragma unroll
         for (int i = 0; i < SR; ++i) {
-            const bool in = j0 + i < n;
-            const double l = Bd.l1[j0 + i], b = bv[j0 + i];
-            al[i] = in ? -l : 0.0;
-            be[i] = in ? b : 0.0;
+            al[i] = -Bd.l1[j0 + i];
+            be[i] = bv[j0 + i];
         }
         // row 3 (Θ₁) also takes -l2·y_1 with y_1 = b_1 - l1_1·b_0
         const double corr = Bd.l2c[3] * fma(al[1], be[0], be[1]);
