@@ -107,6 +107,68 @@ def cpu_baseline(par, bg, ks, ells, dense, threads):
             "est_source_grid_s": t_sg, "est_cltt_s": t_cl, "sample_cpu_s": sum(t_modes) + sum(t_ks)}
 
 
+def serve(args):
+    """Serving workload: seeded Poisson arrivals of spectrum requests, each with
+    its own cosmology drawn from the box h ∈ [0.6, 0.8], Ω_b h² ∈ [0.020, 0.024],
+    Ω_c h² ∈ [0.10, 0.14], n_s ∈ [0.92, 1.0], ln 10¹⁰A_s ∈ [2.9, 3.2], 30 % with
+    one massive ν (Σm_ν ∈ [0.06, 0.3] eV, ℓ_mν = 20); k-grid size from
+    {50, 200, 800}; half P(k) (plin defaults), half C_ℓ^TT to ℓ_max from
+    {500, 1500, 2500} (ℓᵧ = ℓ_ν = 50, reltol 1e-11, 5000 dense k).  Latency is
+    submit to result, host tables included."""
+    import boltmi
+    rng = np.random.default_rng(args.seed)
+    srv = boltmi.Server(slots=args.slots, hbm_gib=args.hbm_gib)
+    reqs = []
+    for _ in range(args.requests):
+        h = rng.uniform(0.6, 0.8)
+        massive = rng.uniform() < 0.3
+        par = boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h**2, Ω_c=rng.uniform(0.10, 0.14) / h**2,
+                                 n=rng.uniform(0.92, 1.0), A=math.exp(rng.uniform(2.9, 3.2)) * 1e-10,
+                                 N_ν=3.046, Σm_ν=(rng.uniform(0.06, 0.3) if massive else 0.0) * boltmi.MASS_NATURAL)
+        nk = int(rng.choice([50, 200, 800]))
+        if rng.uniform() < 0.5:
+            reqs.append(("pk", par, dict(nk=nk)))
+        else:
+            reqs.append(("cltt", par, dict(nk=nk, ell_max=int(rng.choice([500, 1500, 2500])),
+                                           ℓ_mν=20 if massive else 10)))
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    try:
+        # one small request of each kind first (not timed): loads the kernels
+        warm = [srv.submit_pk(boltmi.CosmoParams(), nk=8), srv.submit_cltt(boltmi.CosmoParams(), nk=8, ell_max=50,
+                                                                            nkd=200, ℓᵧ=8, ℓ_ν=8)]
+        for w in warm:
+            srv.wait(w)
+        ids = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for (kind, par, kw), gap in zip(reqs, gaps):
+            t_next += gap
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit_pk(par, **kw) if kind == "pk" else srv.submit_cltt(par, **kw))
+        infos = [srv.wait(i)[1] for i in ids]
+        t1 = time.perf_counter()
+    finally:
+        srv.close()
+    lat = np.array([i["latency_ms"] for i in infos])
+    line = {
+        "metric": METRIC, "value": len(reqs) / (t1 - t0), "unit": "spectra/s", "n_gpus": 1, "steps": len(reqs),
+        "warmup": 2, "ms_per_step": (t1 - t0) * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving: Poisson arrivals at {args.rate}/s, {len(reqs)} requests, "
+                               f"{args.slots} slots; half P(k), half C_l^TT; 30% massive nu",
+                   "seed": args.seed},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "queue_ms_p50": float(np.percentile([i["queue_ms"] for i in infos], 50)),
+        "host_ms_p50": float(np.percentile([i["host_ms"] for i in infos], 50)),
+        "device_ms_p50": float(np.percentile([i["device_ms"] for i in infos], 50)),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,11 +179,22 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the workload (reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--serve", action="store_true",
+                    help="measure the serving workload (Poisson request stream) instead of the full CMB path")
+    ap.add_argument("--rate", type=float, default=50.0, help="--serve: mean request arrivals per second")
+    ap.add_argument("--requests", type=int, default=48, help="--serve: number of requests")
+    ap.add_argument("--slots", type=int, default=4, help="--serve: concurrent request slots")
+    ap.add_argument("--hbm-gib", type=float, default=8.0, help="--serve: HBM bound for the slots")
+    ap.add_argument("--seed", type=int, default=1, help="--serve: seed of the request stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
+
+    if args.serve:
+        serve(args)
+        return
 
     import torch
     import boltmi
