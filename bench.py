@@ -183,8 +183,8 @@ def main():
                     help="measure the serving workload (Poisson request stream) instead of the full CMB path")
     ap.add_argument("--rate", type=float, default=50.0, help="--serve: mean request arrivals per second")
     ap.add_argument("--requests", type=int, default=48, help="--serve: number of requests")
-    ap.add_argument("--slots", type=int, default=4, help="--serve: concurrent request slots")
-    ap.add_argument("--hbm-gib", type=float, default=8.0, help="--serve: HBM bound for the slots")
+    ap.add_argument("--slots", type=int, default=16, help="--serve: concurrent request slots")
+    ap.add_argument("--hbm-gib", type=float, default=16.0, help="--serve: HBM bound for the slots")
     ap.add_argument("--seed", type=int, default=1, help="--serve: seed of the request stream")
     args = ap.parse_args()
     if args.gpus != 1:
