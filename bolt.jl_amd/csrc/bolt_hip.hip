@@ -273,7 +273,7 @@ struct Bands {
 #define NXC 24    // doubles per stored XC
 
 // LDS workspace of one k-mode.  Vectors of n doubles: u, f, u_{n+1}, f_{n+1}, ψ,
-// w, Z₀..Z₅, the border columns c_S, c_P, c_Π, the row recipes (nine
+// w, Z₀..Z₅, the border columns c_S, c_P, c_Π, the row recipes (six
 // coefficient vectors), and the LU of W at the five implicit stage points (3
 // vectors each).  Then the q arrays (mode constants, the current x, the five
 // stage points), the compact second bands, the five stage points' coefficients,
@@ -296,28 +296,28 @@ struct WS {
     __device__ double* rcp_(int j) const { return b + (15 + j) * n; }  // recipe vector j (RC_*)
     __device__ Bands bands(int s) const {
         Bands B;
-        B.l1 = b + (24 + 3 * s) * n; B.d = b + (25 + 3 * s) * n; B.u1 = b + (26 + 3 * s) * n;
-        B.l2c = b + 39 * n + 27 * nq + 2 * CB * s;
+        B.l1 = b + (21 + 3 * s) * n; B.d = b + (22 + 3 * s) * n; B.u1 = b + (23 + 3 * s) * n;
+        B.l2c = b + 36 * n + 27 * nq + 2 * CB * s;
         B.u2c = B.l2c + CB;
         return B;
     }
-    __device__ double* q() const { return b + 39 * n; }
-    __device__ double* df0() const { return b + 39 * n + nq; }
-    __device__ double* base() const { return b + 39 * n + 2 * nq; }
-    __device__ double* qe() const { return b + 39 * n + 3 * nq; }
-    __device__ double* eqdf() const { return b + 39 * n + 4 * nq; }
-    __device__ double* sw() const { return b + 39 * n + 5 * nq; }
-    __device__ double* rw() const { return b + 39 * n + 6 * nq; }
-    __device__ double* qst_base(int s) const { return b + 39 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
+    __device__ double* q() const { return b + 36 * n; }
+    __device__ double* df0() const { return b + 36 * n + nq; }
+    __device__ double* base() const { return b + 36 * n + 2 * nq; }
+    __device__ double* qe() const { return b + 36 * n + 3 * nq; }
+    __device__ double* eqdf() const { return b + 36 * n + 4 * nq; }
+    __device__ double* sw() const { return b + 36 * n + 5 * nq; }
+    __device__ double* rw() const { return b + 36 * n + 6 * nq; }
+    __device__ double* qst_base(int s) const { return b + 36 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
     __device__ QSet qcur() const { QSet Q; Q.qe = qe(); Q.eqdf = eqdf(); Q.sw = sw(); Q.rw = rw(); return Q; }
     __device__ QSet qst(int s) const {
         const double* p = qst_base(s);
         QSet Q; Q.qe = p; Q.eqdf = p + nq; Q.sw = p + 2 * nq; Q.rw = p + 3 * nq;
         return Q;
     }
-    __device__ double* xcs(int s) const { return b + 39 * n + 27 * nq + 10 * CB + NXC * s; }
+    __device__ double* xcs(int s) const { return b + 36 * n + 27 * nq + 10 * CB + NXC * s; }
     // scalar table of stage point s (s < 5) or of the current x (s = 5)
-    __device__ double* ktab(int s) const { return b + 39 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
+    __device__ double* ktab(int s) const { return b + 36 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
     __device__ double* kcur() const { return ktab(5); }
     __device__ double* red() const { return ktab(6); }  // [16]
     __device__ double* dump() const { return red() + 15; }  // store slot nobody reads
@@ -333,7 +333,7 @@ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
     W.nq = nq;
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
-    return sizeof(double) * ((size_t)39 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 16) +
+    return sizeof(double) * ((size_t)36 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 16) +
            sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4) + sizeof(double) * WAVE * SR;  // scan read padding
 }
 
@@ -384,9 +384,19 @@ __device__ __host__ inline int row_desc(const Geo& G, int c) {
 // Row recipes (perturbations.jl:196-265).  Every coefficient of row c is a
 // row constant times one table scalar: l1 = al·K[kl], u1 = au·K[ku],
 // d = dc + dt·τ′ + di/(ℋη) + dr·τ′R, c_S = cs·K[kcs], c_P = cp, c_Π = cpi·τ′,
-// all times K[km] (the RSA factor on photon/ν rows, 1 elsewhere).  The
+// all times K[km] (the RSA factor on photon/ν rows, 1 elsewhere).  dt, dc, dr
+// follow from the row index (d_coef); al, au, di, cs, cp, cpi are stored.  The
 // second band is v_b's u2 = 3τ′R (row 1) and Θ₁'s l2 = τ′/3 (row 3).
-enum { RC_AL = 0, RC_AU, RC_DC, RC_DT, RC_DI, RC_DR, RC_CS, RC_CP, RC_CPI, RC_N };
+enum { RC_AL = 0, RC_AU, RC_DI, RC_CS, RC_CP, RC_CPI, RC_N };
+
+// the diagonal of L at row c: dt = 1 on the photon rows from Θ₁ on (Θ₀ has no
+// τ′ term), dc = -1 on v_b (row 1) and v, dr = 1 on v_b
+__device__ inline double d_coef(const Geo& G, const double* K, int c, double di) {
+    const double dc = (c == 1 || c == G.cD + 1) ? -1.0 : 0.0;
+    const double dtp = (c >= 3 && c < G.cC) ? K[K_TP] : 0.0;
+    const double drp = c == 1 ? K[K_TPR] : 0.0;
+    return dc + dtp + di * K[K_IHETA] + drp;
+}
 
 __device__ inline void row_recipe(const CosmoDev& C, const Geo& G, const WS& W, int c) {
     const int dsc = W.desc()[c];
@@ -396,28 +406,26 @@ __device__ inline void row_recipe(const CosmoDev& C, const Geo& G, const WS& W, 
     const bool photon_nu = type >= R_T0 && type <= R_NN;
     const bool massive = type >= R_M0 && type <= R_MN;
     const int kq = massive ? K_Q0 + qi : K_KH;
-    double r[RC_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double r[RC_N] = {0, 0, 0, 0, 0, 0};
     int kl = K_KH, ku = K_KH, kcs = K_KH;
     if (type == R_TL || type == R_PL || type == R_NL || type == R_ML) {
         // interior of a hierarchy: ℓ/(2ℓ+1) and (ℓ+1)/(2ℓ+1) couplings
         kl = ku = kq;
         r[RC_AL] = (double)l / (double)(2 * l + 1);
         r[RC_AU] = -((double)(l + 1) / (double)(2 * l + 1));
-        r[RC_DT] = photon ? 1.0 : 0.0;
         r[RC_CPI] = (photon && l == 2) ? -0.1 : 0.0;
     } else if (type == R_TN || type == R_PN || type == R_NN || type == R_MN) {
         // truncation rows
         kl = kq;
         r[RC_AL] = 1.0;
         r[RC_DI] = -(double)((type == R_NN ? G.nn : type == R_MN ? G.nm : G.ng) + 1);
-        r[RC_DT] = photon ? 1.0 : 0.0;
     } else {
         switch (type) {
             case R_DB: r[RC_AU] = 1.0; r[RC_CP] = -3.0; break;
-            case R_VB: kl = K_KHCS; r[RC_AL] = -1.0; r[RC_DC] = -1.0; r[RC_DR] = 1.0; r[RC_CS] = -1.0; break;
+            case R_VB: kl = K_KHCS; r[RC_AL] = -1.0; r[RC_CS] = -1.0; break;
             case R_T0: case R_N0: r[RC_AU] = -1.0; r[RC_CP] = -1.0; break;
-            case R_T1: r[RC_AL] = third; r[RC_DT] = 1.0; r[RC_AU] = -2.0 * third; r[RC_CS] = third; break;
-            case R_P0: r[RC_DT] = 1.0; r[RC_AU] = -1.0; r[RC_CPI] = -0.5; break;
+            case R_T1: r[RC_AL] = third; r[RC_AU] = -2.0 * third; r[RC_CS] = third; break;
+            case R_P0: r[RC_AU] = -1.0; r[RC_CPI] = -0.5; break;
             case R_N1: r[RC_AL] = third; r[RC_AU] = -2.0 * third; r[RC_CS] = third; break;
             case R_M0: ku = kq; r[RC_AU] = -1.0; r[RC_CP] = C.df0[qi]; break;
             case R_M1:
@@ -425,7 +433,7 @@ __device__ inline void row_recipe(const CosmoDev& C, const Geo& G, const WS& W, 
                 r[RC_AL] = third; r[RC_AU] = -2.0 * third; r[RC_CS] = -third;
                 break;
             case R_DEL: r[RC_AU] = 1.0; r[RC_CP] = -3.0; break;
-            case R_V: r[RC_DC] = -1.0; r[RC_CS] = -1.0; break;
+            case R_V: r[RC_CS] = -1.0; break;
             default: r[RC_CP] = 1.0; break;  // R_PHI
         }
     }
@@ -504,14 +512,13 @@ __device__ inline void qfill(const CosmoDev& C, const Geo& G, const XC& X, const
 
 // One row of du = L u + c_S Ψ + c_P Φ′ + c_Π Π (perturbations.jl:196-265)
 // from its recipe and the scalar table K of the x point
-__device__ inline double row_eval(const WS& W, const double* K, const double* v, int c, int n, double S, double P,
-                                  double Pi) {
+__device__ inline double row_eval(const Geo& G, const WS& W, const double* K, const double* v, int c, int n, double S,
+                                  double P, double Pi) {
     // branch-free: the first row has no l1 and the last no u1, so clamped
     // neighbours contribute exact zeros
     const int kd = W.kinds()[c];
     const int cm = c > 0 ? c - 1 : 0, cp = c + 1 < n ? c + 1 : c;
-    const double d = W.rcp_(RC_DC)[c] + W.rcp_(RC_DT)[c] * K[K_TP] + W.rcp_(RC_DI)[c] * K[K_IHETA] +
-                     W.rcp_(RC_DR)[c] * K[K_TPR];
+    const double d = d_coef(G, K, c, W.rcp_(RC_DI)[c]);
     double acc = d * v[c];
     acc += (W.rcp_(RC_AL)[c] * K[kd & 255]) * v[cm];
     acc += (W.rcp_(RC_AU)[c] * K[(kd >> 8) & 255]) * v[cp];
@@ -583,7 +590,7 @@ __device__ inline Fun rhs_rows(const CosmoDev& C, const Geo& G, const XC& X, con
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int c = cc[i];
-            r[i] = scale * row_eval(W, K, v, c, G.n, F.S, F.P, F.Pi);
+            r[i] = scale * row_eval(G, W, K, v, c, G.n, F.S, F.P, F.Pi);
             if (make_c) {
                 const int kd = W.kinds()[c];
                 const double m = K[kd >> 24];
@@ -617,14 +624,13 @@ __device__ inline void stage_bands(const Geo& G, const WS& W, double hg, const T
             const int c = cc[i];
             const int kd = W.kinds()[c];
             const int kl = kd & 255, ku = (kd >> 8) & 255, km = kd >> 24;
-            const double al = W.rcp_(RC_AL)[c], au = W.rcp_(RC_AU)[c], dc = W.rcp_(RC_DC)[c], dt = W.rcp_(RC_DT)[c],
-                         di = W.rcp_(RC_DI)[c], dr = W.rcp_(RC_DR)[c];
+            const double al = W.rcp_(RC_AL)[c], au = W.rcp_(RC_AU)[c], di = W.rcp_(RC_DI)[c];
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
                 const double* K = W.ktab(s);
                 const double mh = K[km] * hg;
                 L[i][s] = -mh * (al * K[kl]);
-                D[i][s] = 1.0 - mh * (dc + dt * K[K_TP] + di * K[K_IHETA] + dr * K[K_TPR]);
+                D[i][s] = 1.0 - mh * d_coef(G, K, c, di);
                 U[i][s] = -mh * (au * K[ku]);
             }
         }
@@ -1214,7 +1220,7 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
     if (want_t && T.wv == 0) {
         int c = source_row(G, T.lane);
         if (c >= 0) {
-            W.w()[c] = row_eval(W, W.kcur(), W.arg(), c, G.n, F.S, F.P, F.Pi);
+            W.w()[c] = row_eval(G, W, W.kcur(), W.arg(), c, G.n, F.S, F.P, F.Pi);
         }
     }
     __syncthreads();
