@@ -202,6 +202,13 @@ __device__ inline double rcp(double d) {
     return fma(r1, e2, r1);
 }
 
+// 1/d refined once (relative error e² ≈ 2e-15 from v_rcp_f64's e ≈ 5e-8): the
+// pivots of the banded LU, where the pivot recurrence is the critical path
+__device__ inline double rcp_piv(double d) {
+    const double r = __builtin_amdgcn_rcp(d);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 // largest hierarchy cut-off accepted (row descriptors pack ℓ in 11 bits)
 #define LTAB 512
 
@@ -699,7 +706,7 @@ __device__ inline void factor_tri_block(double* l1, double* d, const double* u1,
 #pragma unroll
     for (int i = 0; i < CB; ++i) {
         const double m1 = L1[i] * ip1;
-        const double inv = rcp(fma(-P[i], ip1, D[i]));
+        const double inv = rcp_piv(fma(-P[i], ip1, D[i]));
         L1[i] = m1; D[i] = inv;
         ip1 = inv;
     }
@@ -716,28 +723,31 @@ __device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m, double* 
     double* l1 = Bd.l1 + c0; double* d = Bd.d + c0; double* u1 = Bd.u1 + c0;
     const bool first = c0 == 0;  // the only segment with a second band
     double ip1 = 0.0, ip2 = 0.0, u1p1 = 0.0, u1p2 = 0.0, u2p1 = 0.0, u2p2 = 0.0;
-    {   // first block: the full bandwidth-2 recurrence
-        double L2[CB], L1[CB], D[CB], U1[CB], U2[CB];
+    {   // rows 0..FB-1: the full bandwidth-2 recurrence (the second band sits in
+        // rows 1 and 3 of the first segment; from row 4 on L is tridiagonal and
+        // row 3's upper entry is unchanged)
+        constexpr int FB = 4;
+        double L2[FB], L1[FB], D[FB], U1[FB], U2[FB];
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
+        for (int i = 0; i < FB; ++i) {
             const double l2 = Bd.l2c[i], u2 = Bd.u2c[i];
             L2[i] = first ? l2 : 0.0; U2[i] = first ? u2 : 0.0;
             L1[i] = l1[i]; D[i] = d[i]; U1[i] = u1[i];
         }
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
+        for (int i = 0; i < FB; ++i) {
             double m2 = L2[i] * ip2;
             double m1 = (L1[i] - m2 * u1p2) * ip1;
             double dd = (D[i] - m2 * u2p2) - m1 * u1p1;
             double uu = U1[i] - m1 * u2p1;
-            double inv = rcp(dd);
+            double inv = rcp_piv(dd);
             L2[i] = m2; L1[i] = m1; D[i] = inv; U1[i] = uu;
             ip2 = ip1; ip1 = inv;
             u1p2 = u1p1; u1p1 = uu;
             u2p2 = u2p1; u2p1 = U2[i];
         }
 #pragma unroll
-        for (int i = 0; i < CB; ++i) {
+        for (int i = 0; i < FB; ++i) {
             const bool in = i < m;
             *(first && in ? Bd.l2c + i : dump) = L2[i];
             *(in ? l1 + i : dump) = L1[i];
@@ -745,9 +755,9 @@ __device__ inline void chain_factor_seg(const Bands& Bd, int c0, int m, double* 
             *(in ? u1 + i : dump) = U1[i];
         }
     }
-    // tridiagonal blocks: full ones unguarded (lanes with shorter segments
-    // leave the loop early), then the partial tail
-    int jb = CB;
+    // tridiagonal blocks of CB rows: full ones unguarded (lanes with shorter
+    // segments leave the loop early), then the partial tail
+    int jb = 4;
     for (; jb + CB <= m; jb += CB) factor_tri_block<false>(l1, d, u1, jb, m, ip1, u1p1, dump);
     if (jb < m) factor_tri_block<true>(l1, d, u1, jb, m, ip1, u1p1, dump);
 }
