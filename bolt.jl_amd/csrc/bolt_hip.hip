@@ -1339,15 +1339,20 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             // the five implicit stage points depend on x and h only: their
             // coefficients, q-quantities, W bands and LU come first, all stages
             // at once
+            // lanes 0..4: each stage point's coefficients; wavefronts 1..3 at
+            // the same time: the q-sets, from a = e^x and kH = k/ℋ formed
+            // exactly as make_xc forms them
             if (T.tid < 5) {
                 const XC Xs = make_xc(C, k, x + KC_C[T.tid + 1] * h);
                 store_xc(W.xcs(T.tid), Xs);
                 ktab_head(W.ktab(T.tid), Xs);
-            }
-            __syncthreads();
-            for (int t = T.tid; t < 5 * G.nq; t += T.nt) {
-                const int st = t / G.nq;
-                qfill_one(C, W, W.xcs(st)[1], W.xcs(st)[7], W.qst_base(st), W.ktab(st), t - st * G.nq);
+            } else if (T.wv > 0) {
+                for (int t = T.tid - WAVE; t < 5 * G.nq; t += T.nt - WAVE) {
+                    const int st = t / G.nq;
+                    const double xs = x + KC_C[st + 1] * h;
+                    const double kHs = k * rcp(spl(C, BOLT_T_HC, xs));
+                    qfill_one(C, W, exp(xs), kHs, W.qst_base(st), W.ktab(st), t - st * G.nq);
+                }
             }
             __syncthreads();
             stage_bands(G, W, hg, T);
