@@ -1121,14 +1121,14 @@ __device__ inline void stage_combine(const Geo& G, const WS& W, double hg, const
 #pragma unroll
             for (int j = 0; j < S; ++j) {
                 const double zj = W.Z(j)[c];
-                if (S < 5) {
+                if constexpr (S < 5) {
                     acc += KCT::A[S + 1][j] * zj;
                 } else {
                     acc += KCT::A[5][j] * zj;
                     err += (KCT::A[5][j] - KCT::BH[j]) * zj;
                 }
             }
-            if (S < 5) {
+            if constexpr (S < 5) {
                 a[i] = acc + KCT::A[S + 1][S] * z[i];
             } else {
                 a[i] = acc + KC_G * z[i];
@@ -1139,7 +1139,7 @@ __device__ inline void stage_combine(const Geo& G, const WS& W, double hg, const
         for (int i = 1; i >= 0; --i) {
             const int c = cc[i];
             W.Z(S)[c] = z[i];
-            if (S < 5) {
+            if constexpr (S < 5) {
                 W.arg()[c] = a[i];
             } else {
                 W.un()[c] = a[i];
