@@ -25,6 +25,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import check, lib, pd, pi
+from .dual import Dual, partial, value
 
 # natural units (c = ħ = k_B = 1, Mpc): 1 eV in Mpc^-1 (Bolt.jl:51)
 MASS_NATURAL = 1.602176634e-19 / (1.054571817e-34 * 299792458.0) * 3.0856775814913673e22
@@ -45,8 +46,19 @@ class CosmoParams:
     def as_tuple(self):
         return (self.h, self.Ω_r, self.Ω_b, self.Ω_c, self.A, self.n, self.Y_p, self.N_ν, self.Σm_ν)
 
+    def is_dual(self):
+        return any(isinstance(v, Dual) for v in self.as_tuple())
+
     def c_struct(self):
+        if self.is_dual():
+            raise TypeError("this entry point takes a plain cosmology; derivatives stop at the host tables "
+                            "(Background / IonizationHistory)")
         return _abi.CosmoParamsC(*[float(v) for v in self.as_tuple()])
+
+    def c_structs_dual(self):
+        """(values, tangent direction) of a CosmoParams holding Dual fields."""
+        return (_abi.CosmoParamsC(*[float(value(v)) for v in self.as_tuple()]),
+                _abi.CosmoParamsC(*[float(partial(v)) for v in self.as_tuple()]))
 
 
 class _Spline:
@@ -60,12 +72,22 @@ class _Spline:
         out = np.empty_like(xs)
         check(lib().bolt_host_spline_eval(ctypes.byref(self._o.c_tables()), self._t, self._d, pd(xs), len(xs),
                                           pd(out)))
-        return out if np.ndim(x) else float(out[0])
+        if self._o.dcoef is None:
+            return out if np.ndim(x) else float(out[0])
+        # the spline is linear in its coefficients: its tangent is the spline of the tangent coefficients
+        dout = np.empty_like(xs)
+        check(lib().bolt_host_spline_eval(ctypes.byref(self._o.c_tables(tangent=True)), self._t, self._d, pd(xs),
+                                          len(xs), pd(dout)))
+        return Dual(out, dout) if np.ndim(x) else Dual(float(out[0]), float(dout[0]))
 
 
 class Background:
     """Background(par; x_grid=-20.0:0.01:0.0, nq=15) with the RECFAST
-    ionization history tables computed alongside (both host-side)."""
+    ionization history tables computed alongside (both host-side).
+
+    If fields of par are Dual numbers, the tables' derivative along their
+    partials is computed too (forward-mode AD, bolt_host_tables_jvp): the
+    splines and scalars then evaluate to Duals, as under ForwardDiff."""
 
     def __init__(self, par: CosmoParams, x_grid=(-20.0, 0.0, 2001), nq=15):
         x0, x1, nx = x_grid
@@ -75,12 +97,21 @@ class Background:
         self.scal = np.zeros(_abi.NSCAL)
         self.quad_pts = np.zeros(self.nq)
         self.quad_wts = np.zeros(self.nq)
-        check(lib().bolt_host_tables(ctypes.byref(par.c_struct()), self.x0, self.x1, self.nx, self.nq,
-                                     pd(self.coef), pd(self.scal), pd(self.quad_pts), pd(self.quad_wts)))
-        self.H0 = self.scal[_abi.S_H0]
-        self.eta0 = self.scal[_abi.S_ETA0]
-        self.rho_crit = self.scal[_abi.S_RHO_CRIT]
-        self.Omega_L = self.scal[_abi.S_OMEGA_L]
+        self.dcoef = self.dscal = None
+        if par.is_dual():
+            pv, pt = par.c_structs_dual()
+            self.dcoef = np.zeros_like(self.coef)
+            self.dscal = np.zeros_like(self.scal)
+            check(lib().bolt_host_tables_jvp(ctypes.byref(pv), ctypes.byref(pt), self.x0, self.x1, self.nx,
+                                             self.nq, pd(self.coef), pd(self.scal), pd(self.dcoef), pd(self.dscal),
+                                             pd(self.quad_pts), pd(self.quad_wts)))
+        else:
+            check(lib().bolt_host_tables(ctypes.byref(par.c_struct()), self.x0, self.x1, self.nx, self.nq,
+                                         pd(self.coef), pd(self.scal), pd(self.quad_pts), pd(self.quad_wts)))
+        self.H0 = self._scalar(_abi.S_H0)
+        self.eta0 = self._scalar(_abi.S_ETA0)
+        self.rho_crit = self._scalar(_abi.S_RHO_CRIT)
+        self.Omega_L = self._scalar(_abi.S_OMEGA_L)
         self.x_grid = np.array([(self.x0 * (self.nx - 1 - j) + self.x1 * j) / (self.nx - 1)
                                 for j in range(self.nx)])
         self.H = _Spline(self, _abi.T_HC)
@@ -91,9 +122,19 @@ class Background:
         self.eta_pp = _Spline(self, _abi.T_ETA_PP)
         self.rho0M = _Spline(self, _abi.T_RHO0M)
         self._ct = None
+        self._ct_tangent = None
         self._uploads = {}
 
-    def c_tables(self):
+    def _scalar(self, i):
+        return float(self.scal[i]) if self.dscal is None else Dual(float(self.scal[i]), float(self.dscal[i]))
+
+    def c_tables(self, tangent=False):
+        if tangent:
+            if self._ct_tangent is None:
+                self._ct_tangent = _abi.TablesC(self.nx, self.nq, self.x0, self.x1,
+                                                (ctypes.c_double * _abi.NSCAL)(*self.dscal), pd(self.quad_pts),
+                                                pd(self.quad_wts), pd(self.dcoef))
+            return self._ct_tangent
         if self._ct is None:
             self._ct = _abi.TablesC(self.nx, self.nq, self.x0, self.x1, (ctypes.c_double * _abi.NSCAL)(*self.scal),
                                     pd(self.quad_pts), pd(self.quad_wts), pd(self.coef))
@@ -104,6 +145,9 @@ class Background:
                     rho_crit=self.rho_crit, xq=self.quad_pts, wq=self.quad_wts, coef=self.coef)
 
     def cosmo_id(self, ctx):
+        if self.dcoef is not None:
+            raise TypeError("a Background built from Dual parameters cannot be uploaded: derivatives stop at the "
+                            "host tables (the tangent-linear perturbation solve is not built)")
         key = id(ctx)
         if key not in self._uploads:
             cid = ctypes.c_int(-1)
@@ -126,7 +170,7 @@ class IonizationHistory:
 
     def __init__(self, rec: RECFAST, par: CosmoParams, bg: Background):
         self.bg = bg
-        self.tau0 = bg.scal[_abi.S_TAU0]
+        self.tau0 = bg._scalar(_abi.S_TAU0)
         self.Xe = _Spline(bg, _abi.T_XE)
         self.tau = _Spline(bg, _abi.T_TAU)
         self.tau_p = _Spline(bg, _abi.T_TAU_P)

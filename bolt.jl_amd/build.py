@@ -37,6 +37,7 @@ def build(verbose=False):
         if src.endswith(".cpp"):
             cmd = [HIPCC, "-x", "c++"] + [f for f in hip_flags() if not f.startswith("--offload-arch")
                                            and f not in ("-Xclang", "-target-feature", "-packed-fp32-ops")] + \
+                  (["-ffp-contract=off"] if src == "host_tables.cpp" else []) + \
                   ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

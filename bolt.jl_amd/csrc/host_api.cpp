@@ -57,6 +57,37 @@ int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx,
     return BOLT_OK;
 }
 
+int bolt_host_tables_jvp(const bolt_cosmo_params* par, const bolt_cosmo_params* dpar, double x0, double x1, int nx,
+                         int nq, double* coef_out, double* scal_out, double* dcoef_out, double* dscal_out,
+                         double* xq_out, double* wq_out) {
+    if (!par || !dpar || !coef_out || !scal_out || !dcoef_out || !dscal_out || !xq_out || !wq_out)
+        return set_error(BOLT_E_ARG, "bolt_host_tables_jvp: NULL argument");
+    if (nx < 4 || nq < 1 || !(x1 > x0)) return set_error(BOLT_E_ARG, "bolt_host_tables_jvp: bad grid");
+    if (!(par->Omega_r > 0) || !(par->N_nu > 0) || !(par->h > 0))
+        return set_error(BOLT_E_ARG, "bolt_host_tables_jvp: Omega_r, N_nu and h must be positive");
+    const double* dp = &dpar->h;
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(dp[i])) return set_error(BOLT_E_ARG, "bolt_host_tables_jvp: non-finite direction");
+    // the Dual run's values are bitwise the double run's; they are taken from
+    // bolt_host_tables itself so that the two entry points share one code path
+    int rc = bolt_host_tables(par, x0, x1, nx, nq, coef_out, scal_out, xq_out, wq_out);
+    if (rc != BOLT_OK) return rc;
+    try {
+        bolt::BackgroundT<bolt::Dual> bg(bolt::ParamsT<bolt::Dual>(*par, *dpar), x0, x1, nx, nq);
+        bg.ionization(-1.0);
+        for (size_t i = 0; i < bg.coef.size(); ++i) {
+            dcoef_out[i] = bg.coef[i].d;
+            if (!std::isfinite(dcoef_out[i]))
+                return set_error(BOLT_E_NUMERIC, "bolt_host_tables_jvp: non-finite tangent");
+        }
+        const bolt::Dual s[BOLT_NSCAL] = {bg.H0, bg.eta0, bg.rho_crit, bg.Omega_L, bg.tau0};
+        for (int i = 0; i < BOLT_NSCAL; ++i) dscal_out[i] = s[i].d;
+    } catch (const std::exception& e) {
+        return set_error(BOLT_E_NUMERIC, std::string("bolt_host_tables_jvp: ") + e.what());
+    }
+    return BOLT_OK;
+}
+
 int bolt_host_recfast_xe(const bolt_cosmo_params* par, double Tnow, const double* z, int nz,
                          double* xe_out) {
     if (!par || !z || !xe_out || nz < 0) return set_error(BOLT_E_ARG, "bolt_host_recfast_xe: bad argument");

@@ -3,6 +3,13 @@
 // for every table, Background and IonizationHistory(RECFAST).  The tables are
 // what the device kernels (and the CPU oracle, in the tests) consume.
 //
+// Everything that depends on the cosmological parameters is a template over
+// the scalar type R and is instantiated twice: R = double makes the tables,
+// R = Dual makes the tables and their derivative along one parameter direction
+// (forward-mode AD).  The Dual instantiation's values are bitwise the double
+// instantiation's; the file is compiled without floating-point contraction so
+// that fused multiply-adds cannot make the two differ.
+//
 // Citations are into Bolt.jl (src/...).
 #include "host_tables.h"
 
@@ -12,6 +19,7 @@
 #include <vector>
 #include <algorithm>
 #include <functional>
+#include <type_traits>
 
 namespace bolt {
 
@@ -85,13 +93,15 @@ void gauss_legendre(int n, double* x, double* w) {
 // Uniform cubic B-spline with Line(OnGrid()) ends == natural cubic spline
 // (util.jl:11-13 via Interpolations.jl BSpline(Cubic(Line(OnGrid())))).
 // Coefficients c[0..n+1]; node j (0-based) sits between c[j] and c[j+2].
-void spline_prefilter(const double* y, int n, double* c) {
+template <class R>
+static void prefilter_t(const R* y, int n, R* c) {
     // End rows: c0 - 2c1 + c2 = 0 together with (c0 + 4c1 + c2)/6 = y0 give c1 = y0.
     c[1] = y[0];
     c[n] = y[n - 1];
     int m = n - 2;  // unknowns c[2..n-1]
     if (m > 0) {
-        std::vector<double> cp(m), d(m);
+        std::vector<double> cp(m);
+        std::vector<R> d(m);
         for (int i = 0; i < m; ++i) d[i] = 6.0 * y[i + 1];
         d[0] -= c[1];
         d[m - 1] -= c[n];
@@ -111,36 +121,44 @@ void spline_prefilter(const double* y, int n, double* c) {
     c[n + 1] = 2.0 * c[n] - c[n - 1];
 }
 
-double spline_eval(const double* c, int n, double x0, double dx, double x, int deriv) {
-    double t = (x - x0) / dx;
-    int i = (int)std::floor(t);
+void spline_prefilter(const double* y, int n, double* c) { prefilter_t(y, n, c); }
+
+template <class R>
+static R eval_t(const R* c, int n, double x0, double dx, R x, int deriv) {
+    R t = (x - x0) / dx;
+    int i = (int)std::floor(val(t));
     if (i < 0) i = 0;
     if (i > n - 2) i = n - 2;
-    double s = t - i, r = 1.0 - s;
-    const double* p = c + i;
+    R s = t - i, r = 1.0 - s;
+    const R* p = c + i;
     if (deriv == 0) {
-        double w0 = r * r * r / 6.0;
-        double w1 = 2.0 / 3.0 - s * s + 0.5 * s * s * s;
-        double w2 = 2.0 / 3.0 - r * r + 0.5 * r * r * r;
-        double w3 = s * s * s / 6.0;
+        R w0 = r * r * r / 6.0;
+        R w1 = 2.0 / 3.0 - s * s + 0.5 * s * s * s;
+        R w2 = 2.0 / 3.0 - r * r + 0.5 * r * r * r;
+        R w3 = s * s * s / 6.0;
         return w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3];
     } else if (deriv == 1) {
-        double w0 = -0.5 * r * r;
-        double w1 = -2.0 * s + 1.5 * s * s;
-        double w2 = 2.0 * r - 1.5 * r * r;
-        double w3 = 0.5 * s * s;
+        R w0 = -0.5 * r * r;
+        R w1 = -2.0 * s + 1.5 * s * s;
+        R w2 = 2.0 * r - 1.5 * r * r;
+        R w3 = 0.5 * s * s;
         return (w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3]) / dx;
     } else {
-        double w0 = r, w1 = -2.0 + 3.0 * s, w2 = -2.0 + 3.0 * r, w3 = s;
+        R w0 = r, w1 = -2.0 + 3.0 * s, w2 = -2.0 + 3.0 * r, w3 = s;
         return (w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3]) / (dx * dx);
     }
 }
 
+double spline_eval(const double* c, int n, double x0, double dx, double x, int deriv) {
+    return eval_t(c, n, x0, dx, x, deriv);
+}
+
 // d/dx and d2/dx2 of the spline at its own nodes (Interpolations.gradient /
 // hessian at x in x_grid), used to build spline_∂ₓ / spline_∂ₓ² (util.jl:12-13).
-static void node_derivatives(const double* c, int n, double dx, int deriv, double* out) {
+template <class R>
+static void node_derivatives(const R* c, int n, double dx, int deriv, R* out) {
     for (int j = 0; j < n; ++j) {
-        const double* p = c + j;  // p[0]=c[j], p[1]=c[j+1] (node), p[2]=c[j+2]
+        const R* p = c + j;  // p[0]=c[j], p[1]=c[j+1] (node), p[2]=c[j+2]
         if (deriv == 1) out[j] = (p[2] - p[0]) / (2.0 * dx);
         else out[j] = (p[0] - 2.0 * p[1] + p[2]) / (dx * dx);
     }
@@ -148,64 +166,74 @@ static void node_derivatives(const double* c, int n, double dx, int deriv, doubl
 
 // ----------------------------------------------------------------------------
 // Cosmology helpers (background.jl:6-77)
-Cosmo::Cosmo(const bolt_cosmo_params& p, int nq) : par(p) {
+template <class R>
+CosmoT<R>::CosmoT(const ParamsT<R>& p, int nq) : par(p) {
     const NaturalUnits& u = natural_units();
     H0 = par.h * u.km_s_Mpc_100;
     rho_crit = (3.0 / (8.0 * M_PI)) * H0 * H0 / u.G_natural;
     const double zeta3 = 1.2020569;
-    double Tg = std::pow(15.0 / (M_PI * M_PI) * rho_crit * par.Omega_r, 0.25);
-    double nufac = (90.0 * zeta3 / (11.0 * std::pow(M_PI, 4))) * (par.Omega_r * par.h * par.h / Tg) *
-                   std::pow(par.N_nu / 3.0, 0.75);
-    double Omega_nu = par.Sum_m_nu * nufac / (par.h * par.h);
+    R Tg = pow(15.0 / (M_PI * M_PI) * rho_crit * par.Omega_r, 0.25);
+    R nufac = (90.0 * zeta3 / (11.0 * std::pow(M_PI, 4))) * (par.Omega_r * par.h * par.h / Tg) *
+              pow(par.N_nu / 3.0, 0.75);
+    R Omega_nu = par.Sum_m_nu * nufac / (par.h * par.h);
     rad_fac = 1.0 + (2.0 / 3.0) * (7.0 * par.N_nu / 8.0) * std::pow(4.0 / 11.0, 4.0 / 3.0);
     Omega_L = 1.0 - (par.Omega_r * rad_fac + par.Omega_b + par.Omega_c + Omega_nu);
-    Tnu = std::pow(par.N_nu / 3.0, 0.25) * std::pow(4.0 / 11.0, 1.0 / 3.0) *
-          std::pow(15.0 / (M_PI * M_PI) * rho_crit * par.Omega_r, 0.25);
-    logqmin = std::log10(Tnu / 30.0);
-    logqmax = std::log10(Tnu * 30.0);
+    Tnu = pow(par.N_nu / 3.0, 0.25) * std::pow(4.0 / 11.0, 1.0 / 3.0) *
+          pow(15.0 / (M_PI * M_PI) * rho_crit * par.Omega_r, 0.25);
+    logqmin = log10(Tnu / 30.0);
+    logqmax = log10(Tnu * 30.0);
     xq.resize(nq);
     wq.resize(nq);
     gauss_legendre(nq, xq.data(), wq.data());
 }
 
-double Cosmo::f0(double q) const {
-    return 2.0 / std::pow(2.0 * M_PI, 3) / (std::exp(q / Tnu) + 1.0);
+template <class R>
+R CosmoT<R>::f0(R q) const {
+    return 2.0 / std::pow(2.0 * M_PI, 3) / (exp(q / Tnu) + 1.0);
 }
 
 // xq2q, dxdq (util.jl:24-27)
-static inline double from_ui(double x, double lo, double hi) { return lo + (hi - lo) / 2.0 * (x + 1.0); }
-static inline double to_ui(double lq, double lo, double hi) { return -1.0 + 2.0 / (hi - lo) * (lq - lo); }
-static inline double dxdq(double q, double lo, double hi) {
-    return (1.0 + to_ui(1.0 + lo, lo, hi)) / (q * std::log(10.0));
+template <class R>
+static inline R from_ui(double x, R lo, R hi) { return lo + (hi - lo) / 2.0 * (x + 1.0); }
+template <class R>
+static inline R to_ui(R lq, R lo, R hi) { return -1.0 + 2.0 / (hi - lo) * (lq - lo); }
+template <class R>
+static inline R dxdq(R q, R lo, R hi) {
+    return (1.0 + to_ui(R(1.0 + lo), lo, hi)) / (q * std::log(10.0));
 }
-static inline double xq2q(double x, double lo, double hi) { return std::pow(10.0, from_ui(x, lo, hi)); }
+template <class R>
+static inline R xq2q(double x, R lo, R hi) { return pow(10.0, from_ui(x, lo, hi)); }
 
 // ρP_0(a)[1] (background.jl:32-47)
-double Cosmo::rho0_massive(double a) const {
-    double m = par.Sum_m_nu, s = 0.0;
+template <class R>
+R CosmoT<R>::rho0_massive(R a) const {
+    R m = par.Sum_m_nu, s = 0.0;
     for (size_t i = 0; i < xq.size(); ++i) {
-        double q = xq2q(xq[i], logqmin, logqmax);
-        double eps = std::sqrt(q * q + (a * m) * (a * m));
-        double I = q * q * eps * f0(q) / dxdq(q, logqmin, logqmax);
+        R q = xq2q(xq[i], logqmin, logqmax);
+        R eps = sqrt(q * q + (a * m) * (a * m));
+        R I = q * q * eps * f0(q) / dxdq(q, logqmin, logqmax);
         s += I * wq[i];
     }
-    return 4.0 * M_PI * std::pow(a, -4) * s;
+    return 4.0 * M_PI * pow(a, -4) * s;
 }
 
 // H_a, ℋ_a (background.jl:57-65)
-double Cosmo::H_a(double a) const {
-    double rho_nu = rho0_massive(a);
-    return H0 * std::sqrt((par.Omega_c + par.Omega_b) * std::pow(a, -3) + rho_nu / rho_crit +
-                          par.Omega_r * std::pow(a, -4) * rad_fac + Omega_L);
+template <class R>
+R CosmoT<R>::H_a(R a) const {
+    R rho_nu = rho0_massive(a);
+    return H0 * sqrt((par.Omega_c + par.Omega_b) * pow(a, -3) + rho_nu / rho_crit +
+                     par.Omega_r * pow(a, -4) * rad_fac + Omega_L);
 }
-double Cosmo::Hc_a(double a) const { return a * H_a(a); }
+template <class R>
+R CosmoT<R>::Hc_a(R a) const { return a * H_a(a); }
 
 // η(x) by GL quadrature in log10 a from -13.75 (background.jl:72-77)
-double Cosmo::eta(double x) const {
-    double lo = -13.75, hi = std::log10(std::exp(x));
-    double s = 0.0;
+template <class R>
+R CosmoT<R>::eta(R x) const {
+    R lo = -13.75, hi = log10(exp(x));
+    R s = 0.0;
     for (size_t i = 0; i < xq.size(); ++i) {
-        double ay = xq2q(xq[i], lo, hi);
+        R ay = xq2q(xq[i], lo, hi);
         s += 1.0 / (ay * Hc_a(ay)) / dxdq(ay, lo, hi) * wq[i];
     }
     return s;
@@ -215,9 +243,25 @@ double Cosmo::eta(double x) const {
 // Explicit Dormand-Prince 5(4) with step clipping onto requested outputs; it
 // stands in for Tsit5 in RECFAST's stage solves (recfast.jl:463-471, 530-532).
 // Outputs are exact to the (tight) tolerance, not Tsit5's interpolant.
-template <int N, class F>
-static void dopri_outputs(F&& f, double z0, double* y, const std::vector<double>& zout,
-                          std::vector<std::array<double, N>>& yout, double rtol, double atol) {
+//
+// The independent variable and the step sizes are plain values and the error
+// norm looks at the values only, so a Dual run takes the double run's steps and
+// differentiates the state along them (its values are the double run's).  A
+// Dual start point z0 enters by the chain rule, y(z) = S(z; z0, y0(z0)) with
+// dS/dz0 = -Phi f(z0, y0): the start state's tangent becomes y0' - f(z0, y0) z0'.
+// A Dual output point adds f(z_out, y) z_out'.  (Carrying tangents on the step
+// sizes instead differentiates the explicit stability function at the edge of
+// its stability region, which diverges on RECFAST's stiff hydrogen stage.)
+// Where a stage starts on the Saha curve, the start tangent has a stiff
+// transient that the explicit steps, sized for the values, resolve only
+// roughly: the first few outputs of the helium stage carry tangents good to
+// about 1e-4 relative; elsewhere they follow the values' accuracy.
+static inline void add_tangent(double&, double, double) {}
+static inline void add_tangent(Dual& y, double slope, double dz) { y.d += slope * dz; }
+
+template <class R, int N, class F>
+static void dopri_outputs(F&& f, R z0_, R* y, const std::vector<R>& zout, std::vector<std::array<R, N>>& yout,
+                          double rtol, double atol) {
     static const double c2 = 1.0 / 5, c3 = 3.0 / 10, c4 = 4.0 / 5, c5 = 8.0 / 9;
     static const double a21 = 1.0 / 5;
     static const double a31 = 3.0 / 40, a32 = 9.0 / 40;
@@ -231,12 +275,17 @@ static void dopri_outputs(F&& f, double z0, double* y, const std::vector<double>
     static const double e1 = 71.0 / 57600, e3 = -71.0 / 16695, e4 = 71.0 / 1920,
                         e5 = -17253.0 / 339200, e6 = 22.0 / 525, e7 = -1.0 / 40;
     yout.resize(zout.size());
+    const double z0 = val(z0_);
     double z = z0;
-    double h = (zout.empty() ? 0.0 : (zout.back() < z0 ? -1e-3 : 1e-3)) * std::max(1.0, std::fabs(z0));
-    double k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], yt[N], yn[N];
-    f(z, y, k1);
+    double h = (zout.empty() ? 0.0 : (val(zout.back()) < z0 ? -1e-3 : 1e-3)) * std::max(1.0, std::fabs(z0));
+    R k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], yt[N], yn[N];
+    f(R(z), y, k1);
+    if (tangent(z0_) != 0.0) {
+        for (int i = 0; i < N; ++i) add_tangent(y[i], -val(k1[i]), tangent(z0_));
+        f(R(z), y, k1);
+    }
     for (size_t io = 0; io < zout.size(); ++io) {
-        double zt = zout[io];
+        const double zt = val(zout[io]);
         int guard = 0;
         while (z != zt) {
             if (++guard > 50000000) break;
@@ -245,24 +294,24 @@ static void dopri_outputs(F&& f, double z0, double* y, const std::vector<double>
             bool last = false;
             if (std::fabs(h) >= std::fabs(zt - z)) { h = zt - z; last = true; }
             for (int i = 0; i < N; ++i) yt[i] = y[i] + h * a21 * k1[i];
-            f(z + c2 * h, yt, k2);
+            f(R(z + c2 * h), yt, k2);
             for (int i = 0; i < N; ++i) yt[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
-            f(z + c3 * h, yt, k3);
+            f(R(z + c3 * h), yt, k3);
             for (int i = 0; i < N; ++i) yt[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
-            f(z + c4 * h, yt, k4);
+            f(R(z + c4 * h), yt, k4);
             for (int i = 0; i < N; ++i)
                 yt[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
-            f(z + c5 * h, yt, k5);
+            f(R(z + c5 * h), yt, k5);
             for (int i = 0; i < N; ++i)
                 yt[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
-            f(z + h, yt, k6);
+            f(R(z + h), yt, k6);
             for (int i = 0; i < N; ++i)
                 yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
-            f(z + h, yn, k7);
+            f(R(z + h), yn, k7);
             double err = 0.0;
             for (int i = 0; i < N; ++i) {
-                double e = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] + e7 * k7[i]);
-                double sc = atol + rtol * std::max(std::fabs(y[i]), std::fabs(yn[i]));
+                double e = val(h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] + e7 * k7[i]));
+                double sc = atol + rtol * std::max(std::fabs(val(y[i])), std::fabs(val(yn[i])));
                 err += (e / sc) * (e / sc);
             }
             err = std::sqrt(err / N);
@@ -276,14 +325,18 @@ static void dopri_outputs(F&& f, double z0, double* y, const std::vector<double>
                 h *= std::max(0.1, 0.9 * std::pow(err, -0.2));
             }
         }
-        for (int i = 0; i < N; ++i) yout[io][i] = y[i];
+        for (int i = 0; i < N; ++i) {
+            yout[io][i] = y[i];
+            add_tangent(yout[io][i], val(k1[i]), tangent(zout[io]));
+        }
     }
 }
 
 // ----------------------------------------------------------------------------
 // RECFAST (recfast.jl:22-121 constants, 156-310 rates, 446-536 stages)
-struct Recfast {
-    const Background* bg;
+template <class R>
+struct RecfastT {
+    const BackgroundT<R>* bg;
     double C = 2.99792458e8, k_B = 1.380658e-23, h_P = 6.6260755e-34, m_e = 9.1093897e-31,
            m_H = 1.673575e-27, not4 = 3.9715e0, sigma = 6.6524616e-29, a = 7.565914e-16,
            G = 6.6742e-11;
@@ -303,9 +356,10 @@ struct Recfast {
         CL, CL_He, CT, Bfact;
     double H_frac = 1e-3;
     int Hswitch = 1, Heswitch = 6;
-    double Yp, OmegaB, HO, OmegaG, Tnow, mu_H, mu_T, fHe, Nnow, fu, b_He = 0.86;
+    R Yp, OmegaB, HO, OmegaG, Tnow, mu_H, mu_T, fHe, Nnow;
+    double fu, b_He = 0.86;
 
-    Recfast(const Background* b, double Yp_, double OmegaB_, double OmegaG_, double Tnow_) : bg(b) {
+    RecfastT(const BackgroundT<R>* b, R Yp_, R OmegaB_, R OmegaG_, double Tnow_) : bg(b) {
         Lalpha = 1 / L_H_alpha;
         Lalpha_He = 1 / L_He_2p;
         DeltaB = h_P * C * (L_H_ion - L_H_alpha);
@@ -327,146 +381,156 @@ struct Recfast {
         const NaturalUnits& u = natural_units();
         HO = bg->H0 / u.H0_conv;
         OmegaG = OmegaG_;
-        Tnow = (Tnow_ > 0) ? Tnow_
-                           : std::pow(15.0 / (M_PI * M_PI) * bg->rho_crit * OmegaG, 0.25) * u.K_conv;
-        mu_H = 1 / (1 - Yp);
+        Tnow = (Tnow_ > 0) ? R(Tnow_) : pow(15.0 / (M_PI * M_PI) * bg->rho_crit * OmegaG, 0.25) * u.K_conv;
+        mu_H = 1.0 / (1.0 - Yp);
         mu_T = not4 / (not4 - (not4 - 1) * Yp);
-        fHe = Yp / (not4 * (1 - Yp));
-        Nnow = 3 * HO * HO * OmegaB / (8 * M_PI * G * mu_H * m_H);
+        fHe = Yp / (not4 * (1.0 - Yp));
+        Nnow = 3.0 * HO * HO * OmegaB / (8 * M_PI * G * mu_H * m_H);
         fu = (Hswitch == 0) ? 1.14 : 1.125;
     }
 
-    double Hc(double x) const { return bg->eval(BOLT_T_HC, x, 0); }
-    double Hc_p(double x) const { return bg->eval(BOLT_T_HC_P, x, 0); }
+    R Hc(R x) const { return bg->eval(BOLT_T_HC, x, 0); }
+    R Hc_p(R x) const { return bg->eval(BOLT_T_HC_P, x, 0); }
 
     // ion_recfast (recfast.jl:156-310)
-    void rates(const double* y, double z, double* f) const {
-        double f1 = 0, f2 = 0, f3 = 0;
-        double x_H = y[0], x_He = y[1], x = x_H + fHe * x_He, Tmat = y[2];
-        double n = Nnow * std::pow(1 + z, 3);
-        double n_He = fHe * Nnow * std::pow(1 + z, 3);
-        double Trad = Tnow * (1 + z);
-        double aa = 1 / (1 + z);
-        double x_a = std::log(aa);
-        double Hz = Hc(x_a) / aa / natural_units().H0_conv;
-        double dHdz = (-Hc_p(x_a) + Hc(x_a)) / natural_units().H0_conv;
-        double Rdown = 1e-19 * a_PPB * std::pow(Tmat / 1e4, b_PPB) / (1. + c_PPB * std::pow(Tmat / 1e4, d_PPB));
-        double Rup = Rdown * std::pow(CR * Tmat, 1.5) * std::exp(-CDB / Tmat);
-        double sq_0 = std::sqrt(Tmat / T_0), sq_1 = std::sqrt(Tmat / T_1);
-        double Rdown_He = a_VF / (sq_0 * std::pow(1 + sq_0, 1 - b_VF));
-        Rdown_He = Rdown_He / std::pow(1 + sq_1, 1 + b_VF);
-        double Rup_He = Rdown_He * std::pow(CR * Tmat, 1.5) * std::exp(-CDB_He / Tmat);
+    void rates(const R* y, R z, R* f) const {
+        R f1 = 0.0, f2 = 0.0, f3 = 0.0;
+        R x_H = y[0], x_He = y[1], x = x_H + fHe * x_He, Tmat = y[2];
+        R n = Nnow * pow(1.0 + z, 3);
+        R n_He = fHe * Nnow * pow(1.0 + z, 3);
+        R Trad = Tnow * (1.0 + z);
+        R aa = 1.0 / (1.0 + z);
+        R x_a = log(aa);
+        R Hz = Hc(x_a) / aa / natural_units().H0_conv;
+        R dHdz = (-Hc_p(x_a) + Hc(x_a)) / natural_units().H0_conv;
+        R Rdown = 1e-19 * a_PPB * pow(Tmat / 1e4, b_PPB) / (1. + c_PPB * pow(Tmat / 1e4, d_PPB));
+        R Rup = Rdown * pow(CR * Tmat, 1.5) * exp(-CDB / Tmat);
+        R sq_0 = sqrt(Tmat / T_0), sq_1 = sqrt(Tmat / T_1);
+        R Rdown_He = a_VF / (sq_0 * pow(1.0 + sq_0, 1 - b_VF));
+        Rdown_He = Rdown_He / pow(1.0 + sq_1, 1 + b_VF);
+        R Rup_He = Rdown_He * pow(CR * Tmat, 1.5) * exp(-CDB_He / Tmat);
         Rup_He = 4. * Rup_He;
-        double He_Boltz = (Bfact / Tmat > 680.) ? std::exp(680.) : std::exp(Bfact / Tmat);
-        double K;
+        R He_Boltz = (Bfact / Tmat > 680.) ? R(std::exp(680.)) : exp(Bfact / Tmat);
+        R K;
         if (Hswitch == 0) K = CK / Hz;
         else
             K = CK / Hz *
-                (1.0 + AGauss1 * std::exp(-std::pow((std::log(1 + z) - zGauss1) / wGauss1, 2)) +
-                 AGauss2 * std::exp(-std::pow((std::log(1 + z) - zGauss2) / wGauss2, 2)));
-        double Rdown_trip = a_trip / (sq_0 * std::pow(1 + sq_0, 1 - b_trip));
-        Rdown_trip = Rdown_trip / std::pow(1 + sq_1, 1 + b_trip);
-        double Rup_trip = Rdown_trip * std::exp(-h_P * C * L_He2St_ion / (k_B * Tmat));
-        Rup_trip = Rup_trip * std::pow(CR * Tmat, 1.5) * (4.0 / 3.0);
+                (1.0 + AGauss1 * exp(-pow((log(1.0 + z) - zGauss1) / wGauss1, 2)) +
+                 AGauss2 * exp(-pow((log(1.0 + z) - zGauss2) / wGauss2, 2)));
+        R Rdown_trip = a_trip / (sq_0 * pow(1.0 + sq_0, 1 - b_trip));
+        Rdown_trip = Rdown_trip / pow(1.0 + sq_1, 1 + b_trip);
+        R Rup_trip = Rdown_trip * exp(-h_P * C * L_He2St_ion / (k_B * Tmat));
+        Rup_trip = Rup_trip * pow(CR * Tmat, 1.5) * (4.0 / 3.0);
         int Heflag = ((x_He < 5.e-9) || (x_He > 0.980)) ? 0 : Heswitch;
-        double K_He, CfHe_t = 0;
+        R K_He, CfHe_t = 0.0;
         if (Heflag == 0) {
             K_He = CK_He / Hz;
         } else {
-            double tauHe_s = A2P_s * CK_He * 3 * n_He * (1 - x_He) / Hz;
-            double pHe_s = (1 - std::exp(-tauHe_s)) / tauHe_s;
-            K_He = 1 / (A2P_s * pHe_s * 3 * n_He * (1 - x_He));
+            R tauHe_s = A2P_s * CK_He * 3 * n_He * (1.0 - x_He) / Hz;
+            R pHe_s = (1.0 - exp(-tauHe_s)) / tauHe_s;
+            K_He = 1.0 / (A2P_s * pHe_s * 3 * n_He * (1.0 - x_He));
             if (((Heflag == 2) || (Heflag >= 5)) && (x_H < 0.9999999)) {
-                double Doppler = 2 * k_B * Tmat / (m_H * not4 * C * C);
-                Doppler = C * L_He_2p * std::sqrt(Doppler);
-                double gamma_2Ps = 3 * A2P_s * fHe * (1 - x_He) * C * C /
-                                   (std::sqrt(M_PI) * sigma_He_2Ps * 8 * M_PI * Doppler * (1 - x_H)) /
-                                   std::pow(C * L_He_2p, 2);
+                R Doppler = 2 * k_B * Tmat / (m_H * not4 * C * C);
+                Doppler = C * L_He_2p * sqrt(Doppler);
+                R gamma_2Ps = 3 * A2P_s * fHe * (1.0 - x_He) * C * C /
+                              (std::sqrt(M_PI) * sigma_He_2Ps * 8 * M_PI * Doppler * (1.0 - x_H)) /
+                              std::pow(C * L_He_2p, 2);
                 double pb = 0.36, qb = b_He;
-                double AHcon = A2P_s / (1 + pb * std::pow(gamma_2Ps, qb));
-                K_He = 1 / ((A2P_s * pHe_s + AHcon) * 3 * n_He * (1 - x_He));
+                R AHcon = A2P_s / (1.0 + pb * pow(gamma_2Ps, qb));
+                K_He = 1.0 / ((A2P_s * pHe_s + AHcon) * 3 * n_He * (1.0 - x_He));
             }
             if (Heflag >= 3) {
-                double tauHe_t = A2P_t * n_He * (1. - x_He) * 3;
+                R tauHe_t = A2P_t * n_He * (1. - x_He) * 3;
                 tauHe_t = tauHe_t / (8 * M_PI * Hz * std::pow(L_He_2Pt, 3));
-                double pHe_t = (1 - std::exp(-tauHe_t)) / tauHe_t;
+                R pHe_t = (1.0 - exp(-tauHe_t)) / tauHe_t;
                 double CL_PSt = h_P * C * (L_He_2Pt - L_He_2St) / k_B;
                 if ((Heflag == 3) || (Heflag == 5) || (x_H > 0.99999)) {
-                    CfHe_t = A2P_t * pHe_t * std::exp(-CL_PSt / Tmat);
+                    CfHe_t = A2P_t * pHe_t * exp(-CL_PSt / Tmat);
                     CfHe_t = CfHe_t / (Rup_trip + CfHe_t);
                 } else {
-                    double Doppler = 2 * k_B * Tmat / (m_H * not4 * C * C);
-                    Doppler = C * L_He_2Pt * std::sqrt(Doppler);
-                    double gamma_2Pt = (3 * A2P_t * fHe * (1 - x_He) * C * C /
-                                        (std::sqrt(M_PI) * sigma_He_2Pt * 8 * M_PI * Doppler * (1 - x_H)) /
-                                        std::pow(C * L_He_2Pt, 2));
+                    R Doppler = 2 * k_B * Tmat / (m_H * not4 * C * C);
+                    Doppler = C * L_He_2Pt * sqrt(Doppler);
+                    R gamma_2Pt = (3 * A2P_t * fHe * (1.0 - x_He) * C * C /
+                                   (std::sqrt(M_PI) * sigma_He_2Pt * 8 * M_PI * Doppler * (1.0 - x_H)) /
+                                   std::pow(C * L_He_2Pt, 2));
                     double pb = 0.66, qb = 0.9;
-                    double AHcon = A2P_t / (1 + pb * std::pow(gamma_2Pt, qb)) / 3;
-                    CfHe_t = (A2P_t * pHe_t + AHcon) * std::exp(-CL_PSt / Tmat);
+                    R AHcon = A2P_t / (1.0 + pb * pow(gamma_2Pt, qb)) / 3;
+                    CfHe_t = (A2P_t * pHe_t + AHcon) * exp(-CL_PSt / Tmat);
                     CfHe_t = CfHe_t / (Rup_trip + CfHe_t);
                 }
             }
         }
-        double timeTh = (1 / (CT * std::pow(Trad, 4))) * (1 + x + fHe) / x;
-        double timeH = 2 / (3 * HO * std::pow(1 + z, 1.5));
+        R timeTh = (1.0 / (CT * pow(Trad, 4))) * (1.0 + x + fHe) / x;
+        R timeH = 2.0 / (3 * HO * pow(1.0 + z, 1.5));
         if (x_H > 0.99) {
             f1 = 0.;
         } else if (x_H > 0.985) {
-            f1 = (x * x_H * n * Rdown - Rup * (1 - x_H) * std::exp(-CL / Tmat)) / (Hz * (1 + z));
+            f1 = (x * x_H * n * Rdown - Rup * (1.0 - x_H) * exp(-CL / Tmat)) / (Hz * (1.0 + z));
         } else {
-            f1 = (((x * x_H * n * Rdown - Rup * (1.0 - x_H) * std::exp(-CL / Tmat)) *
+            f1 = (((x * x_H * n * Rdown - Rup * (1.0 - x_H) * exp(-CL / Tmat)) *
                    (1.0 + K * Lambda * n * (1.0 - x_H))) /
                   (Hz * (1.0 + z) * (1.0 / fu + K * Lambda * n * (1.0 - x_H) / fu + K * Rup * n * (1.0 - x_H))));
         }
         if (x_He < 1e-15) {
             f2 = 0.;
         } else {
-            f2 = (((x * x_He * n * Rdown_He - Rup_He * (1 - x_He) * std::exp(-CL_He / Tmat)) *
-                   (1 + K_He * Lambda_He * n_He * (1 - x_He) * He_Boltz)) /
-                  (Hz * (1 + z) * (1 + K_He * (Lambda_He + Rup_He) * n_He * (1 - x_He) * He_Boltz)));
+            f2 = (((x * x_He * n * Rdown_He - Rup_He * (1.0 - x_He) * exp(-CL_He / Tmat)) *
+                   (1.0 + K_He * Lambda_He * n_He * (1.0 - x_He) * He_Boltz)) /
+                  (Hz * (1.0 + z) * (1.0 + K_He * (Lambda_He + Rup_He) * n_He * (1.0 - x_He) * He_Boltz)));
             if (Heflag >= 3) {
                 f2 = f2 + (x * x_He * n * Rdown_trip -
-                           (1 - x_He) * 3 * Rup_trip * std::exp(-h_P * C * L_He_2St / (k_B * Tmat))) *
-                              CfHe_t / (Hz * (1 + z));
+                           (1.0 - x_He) * 3 * Rup_trip * exp(-h_P * C * L_He_2St / (k_B * Tmat))) *
+                              CfHe_t / (Hz * (1.0 + z));
             }
         }
         if (timeTh < H_frac * timeH) {
-            double epsilon = Hz * (1 + x + fHe) / (CT * std::pow(Trad, 3) * x);
-            f3 = Tnow + epsilon * ((1 + fHe) / (1 + fHe + x)) * ((f1 + fHe * f2) / x) -
-                 epsilon * dHdz / Hz + 3 * epsilon / (1 + z);
+            R epsilon = Hz * (1.0 + x + fHe) / (CT * pow(Trad, 3) * x);
+            f3 = Tnow + epsilon * ((1.0 + fHe) / (1.0 + fHe + x)) * ((f1 + fHe * f2) / x) -
+                 epsilon * dHdz / Hz + 3 * epsilon / (1.0 + z);
         } else {
-            f3 = CT * std::pow(Trad, 4) * x / (1 + x + fHe) * (Tmat - Trad) / (Hz * (1 + z)) + 2 * Tmat / (1 + z);
+            f3 = CT * pow(Trad, 4) * x / (1.0 + x + fHe) * (Tmat - Trad) / (Hz * (1.0 + z)) + 2 * Tmat / (1.0 + z);
         }
         f[0] = f1; f[1] = f2; f[2] = f3;
     }
 
-    double saha_rhs(double z, double CB) const {
-        return std::exp(1.5 * std::log(CR * Tnow / (1 + z)) - CB / (Tnow * (1 + z))) / Nnow;
+    R saha_rhs(R z, double CB) const {
+        return exp(1.5 * log(CR * Tnow / (1.0 + z)) - CB / (Tnow * (1.0 + z))) / Nnow;
     }
     // He Saha total x (recfast.jl:363-366)
-    double he_saha_x0(double z) const {
-        double rhs = saha_rhs(z, CB1_He1) * 4;
-        return 0.5 * (std::sqrt((rhs - 1) * (rhs - 1) + 4 * (1 + fHe) * rhs) - (rhs - 1));
+    R he_saha_x0(R z) const {
+        R rhs = saha_rhs(z, CB1_He1) * 4;
+        return 0.5 * (sqrt((rhs - 1.0) * (rhs - 1.0) + 4 * (1.0 + fHe) * rhs) - (rhs - 1.0));
     }
-    double x_H0_saha(double z) const {  // recfast.jl:402-406
-        double rhs = saha_rhs(z, CB1);
-        return 0.5 * (std::sqrt(rhs * rhs + 4 * rhs) - rhs);
+    R x_H0_saha(R z) const {  // recfast.jl:402-406
+        R rhs = saha_rhs(z, CB1);
+        return 0.5 * (sqrt(rhs * rhs + 4 * rhs) - rhs);
     }
-    double end_of_saha(double z) const { return (he_saha_x0(z) - 1) / fHe - 0.99; }  // :373-380
-    double end_He_evo(double z) const { return x_H0_saha(z) - 0.985; }             // :408
+    R end_of_saha(R z) const { return (he_saha_x0(z) - 1.0) / fHe - 0.99; }  // :373-380
+    R end_He_evo(R z) const { return x_H0_saha(z) - 0.985; }               // :408
 
     // state of recfastsolve (recfast.jl:446-475) sampled on requested redshifts
-    double z_He_evo_start = 0, z_H_He_evo_start = 0;
+    R z_He_evo_start = 0.0, z_H_He_evo_start = 0.0;
 
-    static double bisect(const Recfast& r, double (Recfast::*f)(double) const, double lo, double hi) {
-        double flo = (r.*f)(lo);
+    // Root of g by bisection on values.  A Dual root carries the implicit
+    // function's tangent -g_p / g_z (g_p the Dual part of g at the root, g_z a
+    // central difference in z), which is what differentiating Bolt's bracketing
+    // solve (recfast.jl:451-459) converges to.
+    static R bisect(const RecfastT& r, R (RecfastT::*f)(R) const, double lo, double hi) {
+        double flo = val((r.*f)(R(lo)));
         for (int it = 0; it < 200; ++it) {
             double mid = 0.5 * (lo + hi);
-            double fm = (r.*f)(mid);
+            double fm = val((r.*f)(R(mid)));
             if ((fm > 0) == (flo > 0)) { lo = mid; flo = fm; } else { hi = mid; }
             if (std::fabs(hi - lo) <= 1e-13 * std::max(1.0, std::fabs(mid))) break;
         }
-        return 0.5 * (lo + hi);
+        const double z0 = 0.5 * (lo + hi);
+        if constexpr (std::is_same<R, Dual>::value) {
+            const double dz = 1e-6 * std::max(1.0, std::fabs(z0));
+            const double gz = (val((r.*f)(R(z0 + dz))) - val((r.*f)(R(z0 - dz)))) / (2.0 * dz);
+            return Dual(z0, -tangent((r.*f)(R(z0))) / gz);
+        } else {
+            return z0;
+        }
     }
 
     // Evaluate X_e (pre-reionization, Xe_RECFAST) and Tmat (Tmat_RECFAST) on a
@@ -475,59 +539,62 @@ struct Recfast {
     // with the H+He+Tmat system from z_reio (which must be one of zs) down:
     // reio_dtm(z, X_e, T) is its derivative given the RECFAST X_e at z, and
     // tm_reio[i] receives it at zs[i] <= z_reio (entries above are unset).
-    void solve(const std::vector<double>& zs, std::vector<double>& xe, std::vector<double>& tm,
-               const std::function<double(double, double, double)>* reio_dtm = nullptr, double z_reio = 0.0,
-               std::vector<double>* tm_reio = nullptr) {
-        z_He_evo_start = bisect(*this, &Recfast::end_of_saha, 3500.0, 0.0);
-        z_H_He_evo_start = bisect(*this, &Recfast::end_He_evo, z_He_evo_start, 0.0);
+    void solve(const std::vector<double>& zs, std::vector<R>& xe, std::vector<R>& tm,
+               const std::function<R(R, R, R)>* reio_dtm = nullptr, double z_reio = 0.0,
+               std::vector<R>* tm_reio = nullptr) {
+        z_He_evo_start = bisect(*this, &RecfastT::end_of_saha, 3500.0, 0.0);
+        z_H_He_evo_start = bisect(*this, &RecfastT::end_He_evo, val(z_He_evo_start), 0.0);
+        const auto desc = [](const R& a, const R& b) { return val(a) > val(b); };
+        const auto same = [](const R& a, const R& b) { return val(a) == val(b); };
+        const auto above = [](const R& a, double b) { return val(a) > b; };
         // outputs per stage, descending z
-        std::vector<double> zHe, zHH;
+        std::vector<R> zHe, zHH;
         for (double z : zs) {
-            if (z <= z_He_evo_start && z > z_H_He_evo_start) zHe.push_back(z);
-            else if (z <= z_H_He_evo_start) zHH.push_back(z);
+            if (z <= val(z_He_evo_start) && z > val(z_H_He_evo_start)) zHe.push_back(z);
+            else if (z <= val(z_H_He_evo_start)) zHH.push_back(z);
         }
-        std::sort(zHe.begin(), zHe.end(), std::greater<double>());
-        std::sort(zHH.begin(), zHH.end(), std::greater<double>());
-        zHe.erase(std::unique(zHe.begin(), zHe.end()), zHe.end());
-        zHH.erase(std::unique(zHH.begin(), zHH.end()), zHH.end());
+        std::sort(zHe.begin(), zHe.end(), desc);
+        std::sort(zHH.begin(), zHH.end(), desc);
+        zHe.erase(std::unique(zHe.begin(), zHe.end(), same), zHe.end());
+        zHH.erase(std::unique(zHH.begin(), zHH.end(), same), zHH.end());
         zHe.push_back(z_H_He_evo_start);
         // stage 2: He + Tmat with H in Saha (recfast.jl:461-465, 383-399)
-        double y2[2];
+        R y2[2];
         {
-            double x0 = he_saha_x0(z_He_evo_start);
-            y2[0] = (x0 - 1) / fHe;
-            y2[1] = Tnow * (1 + z_He_evo_start);
+            R x0 = he_saha_x0(z_He_evo_start);
+            y2[0] = (x0 - 1.0) / fHe;
+            y2[1] = Tnow * (1.0 + z_He_evo_start);
         }
-        auto fHe_stage = [this](double z, const double* y, double* dy) {
-            double yy[3] = {x_H0_saha(z), y[0], y[1]}, f[3];
+        auto fHe_stage = [this](R z, const R* y, R* dy) {
+            R yy[3] = {x_H0_saha(z), y[0], y[1]}, f[3];
             rates(yy, z, f);
             dy[0] = f[1];
             dy[1] = f[2];
         };
-        std::vector<std::array<double, 2>> oHe;
-        dopri_outputs<2>(fHe_stage, z_He_evo_start, y2, zHe, oHe, 1e-11, 1e-14);
+        std::vector<std::array<R, 2>> oHe;
+        dopri_outputs<R, 2>(fHe_stage, z_He_evo_start, y2, zHe, oHe, 1e-11, 1e-14);
         // stage 3: H + He + Tmat (recfast.jl:467-471)
-        double y3[3] = {x_H0_saha(z_H_He_evo_start), oHe.back()[0], oHe.back()[1]};
-        auto fHH = [this](double z, const double* y, double* dy) { rates(y, z, dy); };
-        std::vector<std::array<double, 3>> oHH;
-        std::vector<double> oReio;
+        R y3[3] = {x_H0_saha(z_H_He_evo_start), oHe.back()[0], oHe.back()[1]};
+        auto fHH = [this](R z, const R* y, R* dy) { rates(y, z, dy); };
+        std::vector<std::array<R, 3>> oHH;
+        std::vector<R> oReio;
         if (!reio_dtm) {
-            dopri_outputs<3>(fHH, z_H_He_evo_start, y3, zHH, oHH, 1e-11, 1e-14);
+            dopri_outputs<R, 3>(fHH, z_H_He_evo_start, y3, zHH, oHH, 1e-11, 1e-14);
         } else {
             // down to z_reio with three components, then four
-            std::vector<double> zhi, zlo;
-            for (double z : zHH) (z > z_reio ? zhi : zlo).push_back(z);
-            zhi.push_back(z_reio);
-            dopri_outputs<3>(fHH, z_H_He_evo_start, y3, zhi, oHH, 1e-11, 1e-14);
+            std::vector<R> zhi, zlo;
+            for (const R& z : zHH) (val(z) > z_reio ? zhi : zlo).push_back(z);
+            zhi.push_back(R(z_reio));
+            dopri_outputs<R, 3>(fHH, z_H_He_evo_start, y3, zhi, oHH, 1e-11, 1e-14);
             zhi.pop_back();
             oHH.pop_back();
-            double y4[4] = {y3[0], y3[1], y3[2], y3[2]};
-            auto f4 = [this, reio_dtm](double z, const double* y, double* dy) {
+            R y4[4] = {y3[0], y3[1], y3[2], y3[2]};
+            auto f4 = [this, reio_dtm](R z, const R* y, R* dy) {
                 rates(y, z, dy);
                 dy[3] = (*reio_dtm)(z, y[0] + fHe * y[1], y[3]);
             };
-            std::vector<std::array<double, 4>> o4;
-            dopri_outputs<4>(f4, z_reio, y4, zlo, o4, 1e-11, 1e-14);
+            std::vector<std::array<R, 4>> o4;
+            dopri_outputs<R, 4>(f4, R(z_reio), y4, zlo, o4, 1e-11, 1e-14);
             for (const auto& o : o4) {
                 oHH.push_back({o[0], o[1], o[2]});
                 oReio.push_back(o[3]);
@@ -537,15 +604,15 @@ struct Recfast {
         tm.resize(zs.size());
         for (size_t i = 0; i < zs.size(); ++i) {
             double z = zs[i];
-            if (z > z_He_evo_start) {
+            if (z > val(z_He_evo_start)) {
                 xe[i] = xe_saha_early(z);
-                tm[i] = Tnow * (1 + z);
-            } else if (z > z_H_He_evo_start) {
-                size_t j = std::lower_bound(zHe.begin(), zHe.end() - 1, z, std::greater<double>()) - zHe.begin();
+                tm[i] = Tnow * (1.0 + z);
+            } else if (z > val(z_H_He_evo_start)) {
+                size_t j = std::lower_bound(zHe.begin(), zHe.end() - 1, z, above) - zHe.begin();
                 xe[i] = x_H0_saha(z) + fHe * oHe[j][0];  // Xe_He_evolution (recfast.jl:313-318)
                 tm[i] = oHe[j][1];
             } else {
-                size_t j = std::lower_bound(zHH.begin(), zHH.end(), z, std::greater<double>()) - zHH.begin();
+                size_t j = std::lower_bound(zHH.begin(), zHH.end(), z, above) - zHH.begin();
                 xe[i] = oHH[j][0] + fHe * oHH[j][1];
                 tm[i] = oHH[j][2];
                 if (tm_reio && reio_dtm && z <= z_reio) {
@@ -557,57 +624,63 @@ struct Recfast {
     }
 
     // Xe_RECFAST branches above z_He_evo_start (recfast.jl:415-427)
-    double xe_saha_early(double z) const {
-        if (z > 8000.) return 1 + 2 * fHe;
+    R xe_saha_early(R z) const {
+        if (z > 8000.) return 1.0 + 2 * fHe;
         if (z > 5000.) {
-            double rhs = saha_rhs(z, CB1_He2);
-            return 0.5 * (std::sqrt((rhs - 1 - fHe) * (rhs - 1 - fHe) + 4 * (1 + 2 * fHe) * rhs) - (rhs - 1 - fHe));
+            R rhs = saha_rhs(z, CB1_He2);
+            return 0.5 * (sqrt((rhs - 1.0 - fHe) * (rhs - 1.0 - fHe) + 4 * (1.0 + 2 * fHe) * rhs) - (rhs - 1.0 - fHe));
         }
-        if (z > 3500.) return 1 + fHe;
+        if (z > 3500.) return 1.0 + fHe;
         return he_saha_x0(z);
     }
 };
 
 // ----------------------------------------------------------------------------
-Background::Background(const bolt_cosmo_params& p, double x0_, double x1_, int nx_, int nq)
+template <class R>
+BackgroundT<R>::BackgroundT(const ParamsT<R>& p, double x0_, double x1_, int nx_, int nq)
     : cosmo(p, nq), nx(nx_), x0(x0_), x1(x1_) {
     dx = (x1 - x0) / (nx - 1);
     H0 = cosmo.H0;
     rho_crit = cosmo.rho_crit;
     Omega_L = cosmo.Omega_L;
-    coef.assign((size_t)BOLT_NTAB * (nx + 2), 0.0);
-    std::vector<double> xg(nx), y(nx), d(nx);
+    coef.assign((size_t)BOLT_NTAB * (nx + 2), R(0.0));
+    std::vector<double> xg(nx);
+    std::vector<R> y(nx);
     for (int j = 0; j < nx; ++j) xg[j] = node(j);
     // ρ₀ℳ, ℋ, η on the grid (background.jl:106-108)
-    for (int j = 0; j < nx; ++j) y[j] = cosmo.rho0_massive(std::exp(xg[j]));
-    spline_prefilter(y.data(), nx, tab(BOLT_T_RHO0M));
-    for (int j = 0; j < nx; ++j) y[j] = cosmo.Hc_a(std::exp(xg[j]));
-    spline_prefilter(y.data(), nx, tab(BOLT_T_HC));
-    for (int j = 0; j < nx; ++j) y[j] = cosmo.eta(xg[j]);
-    spline_prefilter(y.data(), nx, tab(BOLT_T_ETA));
+    for (int j = 0; j < nx; ++j) y[j] = cosmo.rho0_massive(R(std::exp(xg[j])));
+    prefilter_t(y.data(), nx, tab(BOLT_T_RHO0M));
+    for (int j = 0; j < nx; ++j) y[j] = cosmo.Hc_a(R(std::exp(xg[j])));
+    prefilter_t(y.data(), nx, tab(BOLT_T_HC));
+    for (int j = 0; j < nx; ++j) y[j] = cosmo.eta(R(xg[j]));
+    prefilter_t(y.data(), nx, tab(BOLT_T_ETA));
     derive(BOLT_T_HC, BOLT_T_HC_P, BOLT_T_HC_PP);
     derive(BOLT_T_ETA, BOLT_T_ETA_P, BOLT_T_ETA_PP);
-    eta0 = cosmo.eta(0.0);  // background.jl:112
+    eta0 = cosmo.eta(R(0.0));  // background.jl:112
 }
 
-double Background::node(int j) const { return (x0 * (nx - 1 - j) + x1 * j) / (nx - 1); }
+template <class R>
+double BackgroundT<R>::node(int j) const { return (x0 * (nx - 1 - j) + x1 * j) / (nx - 1); }
 
-void Background::derive(int src, int d1, int d2) {
-    std::vector<double> d(nx);
+template <class R>
+void BackgroundT<R>::derive(int src, int d1, int d2) {
+    std::vector<R> d(nx);
     node_derivatives(tab(src), nx, dx, 1, d.data());
-    spline_prefilter(d.data(), nx, tab(d1));
+    prefilter_t(d.data(), nx, tab(d1));
     node_derivatives(tab(src), nx, dx, 2, d.data());
-    spline_prefilter(d.data(), nx, tab(d2));
+    prefilter_t(d.data(), nx, tab(d2));
 }
 
-double Background::eval(int t, double x, int deriv) const {
-    return spline_eval(coef.data() + (size_t)t * (nx + 2), nx, x0, dx, x, deriv);
+template <class R>
+R BackgroundT<R>::eval(int t, R x, int deriv) const {
+    return eval_t(coef.data() + (size_t)t * (nx + 2), nx, x0, dx, x, deriv);
 }
 
 // IonizationHistory(𝕣::RECFAST, par, bg) (recfast.jl:674-726)
-void Background::ionization(double Tnow_override) {
-    const bolt_cosmo_params& par = cosmo.par;
-    Recfast rf(this, par.Y_p, par.Omega_b, par.Omega_r, Tnow_override);
+template <class R>
+void BackgroundT<R>::ionization(double Tnow_override) {
+    const ParamsT<R>& par = cosmo.par;
+    RecfastT<R> rf(this, par.Y_p, par.Omega_b, par.Omega_r, Tnow_override);
     const NaturalUnits& u = natural_units();
     const double zinitial = 10000.0;
     const double x_init = std::log(1.0 / (zinitial + 1.0));  // z2x(zinitial)
@@ -620,92 +693,99 @@ void Background::ionization(double Tnow_override) {
         if (!(xg[j] < x_init)) zs.push_back(1.0 / std::exp(xg[j]) - 1.0);
     zs.push_back(zre_ini);
     // reionization (recfast.jl:478-536)
-    double X_fin = 1 + rf.Yp / (rf.not4 * (1 - rf.Yp));
+    R X_fin = 1.0 + rf.Yp / (rf.not4 * (1.0 - rf.Yp));
     const double zre = 7.6711, alpha = 1.5, dH = 0.5, zHe = 3.5, dHe = 0.5;
-    double fHe_r = X_fin - 1;
-    auto reio_xe = [&](double z, double x_orig) {
-        double xH = (X_fin - x_orig) / 2 *
-                        (1 + std::tanh((std::pow(1 + zre, alpha) - std::pow(1 + z, alpha)) /
-                                       (alpha * std::pow(1 + zre, alpha - 1)) / dH)) + x_orig;
-        double xHe = fHe_r / 2 * (1 + std::tanh((zHe - z) / dHe));
+    R fHe_r = X_fin - 1.0;
+    auto reio_xe = [&](R z, R x_orig) -> R {
+        R xH = (X_fin - x_orig) / 2 *
+                   (1.0 + tanh((std::pow(1 + zre, alpha) - pow(1.0 + z, alpha)) /
+                               (alpha * std::pow(1 + zre, alpha - 1)) / dH)) + x_orig;
+        R xHe = fHe_r / 2 * (1.0 + tanh((zHe - z) / dHe));
         return xH + xHe;
     };
     // the reionized matter temperature from z = 50 (recfast.jl:513-534), its
     // X_e taken from the RECFAST state integrated alongside
-    std::function<double(double, double, double)> dtm = [&](double z, double xe_orig, double T) {
-        double xr = reio_xe(z, xe_orig);
-        double aa = 1 / (1 + z);
-        double Hz = rf.Hc(std::log(aa)) / aa / u.H0_conv;
-        double Trad = rf.Tnow * (1 + z);
-        return rf.CT * std::pow(Trad, 4) * xr / (1 + xr + rf.fHe) * (T - Trad) / (Hz * (1 + z)) + 2 * T / (1 + z);
+    std::function<R(R, R, R)> dtm = [&](R z, R xe_orig, R T) -> R {
+        R xr = reio_xe(z, xe_orig);
+        R aa = 1.0 / (1.0 + z);
+        R Hz = rf.Hc(log(aa)) / aa / u.H0_conv;
+        R Trad = rf.Tnow * (1.0 + z);
+        return rf.CT * pow(Trad, 4) * xr / (1.0 + xr + rf.fHe) * (T - Trad) / (Hz * (1.0 + z)) + 2 * T / (1.0 + z);
     };
-    std::vector<double> xe_r, tm_r, tmre_r;
+    std::vector<R> xe_r, tm_r, tmre_r;
     rf.solve(zs, xe_r, tm_r, &dtm, zre_ini, &tmre_r);
-    std::vector<double> zlate, tm_reio;
+    std::vector<double> zlate;
+    std::vector<R> tm_reio;
     for (size_t i = 0; i + 1 < zs.size(); ++i)
         if (zs[i] <= zre_ini) { zlate.push_back(zs[i]); tm_reio.push_back(tmre_r[i]); }
     {   // zlate descending, tm_reio alongside
         std::vector<size_t> o(zlate.size());
         for (size_t i = 0; i < o.size(); ++i) o[i] = i;
         std::sort(o.begin(), o.end(), [&](size_t a, size_t b) { return zlate[a] > zlate[b]; });
-        std::vector<double> z2(o.size()), t2(o.size());
+        std::vector<double> z2(o.size());
+        std::vector<R> t2(o.size());
         for (size_t i = 0; i < o.size(); ++i) { z2[i] = zlate[o[i]]; t2[i] = tm_reio[o[i]]; }
         zlate.swap(z2);
         tm_reio.swap(t2);
     }
     // X_e(x), Tmat(x) on the grid (recfast.jl:692-696)
-    std::vector<double> Xe(nx), Tm(nx);
+    std::vector<R> Xe(nx), Tm(nx);
     size_t ir = 0;
     for (int j = 0; j < nx; ++j) {
         double x = xg[j];
-        double Trad = rf.Tnow * (1 + (1.0 / std::exp(x) - 1.0));
+        R Trad = rf.Tnow * (1 + (1.0 / std::exp(x) - 1.0));
         if (x < x_init) {
-            Xe[j] = 1 + 2 * rf.fHe;  // Xe_RECFAST(rhist, zinitial)
+            Xe[j] = 1.0 + 2 * rf.fHe;  // Xe_RECFAST(rhist, zinitial)
             Tm[j] = Trad;
         } else {
             double z = 1.0 / std::exp(x) - 1.0;
-            double xo = xe_r[ir], to = tm_r[ir];
+            R xo = xe_r[ir], to = tm_r[ir];
             ++ir;
             if (z > zre_ini) {
                 Xe[j] = xo;
                 Tm[j] = to;
             } else {
-                Xe[j] = reio_xe(z, xo);
+                Xe[j] = reio_xe(R(z), xo);
                 size_t k = std::lower_bound(zlate.begin(), zlate.end(), z, std::greater<double>()) - zlate.begin();
                 Tm[j] = tm_reio[k];
             }
         }
     }
     // τ′ on the grid and the reverse cumulative trapezoid (ionization.jl:107-129)
-    std::vector<double> tp(nx), tau(nx), g(nx), y(nx);
+    std::vector<R> tp(nx), tau(nx), g(nx), y(nx);
     for (int j = 0; j < nx; ++j) {
         double a = std::exp(xg[j]);
-        double nH = par.Omega_b * rho_crit / (u.m_H * a * a * a) * (1 - par.Y_p);
-        tp[j] = -Xe[j] * nH * a * u.sigma_T / eval(BOLT_T_HC, xg[j], 0);
+        R nH = par.Omega_b * rho_crit / (u.m_H * a * a * a) * (1.0 - par.Y_p);
+        tp[j] = -Xe[j] * nH * a * u.sigma_T / eval(BOLT_T_HC, R(xg[j]), 0);
     }
     tau[nx - 1] = 0.0;
     for (int j = nx - 2; j >= 0; --j) tau[j] = tau[j + 1] + (xg[j] - xg[j + 1]) * (tp[j] + tp[j + 1]) * 0.5;
-    for (int j = 0; j < nx; ++j) g[j] = -tp[j] * std::exp(-tau[j]);  // g̃_function
-    spline_prefilter(Xe.data(), nx, tab(BOLT_T_XE));
-    spline_prefilter(tau.data(), nx, tab(BOLT_T_TAU));
-    spline_prefilter(g.data(), nx, tab(BOLT_T_G));
-    spline_prefilter(Tm.data(), nx, tab(BOLT_T_TMAT));
+    for (int j = 0; j < nx; ++j) g[j] = -tp[j] * exp(-tau[j]);  // g̃_function
+    prefilter_t(Xe.data(), nx, tab(BOLT_T_XE));
+    prefilter_t(tau.data(), nx, tab(BOLT_T_TAU));
+    prefilter_t(g.data(), nx, tab(BOLT_T_G));
+    prefilter_t(Tm.data(), nx, tab(BOLT_T_TMAT));
     derive(BOLT_T_TAU, BOLT_T_TAU_P, BOLT_T_TAU_PP);
     derive(BOLT_T_G, BOLT_T_G_P, BOLT_T_G_PP);
     // sound speed (recfast.jl:710-712)
-    std::vector<double> dT(nx);
+    std::vector<R> dT(nx);
     node_derivatives(tab(BOLT_T_TMAT), nx, dx, 1, dT.data());
     for (int j = 0; j < nx; ++j) {
-        double pre = std::pow(rf.C, -2) * rf.k_B / rf.m_H * (1 / rf.mu_T + (1 - rf.Yp) * eval(BOLT_T_XE, xg[j], 0));
-        y[j] = pre * (eval(BOLT_T_TMAT, xg[j], 0) - 1.0 / 3.0 * dT[j]);
+        R pre = std::pow(rf.C, -2) * rf.k_B / rf.m_H * (1.0 / rf.mu_T + (1.0 - rf.Yp) * eval(BOLT_T_XE, R(xg[j]), 0));
+        y[j] = pre * (eval(BOLT_T_TMAT, R(xg[j]), 0) - 1.0 / 3.0 * dT[j]);
     }
-    spline_prefilter(y.data(), nx, tab(BOLT_T_CSB2));
+    prefilter_t(y.data(), nx, tab(BOLT_T_CSB2));
     tau0 = tau[nx - 1];  // τ(0.)
 }
 
+template struct CosmoT<double>;
+template struct CosmoT<Dual>;
+template struct BackgroundT<double>;
+template struct BackgroundT<Dual>;
+
 int recfast_xe(const bolt_cosmo_params& par, double Tnow, const double* z, int nz, double* out) {
     Background bg(par, -20.0, 0.0, 2001, 15);
-    Recfast rf(&bg, par.Y_p, par.Omega_b, par.Omega_r, Tnow);
+    RecfastT<double> rf(&bg, par.Y_p, par.Omega_b, par.Omega_r, Tnow);
     std::vector<double> zs(z, z + nz), xe, tm;
     rf.solve(zs, xe, tm);
     for (int i = 0; i < nz; ++i) out[i] = xe[i];

@@ -106,6 +106,20 @@ int bolt_state_length(const bolt_hier_cfg* cfg);
 int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq,
                      double* coef_out, double* scal_out, double* xq_out, double* wq_out);
 
+/* Forward-mode derivative of bolt_host_tables along the parameter direction
+ * dpar: with p(t) = par + t*dpar, dcoef_out = d coef/dt and dscal_out =
+ * d scal/dt at t = 0 (same layouts as coef_out / scal_out).  This is what
+ * ForwardDiff.Dual parameters carry through Background and
+ * IonizationHistory(RECFAST) (src/background.jl, src/ionization/recfast.jl;
+ * exercised by test/runtests.jl:54-80).  coef_out, scal_out, xq_out and wq_out
+ * receive exactly what bolt_host_tables writes.  Branch decisions follow the
+ * values (as under ForwardDiff); the RECFAST switch redshifts carry the
+ * implicit-function tangent of their root conditions; the RECFAST ODE takes
+ * the values' steps. */
+int bolt_host_tables_jvp(const bolt_cosmo_params* par, const bolt_cosmo_params* dpar, double x0, double x1,
+                         int nx, int nq, double* coef_out, double* scal_out, double* dcoef_out,
+                         double* dscal_out, double* xq_out, double* wq_out);
+
 /* RECFAST X_e(z) before reionization, Xe_RECFAST(recfastsolve(RECFAST(bg; Tnow)))
  * (src/ionization/recfast.jl:413-475).  Tnow <= 0 takes the default derived from
  * Omega_r (recfast.jl:108). */
