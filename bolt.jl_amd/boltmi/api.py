@@ -89,7 +89,7 @@ class Background:
     partials is computed too (forward-mode AD, bolt_host_tables_jvp): the
     splines and scalars then evaluate to Duals, as under ForwardDiff."""
 
-    def __init__(self, par: CosmoParams, x_grid=(-20.0, 0.0, 2001), nq=15):
+    def __init__(self, par: CosmoParams, x_grid=(-20.0, 0.0, 2001), nq=15, _tables=None):
         x0, x1, nx = x_grid
         self.par = par
         self.nx, self.nq, self.x0, self.x1 = int(nx), int(nq), float(x0), float(x1)
@@ -98,7 +98,9 @@ class Background:
         self.quad_pts = np.zeros(self.nq)
         self.quad_wts = np.zeros(self.nq)
         self.dcoef = self.dscal = None
-        if par.is_dual():
+        if _tables is not None:  # built by backgrounds()
+            self.coef, self.scal, self.quad_pts, self.quad_wts = _tables
+        elif par.is_dual():
             pv, pt = par.c_structs_dual()
             self.dcoef = np.zeros_like(self.coef)
             self.dscal = np.zeros_like(self.scal)
@@ -155,6 +157,28 @@ class Background:
                                           ctypes.byref(self.c_tables()), ctypes.byref(cid)))
             self._uploads[key] = cid.value
         return self._uploads[key]
+
+
+def backgrounds(pars, x_grid=(-20.0, 0.0, 2001), nq=15, threads=0):
+    """[(par, Background(par), IonizationHistory(RECFAST(bg), par, bg)) for par
+    in pars], the tables built on `threads` host threads (0: all hardware
+    threads) by bolt_host_tables_batch; each bitwise what Background(par) builds.
+    This is the table stage of a many-cosmology batch (source_grid_batch)."""
+    pars = list(pars)
+    x0, x1, nx = float(x_grid[0]), float(x_grid[1]), int(x_grid[2])
+    nc = len(pars)
+    coef = np.zeros((nc, _abi.NTAB, nx + 2))
+    scal = np.zeros((nc, _abi.NSCAL))
+    xq, wq = np.zeros(nq), np.zeros(nq)
+    status = np.zeros(max(nc, 1), dtype=np.int32)
+    cp = (_abi.CosmoParamsC * max(nc, 1))(*[p.c_struct() for p in pars])
+    check(lib().bolt_host_tables_batch(cp, nc, x0, x1, nx, int(nq), int(threads), pd(coef), pd(scal), pd(xq),
+                                       pd(wq), pi(status)))
+    out = []
+    for c, par in enumerate(pars):
+        bg = Background(par, (x0, x1, nx), nq, _tables=(coef[c], scal[c], xq.copy(), wq.copy()))
+        out.append((par, bg, IonizationHistory(RECFAST(bg), par, bg)))
+    return out
 
 
 class RECFAST:

@@ -1,9 +1,13 @@
 // C ABI for the host-side producers and the shared error/version plumbing.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_tables.h"
@@ -54,6 +58,50 @@ int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx,
     } catch (const std::exception& e) {
         return set_error(BOLT_E_NUMERIC, std::string("bolt_host_tables: ") + e.what());
     }
+    return BOLT_OK;
+}
+
+int bolt_host_tables_batch(const bolt_cosmo_params* pars, int ncosmo, double x0, double x1, int nx, int nq,
+                           int nthreads, double* coef_out, double* scal_out, double* xq_out, double* wq_out,
+                           int* status) {
+    if (!pars || ncosmo < 0 || !coef_out || !scal_out || !xq_out || !wq_out || !status)
+        return set_error(BOLT_E_ARG, "bolt_host_tables_batch: NULL argument or negative count");
+    if (nx < 4 || nq < 1 || !(x1 > x0)) return set_error(BOLT_E_ARG, "bolt_host_tables_batch: bad grid");
+    if (ncosmo == 0) return BOLT_OK;
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    nt = std::max(1, std::min({nt, ncosmo, 256}));
+    const size_t tab = (size_t)BOLT_NTAB * (nx + 2);
+    std::atomic<int> next(0);
+    std::mutex mu;
+    std::string first_error;
+    auto work = [&]() {
+        std::vector<double> xq(nq), wq(nq);
+        for (int c = next.fetch_add(1); c < ncosmo; c = next.fetch_add(1)) {
+            // each cosmology is one bolt_host_tables call: the same code path and
+            // the same bits as building it alone
+            const int rc = bolt_host_tables(pars + c, x0, x1, nx, nq, coef_out + tab * c,
+                                            scal_out + (size_t)BOLT_NSCAL * c, xq.data(), wq.data());
+            status[c] = rc;
+            if (rc != BOLT_OK) {
+                std::lock_guard<std::mutex> g(mu);
+                if (first_error.empty())
+                    first_error = "bolt_host_tables_batch: cosmology " + std::to_string(c) + ": " + bolt::g_last_error;
+            } else if (c == 0) {
+                std::memcpy(xq_out, xq.data(), sizeof(double) * nq);
+                std::memcpy(wq_out, wq.data(), sizeof(double) * nq);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    } catch (const std::exception&) {
+        // fewer threads than asked for: the ones started and this one finish the queue
+    }
+    work();
+    for (auto& th : pool) th.join();
+    if (status[0] != BOLT_OK) bolt::gauss_legendre(nq, xq_out, wq_out);
+    if (!first_error.empty()) return set_error(BOLT_E_NUMERIC, first_error);
     return BOLT_OK;
 }
 

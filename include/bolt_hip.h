@@ -106,6 +106,17 @@ int bolt_state_length(const bolt_hier_cfg* cfg);
 int bolt_host_tables(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq,
                      double* coef_out, double* scal_out, double* xq_out, double* wq_out);
 
+/* bolt_host_tables for ncosmo cosmologies on nthreads host threads (<= 0: one
+ * per hardware thread), the (cosmology, k) batch's table stage.  Bolt builds
+ * Background + IonizationHistory per cosmology on one thread.  Cosmology c's
+ * tables go to coef_out + c*BOLT_NTAB*(nx+2) and scal_out + c*BOLT_NSCAL, bitwise
+ * what bolt_host_tables writes for it; xq_out, wq_out: [nq], shared.  status[c]
+ * is that call's return code; the function returns BOLT_E_NUMERIC (message of
+ * the first failure) if any cosmology failed, after filling all the others. */
+int bolt_host_tables_batch(const bolt_cosmo_params* pars, int ncosmo, double x0, double x1, int nx, int nq,
+                           int nthreads, double* coef_out, double* scal_out, double* xq_out, double* wq_out,
+                           int* status);
+
 /* Forward-mode derivative of bolt_host_tables along the parameter direction
  * dpar: with p(t) = par + t*dpar, dcoef_out = d coef/dt and dscal_out =
  * d scal/dt at t = 0 (same layouts as coef_out / scal_out).  This is what

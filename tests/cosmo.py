@@ -25,6 +25,16 @@ def background(kind):
 
 
 @functools.lru_cache(maxsize=None)
+def massive_nq(nq):
+    """the one-massive-neutrino cosmology with nq momentum bins: tables, ionization
+    history and the oracle on those tables"""
+    par = boltmi.CosmoParams(h=0.67, Ω_b=0.049, Ω_c=0.26, n=0.965, A=2.1e-9, Σm_ν=0.2 * boltmi.MASS_NATURAL)
+    bg = boltmi.Background(par, nq=nq)
+    ih = boltmi.IonizationHistory(boltmi.RECFAST(bg), par, bg)
+    return par, bg, ih, pyoracle.Oracle(par.as_tuple(), bg.tables_dict())
+
+
+@functools.lru_cache(maxsize=None)
 def oracle(kind):
     par, bg, ih = background(kind)
     return pyoracle.Oracle(par.as_tuple(), bg.tables_dict())

@@ -208,6 +208,37 @@ def test_plin_matches_oracle(k):
     assert abs(pk / ref - 1) < 1e-6
 
 
+# hierarchy sizes across the precision-sweep range n = 128..317 (explicit
+# (ℓᵧ, ℓ_ν, ℓ_mν, nq) tuples), with nq on both sides of the 16-lane q-bin passes
+SWEEP = [(20, 20, 5, 10), (12, 12, 8, 20), (30, 25, 6, 32), (16, 16, 4, 40)]
+
+
+@pytest.mark.parametrize("sizes", SWEEP)
+def test_hierarchy_size_sweep_matches_oracle(sizes):
+    ℓg, ℓn, ℓm, nq = sizes
+    par, bg, ih, o = cosmo.massive_nq(nq)
+    cfg = boltmi.hier_cfg(ℓg, ℓn, ℓm, nq)
+    oc = cosmo.ocfg(ℓg, ℓn, ℓm, nq)
+    n = boltmi.state_length(cfg)
+    ks = np.array([1e-3, 0.05, 0.3])
+    for x in [-15.0, -6.9, -1.0]:
+        U = rand_state(n, len(ks), 11)
+        dU, _ = boltmi.rhs_batch(par, bg, cfg, ks, x, U)
+        for b, k in enumerate(ks):
+            ref, _ = o.rhs(oc, k, x, U[:, b])
+            assert scaled_err(dU[:, b], ref) < 1e-12, (x, k)
+    xs = np.linspace(-20, 0, 40)
+    out, status, st = boltmi.boltsolve_batch(par, bg, boltmi.hier_cfg(ℓg, ℓn, ℓm, nq, 1e-6, 1e-6), [0.05], xs)
+    ref, ost = o.boltsolve(cosmo.ocfg(ℓg, ℓn, ℓm, nq, 1e-6, 1e-6), 0.05, xs)
+    assert status[0] == 0
+    assert st["accepted"] == ost.accepted and st["rejected"] == ost.rejected
+    assert scaled_err(out[0], ref) < 1e-6
+    for k in (0.01, 0.2):
+        pk = boltmi.plin(k, par, bg, ih, n_q=nq, ℓᵧ=ℓg, ℓ_ν=ℓn, ℓ_mν=ℓm)
+        pref, _ = o.plin(cosmo.ocfg(ℓg, ℓn, ℓm, nq, 1e-5, 1e-6), k)
+        assert abs(pk / pref - 1) < 1e-6, k
+
+
 def test_boltsolve_rsa_vs_class_fixture(golden):
     """The reference's own CLASS comparison (runtests.jl:83-147) on the GPU path."""
     from scipy.interpolate import CubicSpline

@@ -124,3 +124,48 @@ def test_ionization_history_shape():
     # the trapezoid discretisation of τ)
     g = ih.g(xg)
     assert np.trapz(g, xg) == pytest.approx(1.0, rel=2e-3)
+
+
+def _box(n, seed):
+    """cosmologies drawn from the batch workload's parameter box"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        h = rng.uniform(0.6, 0.8)
+        out.append(boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h ** 2, Ω_c=rng.uniform(0.10, 0.14) / h ** 2,
+                                      n=rng.uniform(0.92, 1.0), A=np.exp(rng.uniform(2.9, 3.2)) / 1e10,
+                                      Σm_ν=(rng.uniform(0.06, 0.3) if i % 2 else 0.0) * boltmi.MASS_NATURAL))
+    return out
+
+
+def test_host_tables_batch_is_bitwise_the_single_builds():
+    pars = _box(6, 3)
+    grid = (-20.0, 0.0, 501)
+    built = boltmi.backgrounds(pars, x_grid=grid, nq=15, threads=3)
+    for par, bg, ih in built:
+        one = boltmi.Background(par, x_grid=grid, nq=15)
+        assert np.array_equal(bg.coef, one.coef) and np.array_equal(bg.scal, one.scal)
+        assert np.array_equal(bg.quad_pts, one.quad_pts) and np.array_equal(bg.quad_wts, one.quad_wts)
+        assert ih.tau0 == one.scal[_abi.S_TAU0] and bg.eta(-3.0) == one.eta(-3.0)
+    assert boltmi.backgrounds([], x_grid=grid) == []
+
+
+def test_host_tables_batch_reports_each_failure_and_fills_the_rest():
+    pars = _box(4, 5)
+    pars[2] = boltmi.CosmoParams(h=-0.7)  # rejected by bolt_host_tables
+    nx, nq = 201, 15
+    cp = (_abi.CosmoParamsC * 4)(*[p.c_struct() for p in pars])
+    coef = np.zeros((4, _abi.NTAB, nx + 2))
+    scal = np.zeros((4, _abi.NSCAL))
+    xq, wq = np.zeros(nq), np.zeros(nq)
+    status = np.zeros(4, dtype=np.int32)
+    L = _abi.lib()
+    rc = L.bolt_host_tables_batch(cp, 4, -20.0, 0.0, nx, nq, 2, _abi.pd(coef), _abi.pd(scal), _abi.pd(xq),
+                                  _abi.pd(wq), _abi.pi(status))
+    assert rc == _abi.BOLT_E_NUMERIC and b"cosmology 2" in L.bolt_last_error()
+    assert list(status) == [0, 0, _abi.BOLT_E_ARG, 0]
+    for c in (0, 1, 3):
+        one = boltmi.Background(pars[c], x_grid=(-20.0, 0.0, nx), nq=nq)
+        assert np.array_equal(coef[c], one.coef)
+    with pytest.raises(_abi.BoltError):
+        boltmi.backgrounds(pars, x_grid=(-20.0, 0.0, nx))
