@@ -7,6 +7,13 @@ cltt for ℓ = 2..2500 on quadratic_k(0.01H₀, 1000H₀, 5000), flat ΛCDM with
 h = 0.7, Ω_b = 0.046, Ω_c = 0.224, n_s = 0.96, A_s = 2.1e-9, N_eff = 3.046,
 massless ν.  Tables, k grids and the source grid stay resident in HBM.
 
+Steps are independent spectra.  With --inflight N (default 3) they run on N
+contexts, each with its own HIP stream, result buffers and host thread, so a
+spectrum's solve can use the CUs that the previous one's longest modes leave
+idle; value counts completed spectra per second of wall time, and latency_ms
+gives the time of one spectrum as its caller sees it.  --inflight 1 runs them
+one after another.
+
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
 import argparse
@@ -172,8 +179,10 @@ def serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--inflight", type=int, default=3,
+                    help="independent spectra in flight, each on its own context (HIP stream) and host thread")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -201,18 +210,27 @@ def main():
     from boltmi import _abi
 
     par, bg, ih, ks, cfg, ells, dense = workload()
-    ctx = boltmi.default_context()
-    cid = bg.cosmo_id(ctx)
+    nlanes = max(1, args.inflight)
     L = _abi.lib()
     dev = torch.device("cuda:0")
     d_k = torch.tensor(ks, dtype=torch.float64, device=dev)
-    d_S = torch.empty((NK, bg.nx), dtype=torch.float64, device=dev)
-    d_status = torch.zeros(NK, dtype=torch.int32, device=dev)
-    d_ctr = torch.zeros((NK, 4), dtype=torch.int64, device=dev)
     d_dense = torch.tensor(dense, dtype=torch.float64, device=dev)
-    d_cl = torch.empty(len(ells), dtype=torch.float64, device=dev)
-    d_steps = torch.zeros(NKD - 1, dtype=torch.int64, device=dev)
     ells_c = np.ascontiguousarray(ells)
+
+    class Lane:
+        """one spectrum in flight: a context (its own HIP stream) and its own result buffers"""
+
+        def __init__(self, ctx):
+            self.ctx = ctx
+            self.cid = bg.cosmo_id(ctx)
+            self.d_S = torch.empty((NK, bg.nx), dtype=torch.float64, device=dev)
+            self.d_status = torch.zeros(NK, dtype=torch.int32, device=dev)
+            self.d_ctr = torch.zeros((NK, 4), dtype=torch.int64, device=dev)
+            self.d_cl = torch.empty(len(ells), dtype=torch.float64, device=dev)
+            self.d_steps = torch.zeros(NKD - 1, dtype=torch.int64, device=dev)
+
+    lanes = [Lane(boltmi.default_context())] + [Lane(boltmi.Context(0)) for _ in range(nlanes - 1)]
+    d_status, d_ctr, d_steps = lanes[0].d_status, lanes[0].d_ctr, lanes[0].d_steps
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
@@ -222,30 +240,64 @@ def main():
         del src, dst
     torch.cuda.synchronize()
 
-    def step():
+    def step(ln):
+        """one whole spectrum on lane ln: source_grid, then cltt from that grid"""
         ms1 = ctypes.c_float()
-        _abi.check(L.bolt_source_grid_dev(ctx.handle, cid, ctypes.byref(cfg), d_k.data_ptr(), NK, d_S.data_ptr(),
-                                          d_status.data_ptr(), d_ctr.data_ptr(), ctypes.byref(ms1)))
+        _abi.check(L.bolt_source_grid_dev(ln.ctx.handle, ln.cid, ctypes.byref(cfg), d_k.data_ptr(), NK,
+                                          ln.d_S.data_ptr(), ln.d_status.data_ptr(), ln.d_ctr.data_ptr(),
+                                          ctypes.byref(ms1)))
         ms2 = ctypes.c_float()
-        _abi.check(L.bolt_cltt_dev(ctx.handle, cid, d_S.data_ptr(), d_k.data_ptr(), NK, d_dense.data_ptr(), NKD,
-                                   _abi.pi(ells_c), len(ells_c), d_cl.data_ptr(), d_steps.data_ptr(),
+        _abi.check(L.bolt_cltt_dev(ln.ctx.handle, ln.cid, ln.d_S.data_ptr(), d_k.data_ptr(), NK, d_dense.data_ptr(),
+                                   NKD, _abi.pi(ells_c), len(ells_c), ln.d_cl.data_ptr(), ln.d_steps.data_ptr(),
                                    ctypes.byref(ms2)))
         return ms1.value, ms2.value
 
-    for _ in range(args.warmup):
-        step()
+    def run(counts, label):
+        """counts[j] steps on lane j, the lanes concurrently (the C calls release the GIL)"""
+        import threading
+        res = [[] for _ in lanes]
+        err = []
+        done = [0]
+        lock = threading.Lock()
+
+        def work(j):
+            try:
+                for _ in range(counts[j]):
+                    t = time.perf_counter()
+                    a, b = step(lanes[j])
+                    res[j].append((a, b, (time.perf_counter() - t) * 1e3))
+                    with lock:
+                        done[0] += 1
+                        print(f"# {label} {done[0]}/{sum(counts)} (lane {j}): solver {a:.1f} ms, LOS {b:.2f} ms",
+                              file=sys.stderr, flush=True)
+            except Exception as e:  # reported after the join
+                err.append(e)
+
+        th = [threading.Thread(target=work, args=(j,)) for j in range(len(lanes))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if err:
+            raise err[0]
+        return [r for rr in res for r in rr]
+
+    def split(k):
+        return [k // len(lanes) + (1 if j < k % len(lanes) else 0) for j in range(len(lanes))]
+
+    # every lane runs at least one untimed step (its context's first call sizes its cached buffers)
+    run([max(c, 1) for c in split(args.warmup)], "warmup")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    kms = []
-    for i in range(args.steps):
-        kms.append(step())
-        print(f"# step {i + 1}/{args.steps}: solver {kms[-1][0]:.1f} ms, LOS {kms[-1][1]:.2f} ms", file=sys.stderr,
-              flush=True)
+    timed = run(split(args.steps), "step")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    kms = [(a, b) for a, b, _ in timed]
+    lat = [w for _, _, w in timed]
     dt = (t1 - t0) / args.steps
-    if int(d_status.max().item()) != 0:
-        raise RuntimeError("a k-mode failed in the benchmark run")
+    for ln in lanes:
+        if int(ln.d_status.max().item()) != 0:
+            raise RuntimeError("a k-mode failed in the benchmark run")
     ctr = d_ctr.cpu().numpy()
     steps_total = int(ctr[:, 0].sum() + ctr[:, 1].sum())
     acc_total = int(ctr[:, 0].sum())
@@ -273,8 +325,11 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "full CMB path: source_grid 800 log k in [1e-4,0.5]/Mpc (l_g=l_nu=50, l_mnu=10, "
                                "nq=15, reltol 1e-11) + C_l^TT l=2..2500 on quadratic_k(0.01H0,1000H0,5000)",
-                   "n_kmodes": NK, "state_length": n, "n_ell": len(ells), "n_kdense": NKD},
+                   "n_kmodes": NK, "state_length": n, "n_ell": len(ells), "n_kdense": NKD,
+                   "inflight": len(lanes)},
         "kmode_solves_per_s": NK / dt,
+        "inflight": len(lanes),
+        "latency_ms": {"mean": float(np.mean(lat)), "max": float(np.max(lat))},
         "kernels_ms": {"k_boltsolve": ms_ode, "k_los": ms_los},
         "ode_steps_total": steps_total,
         "los": {"nominal_tflops": los_tf, "executed_tflops": los_tf_exec, "executed_steps": los_exec,

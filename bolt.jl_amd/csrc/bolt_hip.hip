@@ -1967,6 +1967,31 @@ struct CosmoSlot {
 struct Server;
 static void server_stop(Server* sv);
 
+// A device buffer a context keeps between its calls and only ever grows.
+// hipFree waits for the whole device, so a call that allocated and freed its
+// launch descriptors would wait for other contexts' kernels too; calls that
+// fit the cached capacity allocate nothing and their streams can overlap.
+struct GBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        if (p && bytes <= cap) return hipSuccess;
+        const size_t want = std::max(bytes, 2 * cap);
+        release();
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
 struct bolt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1974,6 +1999,8 @@ struct bolt_ctx {
     int lds_max = 64 * 1024;
     std::vector<CosmoSlot> cosmos;
     std::mutex mu;
+    // launch descriptors and LOS work arrays of calls on `stream` (under mu)
+    GBuf c_modes, c_cfgs, c_cosmos, c_counter, c_lidx, c_eta, c_dx, c_contrib;
     Server* server = nullptr;  // request server (bolt_serve_start), if running
     std::mutex server_mu;
 };
@@ -2066,6 +2093,9 @@ int bolt_ctx_destroy(bolt_ctx* ctx) {
         if (s.d_q) (void)hipFree(s.d_q);
         if (s.d_self) (void)hipFree(s.d_self);
     }
+    for (GBuf* g : {&ctx->c_modes, &ctx->c_cfgs, &ctx->c_cosmos, &ctx->c_counter, &ctx->c_lidx, &ctx->c_eta,
+                    &ctx->c_dx, &ctx->c_contrib})
+        g->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BOLT_OK;
@@ -2190,19 +2220,18 @@ static int launch_modes(bolt_ctx* ctx, hipStream_t stream, const SolveBufs* pre,
     size_t lds = 0;
     for (const CfgDev& c : cfgs) lds = std::max(lds, ws_bytes(c.n, c.nq));
     if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
-    DBuf dmodes, dcfg, dcos, dctr;
     SolveBufs b;
     if (pre) {
         if (B > pre->cap_modes || (int)cfgs.size() > pre->cap_cfgs || (int)cosmos.size() > pre->cap_cosmos)
             return set_error(BOLT_E_ARG, "launch exceeds the preallocated solver buffers");
         b = *pre;
-    } else {
-        HIPCHK(dmodes.alloc(sizeof(ModeDesc) * B));
-        HIPCHK(dcfg.alloc(sizeof(CfgDev) * cfgs.size()));
-        HIPCHK(dcos.alloc(sizeof(CosmoDev) * cosmos.size()));
-        HIPCHK(dctr.alloc(sizeof(int)));
-        b.modes = dmodes.as<ModeDesc>(); b.cfgs = dcfg.as<CfgDev>(); b.cosmos = dcos.as<CosmoDev>();
-        b.counter = dctr.as<int>();
+    } else {  // the context's own stream, under its lock: its cached buffers
+        HIPCHK(ctx->c_modes.ensure(sizeof(ModeDesc) * B));
+        HIPCHK(ctx->c_cfgs.ensure(sizeof(CfgDev) * cfgs.size()));
+        HIPCHK(ctx->c_cosmos.ensure(sizeof(CosmoDev) * cosmos.size()));
+        HIPCHK(ctx->c_counter.ensure(sizeof(int)));
+        b.modes = ctx->c_modes.as<ModeDesc>(); b.cfgs = ctx->c_cfgs.as<CfgDev>();
+        b.cosmos = ctx->c_cosmos.as<CosmoDev>(); b.counter = ctx->c_counter.as<int>();
     }
     HIPCHK(hipMemcpyAsync(b.modes, md.data(), sizeof(ModeDesc) * B, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(b.cfgs, cfgs.data(), sizeof(CfgDev) * cfgs.size(), hipMemcpyHostToDevice, stream));
@@ -2666,19 +2695,19 @@ static int cl_core(bolt_ctx* ctx, hipStream_t stream, const LosBufs* pre, const 
     std::vector<int> lidx(lmax + 1, -1);
     for (int e = 0; e < nell; ++e) lidx[ells[e]] = e;
     const int nout = two ? 3 : 1;
-    DBuf dl, deta, ddx, dcon;
     LosBufs lb;
     const size_t ncon = (size_t)nout * nell * (nkd - 1);
     if (pre) {
         if ((size_t)(lmax + 1) > pre->cap_lidx || (size_t)nxi > pre->cap_nxi || ncon > pre->cap_contrib)
             return set_error(BOLT_E_ARG, "C_l request exceeds the preallocated LOS buffers");
         lb = *pre;
-    } else {
-        HIPCHK(dl.alloc(sizeof(int) * (lmax + 1)));
-        HIPCHK(deta.alloc(sizeof(double) * nxi));
-        HIPCHK(ddx.alloc(sizeof(double) * nxi));
-        HIPCHK(dcon.alloc(sizeof(double) * ncon));
-        lb.lidx = dl.as<int>(); lb.etaj = deta.as<double>(); lb.dxj = ddx.as<double>(); lb.contrib = dcon.as<double>();
+    } else {  // the context's own stream, under its lock: its cached buffers
+        HIPCHK(ctx->c_lidx.ensure(sizeof(int) * (lmax + 1)));
+        HIPCHK(ctx->c_eta.ensure(sizeof(double) * nxi));
+        HIPCHK(ctx->c_dx.ensure(sizeof(double) * nxi));
+        HIPCHK(ctx->c_contrib.ensure(sizeof(double) * ncon));
+        lb.lidx = ctx->c_lidx.as<int>(); lb.etaj = ctx->c_eta.as<double>(); lb.dxj = ctx->c_dx.as<double>();
+        lb.contrib = ctx->c_contrib.as<double>();
     }
     HIPCHK(hipMemcpyAsync(lb.lidx, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, stream, d_cosmo, i0, nxi, lb.etaj, lb.dxj);
