@@ -51,3 +51,42 @@ def test_bad_request_is_rejected_before_queueing():
             srv.submit_pk(par, k=[0.1, 0.05])
     finally:
         srv.close()
+
+
+def test_concurrent_contexts_match_one_context():
+    """Calls on different contexts run concurrently (each context has its own
+    stream and cached launch buffers); every result is bitwise what the same
+    call gives alone, on the first call of a context and on later ones."""
+    import threading
+
+    par, bg, ih = cosmo.background("default")
+    ks = np.logspace(-3, np.log10(0.2), 24)
+    cfg = dict(ℓᵧ=8, reltol=1e-8)
+    ref = boltmi.source_grid(par, bg, ih, ks, **cfg).grid
+    ref_pk = boltmi.plin(ks[:6], par, bg, ih, ℓᵧ=12, ℓ_ν=12, ℓ_mν=8)
+    ctxs = [boltmi.Context(0) for _ in range(3)]
+    out = [[] for _ in ctxs]
+    errs = []
+
+    def work(j):
+        try:
+            for _ in range(2):
+                out[j].append(boltmi.source_grid(par, bg, ih, ks, ctx=ctxs[j], **cfg).grid)
+                out[j].append(boltmi.plin(ks[:6], par, bg, ih, ℓᵧ=12, ℓ_ν=12, ℓ_mν=8, ctx=ctxs[j]))
+        except Exception as e:
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(j,)) for j in range(len(ctxs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for c in ctxs:
+        c.close()
+    assert not errs, errs
+    for res in out:
+        assert len(res) == 4
+        for i in (0, 2):
+            assert np.array_equal(res[i], ref)
+        for i in (1, 3):
+            assert np.array_equal(res[i], ref_pk)
