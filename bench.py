@@ -14,6 +14,10 @@ idle; value counts completed spectra per second of wall time, and latency_ms
 gives the time of one spectrum as its caller sees it.  --inflight 1 runs them
 one after another.
 
+--batch measures the many-cosmology workload instead (k-mode solves/s of one
+persistent launch over 64 cosmologies × 800 k), and --serve a Poisson request
+stream; each prints its own JSON line.
+
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
 import argparse
@@ -114,6 +118,76 @@ def cpu_baseline(par, bg, ks, ells, dense, threads):
             "est_source_grid_s": t_sg, "est_cltt_s": t_cl, "sample_cpu_s": sum(t_modes) + sum(t_ks)}
 
 
+def batch(args):
+    """The many-cosmology workload, measured apart from the default line.
+    --ncosmo cosmologies (64) are drawn from the parameter box:
+    h in [0.6, 0.8], ω_b in [0.020, 0.024], ω_c in [0.10, 0.14], n_s in [0.92, 1.0],
+    ln(10¹⁰A_s) in [2.9, 3.2]. Odd ones have one massive ν in [0.06, 0.3] eV
+    with ℓ_mν = 20 (n = 473); even ones are massless with ℓ_mν = 10 (n = 323).
+    Each gets 800 log k in [1e-4, 0.5] Mpc⁻¹ and source_grid at reltol 1e-11:
+    51,200 (cosmology, k) pairs in one persistent launch, sources left in HBM.
+    The host table stage (bolt_host_tables_batch on --cpu-threads threads) is
+    timed separately."""
+    import torch
+    import boltmi
+    from boltmi import _abi
+    rng = np.random.default_rng(args.seed)
+    pars, lm = [], []
+    for i in range(args.ncosmo):
+        h = rng.uniform(0.6, 0.8)
+        massive = i % 2 == 1
+        pars.append(boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h ** 2, Ω_c=rng.uniform(0.10, 0.14) / h ** 2,
+                                       n=rng.uniform(0.92, 1.0), A=math.exp(rng.uniform(2.9, 3.2)) / 1e10,
+                                       Σm_ν=(rng.uniform(0.06, 0.3) if massive else 0.0) * boltmi.MASS_NATURAL))
+        lm.append(20 if massive else 10)
+    thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    cosmos = boltmi.backgrounds(pars, threads=thr)
+    t_tab = time.perf_counter() - t0
+    ctx = boltmi.default_context()
+    L = _abi.lib()
+    nc = len(cosmos)
+    ids = np.array([bg.cosmo_id(ctx) for _, bg, _ in cosmos], dtype=np.int32)
+    cfgs = (_abi.HierCfgC * nc)(*[boltmi.hier_cfg(LG, LN, m, NQ, RELTOL, ABSTOL) for m in lm])
+    ks = np.logspace(-4, math.log10(0.5), NK)
+    dev = torch.device("cuda:0")
+    d_k = torch.tensor(np.tile(ks, (nc, 1)), dtype=torch.float64, device=dev)
+    d_S = torch.empty((nc, NK, cosmos[0][1].nx), dtype=torch.float64, device=dev)
+    d_status = torch.zeros((nc, NK), dtype=torch.int32, device=dev)
+    d_ctr = torch.zeros((nc * NK, 4), dtype=torch.int64, device=dev)
+
+    def run(m):
+        ms = ctypes.c_float()
+        _abi.check(L.bolt_source_grid_batch_dev(ctx.handle, m, _abi.pi(ids), cfgs, d_k.data_ptr(), NK,
+                                                d_S.data_ptr(), None, d_status.data_ptr(), d_ctr.data_ptr(),
+                                                ctypes.byref(ms)))
+        return ms.value
+
+    run(min(2, nc))  # warm-up on the first two cosmologies (one of each hierarchy)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = run(nc)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"# batch: {nc} cosmologies x {NK} k in {dt:.2f} s (kernel {ms:.0f} ms), tables {t_tab:.2f} s on {thr} "
+          f"threads", file=sys.stderr, flush=True)
+    if int(d_status.max().item()) != 0:
+        raise RuntimeError("a k-mode failed in the batch run")
+    ctr = d_ctr.cpu().numpy()
+    line = {
+        "metric": "k-mode solves/s, many-cosmology batch (source_grid, one launch)", "value": nc * NK / dt,
+        "unit": "k-mode solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{nc} cosmologies from the parameter box (half with one massive nu, l_mnu=20, "
+                               f"n=473; half massless, l_mnu=10, n=323) x {NK} log k in [1e-4,0.5]/Mpc, "
+                               f"l_g=l_nu=50, nq=15, reltol 1e-11, one persistent launch",
+                   "n_cosmo": nc, "n_kmodes": nc * NK, "seed": args.seed},
+        "batch_s": dt, "kernel_ms": ms, "source_grids_per_s": nc / dt,
+        "ode_steps_total": int(ctr[:, 0].sum() + ctr[:, 1].sum()),
+        "host_tables": {"seconds": t_tab, "threads": thr, "per_cosmology_ms": t_tab / nc * 1e3 * thr},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def serve(args):
     """Serving workload: seeded Poisson arrivals of spectrum requests, each with
     its own cosmology drawn from the box h ∈ [0.6, 0.8], Ω_b h² ∈ [0.020, 0.024],
@@ -188,6 +262,9 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the workload (reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--batch", action="store_true",
+                    help="measure the many-cosmology batch (source_grid for --ncosmo cosmologies in one launch)")
+    ap.add_argument("--ncosmo", type=int, default=64, help="--batch: number of cosmologies")
     ap.add_argument("--serve", action="store_true",
                     help="measure the serving workload (Poisson request stream) instead of the full CMB path")
     ap.add_argument("--rate", type=float, default=50.0, help="--serve: mean request arrivals per second")
@@ -203,6 +280,9 @@ def main():
 
     if args.serve:
         serve(args)
+        return
+    if args.batch:
+        batch(args)
         return
 
     import torch
