@@ -169,3 +169,21 @@ def test_host_tables_batch_reports_each_failure_and_fills_the_rest():
         assert np.array_equal(coef[c], one.coef)
     with pytest.raises(_abi.BoltError):
         boltmi.backgrounds(pars, x_grid=(-20.0, 0.0, nx))
+
+
+def test_k_grids_quadratic_and_log10():
+    """quadratic_k and log10_k (spectra.jl:60-68) by their defining properties:
+    the quadratic grid's first point lies above kmin (i starts at 1), its last is
+    kmax and its square-root spacing is uniform; the log grid runs from kmin to
+    kmax with a constant ratio."""
+    kmin, kmax, nk = 0.01 * 2.3349e-4, 1000 * 2.3349e-4, 5000
+    q = boltmi.quadratic_k(kmin, kmax, nk)
+    assert len(q) == nk and q[0] > kmin and q[-1] == kmax
+    assert q[0] == pytest.approx(kmin + (kmax - kmin) / nk ** 2, rel=1e-15)
+    s = np.sqrt((q - kmin) / (kmax - kmin))
+    np.testing.assert_allclose(np.diff(s), 1.0 / nk, rtol=1e-9)
+    g = boltmi.log10_k(1e-4, 0.5, 800)
+    assert len(g) == 800 and g[0] == pytest.approx(1e-4, rel=1e-14) and g[-1] == pytest.approx(0.5, rel=1e-14)
+    r = g[1:] / g[:-1]
+    np.testing.assert_allclose(r, r[0], rtol=1e-12)
+    np.testing.assert_allclose(g, np.logspace(-4, np.log10(0.5), 800), rtol=1e-13)
