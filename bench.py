@@ -118,6 +118,39 @@ def cpu_baseline(par, bg, ks, ells, dense, threads):
             "est_source_grid_s": t_sg, "est_cltt_s": t_cl, "sample_cpu_s": sum(t_modes) + sum(t_ks)}
 
 
+def plain_gpu_baseline(lane, L, cfg, d_k, value):
+    """The plain one-mode-per-launch FP64 GPU form of the same spectrum: this
+    build's own FP64 solver launched for one k-mode at a time (no batching
+    across modes), then one LOS pass.  A sample of 40 of the 800 modes, spread
+    over k, is launched one by one and timed by wall clock; the sum is scaled
+    to 800 launches."""
+    from boltmi import _abi
+    import torch
+    idx = list(range(10, NK, NK // 40))[:40]
+    t_modes = 0.0
+    for i in idx:
+        ms = ctypes.c_float()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _abi.check(L.bolt_source_grid_dev(lane.ctx.handle, lane.cid, ctypes.byref(cfg), d_k[i:].data_ptr(), 1,
+                                          lane.d_S[i:].data_ptr(), lane.d_status[i:].data_ptr(),
+                                          lane.d_ctr[i:].data_ptr(), ctypes.byref(ms)))
+        t_modes += time.perf_counter() - t0
+    t_sg = t_modes * NK / len(idx)
+    ms_los = ctypes.c_float()
+    t0 = time.perf_counter()
+    _abi.check(L.bolt_cltt_dev(lane.ctx.handle, lane.cid, lane.d_S.data_ptr(), d_k.data_ptr(), NK,
+                               lane.d_dense.data_ptr(), NKD, _abi.pi(lane.ells), len(lane.ells),
+                               lane.d_cl.data_ptr(), lane.d_steps.data_ptr(), ctypes.byref(ms_los)))
+    t_los = time.perf_counter() - t0
+    plain = 1.0 / (t_sg + t_los)
+    if int(lane.d_status.max().item()) != 0:
+        raise RuntimeError("a k-mode failed in the plain baseline")
+    return {"value": plain, "unit": "spectra/s", "ratio": value / plain,
+            "sample": f"{len(idx)} of {NK} k-modes launched one per launch (wall time, summed and scaled to {NK}), "
+                      f"plus one LOS launch", "est_source_grid_s": t_sg, "los_s": t_los}
+
+
 def batch(args):
     """The many-cosmology workload, measured apart from the default line.
     --ncosmo cosmologies (64) are drawn from the parameter box:
@@ -258,6 +291,7 @@ def main():
     ap.add_argument("--inflight", type=int, default=3,
                     help="independent spectra in flight, each on its own context (HIP stream) and host thread")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-mode-per-launch GPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the workload (reference "
@@ -308,6 +342,8 @@ def main():
             self.d_ctr = torch.zeros((NK, 4), dtype=torch.int64, device=dev)
             self.d_cl = torch.empty(len(ells), dtype=torch.float64, device=dev)
             self.d_steps = torch.zeros(NKD - 1, dtype=torch.int64, device=dev)
+            self.d_dense = d_dense
+            self.ells = ells_c
 
     lanes = [Lane(boltmi.default_context())] + [Lane(boltmi.Context(0)) for _ in range(nlanes - 1)]
     d_status, d_ctr, d_steps = lanes[0].d_status, lanes[0].d_ctr, lanes[0].d_steps
@@ -416,6 +452,8 @@ def main():
                 "nominal_steps": f_los_nominal / 4.0},
         "roofline": roof,
     }
+    if not args.no_plain:
+        line["plain_gpu_baseline"] = plain_gpu_baseline(lanes[0], L, cfg, d_k, value=1.0 / dt)
     if not args.no_cpu:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(par, bg, ks, ells, dense, thr)
