@@ -151,6 +151,75 @@ def plain_gpu_baseline(lane, L, cfg, d_k, value):
                       f"plus one LOS launch", "est_source_grid_s": t_sg, "los_s": t_los}
 
 
+def c1(args):
+    """The plain-method workload, measured apart from the default line.
+    Cosmology: flat ΛCDM (h = 0.7, Ω_b = 0.046, Ω_c = 0.224, n_s = 0.96,
+    A_s = 2.1e-9, N_eff = 3.046, massless ν).
+    Hierarchy: ℓᵧ = ℓ_ν = ℓ_mν = 10, nq = 15 (n = 203).
+    k: 100 log-spaced in [1e-4, 1] h/Mpc, solved at reltol = abstol = 1e-6.
+    Output: states at 1000 x from -20 to 0 (δ_c and Φ among them), then P(k)
+    for the same 100 k (plin, reltol 1e-5).
+    One step is both; value is k-mode solves/s of wall time, states copied to
+    the host included.  Beside it: the oracle on a sample of the modes (its
+    time and the largest scaled difference of δ_c and Φ)."""
+    import torch
+    import boltmi
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    par = boltmi.CosmoParams(h=0.7, Ω_b=0.046, Ω_c=0.224, n=0.96, A=2.1e-9, N_ν=3.046, Σm_ν=0.0)
+    bg = boltmi.Background(par)
+    ih = boltmi.IonizationHistory(boltmi.RECFAST(bg), par, bg)
+    ks = 0.7 * np.logspace(-4, 0, 100)
+    xs = np.linspace(-20.0, 0.0, 1000)
+    cfg = boltmi.hier_cfg(10, 10, 10, NQ, 1e-6, 1e-6)
+    n = boltmi.state_length(cfg)
+
+    def step():
+        out, status, st = boltmi.boltsolve_batch(par, bg, cfg, ks, xs)
+        pk = boltmi.plin(ks, par, bg, ih, n_q=NQ, ℓᵧ=10, ℓ_ν=10, ℓ_mν=10)
+        return out, st, pk
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st, pk = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    line = {"metric": "k-mode solves/s, plain-method workload (solve with 1000 saved states + P(k))",
+            "value": len(ks) / dt, "unit": "k-mode solves/s", "n_gpus": 1, "steps": args.steps,
+            "ms_per_step": dt * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "flat LCDM, l_g=l_nu=l_mnu=10, nq=15 (n=203), 100 log k in [1e-4,1] h/Mpc, "
+                                   "reltol=abstol=1e-6, states at 1000 x, then P(k) at the same k",
+                       "n_kmodes": len(ks), "state_length": n},
+            "solver_kernel_ms": st["kernel_ms"], "steps_accepted": st["accepted"]}
+    if not args.no_cpu:
+        import pyoracle
+        from concurrent.futures import ThreadPoolExecutor
+        o = pyoracle.Oracle(par.as_tuple(), bg.tables_dict())
+        oc = pyoracle.Oracle.cfg(10, 10, 10, NQ, 1e-6, 1e-6, 0, 1)
+        sample = list(range(0, 100, 10))
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+
+        def one(i):
+            t = time.perf_counter()
+            ref, _ = o.boltsolve(oc, float(ks[i]), xs)
+            return time.perf_counter() - t, ref
+
+        with ThreadPoolExecutor(thr) as ex:
+            res = list(ex.map(one, sample))
+        err = 0.0
+        for (_, ref), i in zip(res, sample):
+            for col in (n - 5, n - 4):  # Φ and δ_c
+                err = max(err, float(np.max(np.abs(out[i][:, col] - ref[:, col])) / np.max(np.abs(ref[:, col]))))
+        t_cpu = sum(t for t, _ in res) * len(ks) / len(sample) / thr
+        line["cpu_baseline"] = {"value": len(ks) / t_cpu, "unit": "k-mode solves/s (solve only)", "cores": thr,
+                                "kind": "port", "sample": f"oracle boltsolve on {len(sample)} of {len(ks)} modes, "
+                                f"{thr} threads; per-mode times summed and scaled"}
+        line["max_scaled_err_phi_delta_c"] = err
+    print(json.dumps(line), flush=True)
+
+
 def batch(args):
     """The many-cosmology workload, measured apart from the default line.
     --ncosmo cosmologies (64) are drawn from the parameter box:
@@ -296,6 +365,7 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the workload (reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--c1", action="store_true", help="measure the plain-method workload (100 k, l=10, states + P(k))")
     ap.add_argument("--batch", action="store_true",
                     help="measure the many-cosmology batch (source_grid for --ncosmo cosmologies in one launch)")
     ap.add_argument("--ncosmo", type=int, default=64, help="--batch: number of cosmologies")
@@ -317,6 +387,9 @@ def main():
         return
     if args.batch:
         batch(args)
+        return
+    if args.c1:
+        c1(args)
         return
 
     import torch
