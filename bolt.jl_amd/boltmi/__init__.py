@@ -6,6 +6,6 @@ from .api import (MASS_NATURAL, Background, backgrounds, BasicNewtonian, Context
                   initial_conditions_batch, log10_k, plin, quadratic_k, recfast_xe, rhs_batch, source_grid,
                   source_grid_P, source_grid_batch, source_grids_TP,
                   state_length)
-from .dual import Dual, derivative
+from .dual import PARAM_FIELDS, Dual, derivative, tables_jacobian
 
 __all__ = [n for n in dir() if not n.startswith("_")]

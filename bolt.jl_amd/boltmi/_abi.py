@@ -67,6 +67,7 @@ SIGNATURES = {
     "bolt_state_length": (_I, [ctypes.POINTER(HierCfgC)]),
     "bolt_host_tables": (_I, [ctypes.POINTER(CosmoParamsC), _D, _D, _I, _I, _PD, _PD, _PD, _PD]),
     "bolt_host_tables_batch": (_I, [ctypes.POINTER(CosmoParamsC), _I, _D, _D, _I, _I, _I, _PD, _PD, _PD, _PD, _PI]),
+    "bolt_host_tables_jacobian": (_I, [ctypes.POINTER(CosmoParamsC), _D, _D, _I, _I, _PD, _PD, _PD, _PD, _PD, _PD]),
     "bolt_host_tables_jvp": (_I, [ctypes.POINTER(CosmoParamsC), ctypes.POINTER(CosmoParamsC), _D, _D, _I, _I, _PD, _PD,
                                   _PD, _PD, _PD, _PD]),
     "bolt_host_recfast_xe": (_I, [ctypes.POINTER(CosmoParamsC), _D, _PD, _I, _PD]),

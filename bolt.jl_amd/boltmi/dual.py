@@ -93,3 +93,26 @@ def derivative(f, x):
     """ForwardDiff.derivative(f, x) for functions of one real that build their
     result from host-side quantities (Background, IonizationHistory)."""
     return partial(f(Dual(float(x), 1.0)))
+
+
+PARAM_FIELDS = ("h", "Ω_r", "Ω_b", "Ω_c", "A", "n", "Y_p", "N_ν", "Σm_ν")
+
+
+def tables_jacobian(par, x_grid=(-20.0, 0.0, 2001), nq=15):
+    """The host tables and their derivatives with respect to every CosmoParams
+    field in one pass (ForwardDiff's chunked Duals; bolt_host_tables_jacobian).
+    Returns (Background(par), dcoef [9][NTAB][nx+2], dscal [9][NSCAL]) with the
+    partials in PARAM_FIELDS order."""
+    import ctypes
+    from . import _abi
+    from ._abi import check, lib, pd
+    from .api import Background
+    x0, x1, nx = float(x_grid[0]), float(x_grid[1]), int(x_grid[2])
+    coef = np.zeros((_abi.NTAB, nx + 2))
+    scal = np.zeros(_abi.NSCAL)
+    dcoef = np.zeros((len(PARAM_FIELDS), _abi.NTAB, nx + 2))
+    dscal = np.zeros((len(PARAM_FIELDS), _abi.NSCAL))
+    xq, wq = np.zeros(nq), np.zeros(nq)
+    check(lib().bolt_host_tables_jacobian(ctypes.byref(par.c_struct()), x0, x1, nx, int(nq), pd(coef), pd(scal),
+                                          pd(dcoef), pd(dscal), pd(xq), pd(wq)))
+    return Background(par, (x0, x1, nx), nq, _tables=(coef, scal, xq, wq)), dcoef, dscal

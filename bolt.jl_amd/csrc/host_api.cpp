@@ -124,14 +124,42 @@ int bolt_host_tables_jvp(const bolt_cosmo_params* par, const bolt_cosmo_params* 
         bolt::BackgroundT<bolt::Dual> bg(bolt::ParamsT<bolt::Dual>(*par, *dpar), x0, x1, nx, nq);
         bg.ionization(-1.0);
         for (size_t i = 0; i < bg.coef.size(); ++i) {
-            dcoef_out[i] = bg.coef[i].d;
+            dcoef_out[i] = bg.coef[i].d[0];
             if (!std::isfinite(dcoef_out[i]))
                 return set_error(BOLT_E_NUMERIC, "bolt_host_tables_jvp: non-finite tangent");
         }
         const bolt::Dual s[BOLT_NSCAL] = {bg.H0, bg.eta0, bg.rho_crit, bg.Omega_L, bg.tau0};
-        for (int i = 0; i < BOLT_NSCAL; ++i) dscal_out[i] = s[i].d;
+        for (int i = 0; i < BOLT_NSCAL; ++i) dscal_out[i] = s[i].d[0];
     } catch (const std::exception& e) {
         return set_error(BOLT_E_NUMERIC, std::string("bolt_host_tables_jvp: ") + e.what());
+    }
+    return BOLT_OK;
+}
+
+int bolt_host_tables_jacobian(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq, double* coef_out,
+                              double* scal_out, double* dcoef_out, double* dscal_out, double* xq_out, double* wq_out) {
+    if (!par || !coef_out || !scal_out || !dcoef_out || !dscal_out || !xq_out || !wq_out)
+        return set_error(BOLT_E_ARG, "bolt_host_tables_jacobian: NULL argument");
+    if (nx < 4 || nq < 1 || !(x1 > x0)) return set_error(BOLT_E_ARG, "bolt_host_tables_jacobian: bad grid");
+    if (!(par->Omega_r > 0) || !(par->N_nu > 0) || !(par->h > 0))
+        return set_error(BOLT_E_ARG, "bolt_host_tables_jacobian: Omega_r, N_nu and h must be positive");
+    int rc = bolt_host_tables(par, x0, x1, nx, nq, coef_out, scal_out, xq_out, wq_out);
+    if (rc != BOLT_OK) return rc;
+    try {
+        bolt::BackgroundT<bolt::DualP> bg(bolt::ParamsT<bolt::DualP>::seeded(*par), x0, x1, nx, nq);
+        bg.ionization(-1.0);
+        const size_t ntab = bg.coef.size();
+        for (int p = 0; p < bolt::NPAR; ++p)
+            for (size_t i = 0; i < ntab; ++i) {
+                const double d = bg.coef[i].d[p];
+                if (!std::isfinite(d)) return set_error(BOLT_E_NUMERIC, "bolt_host_tables_jacobian: non-finite tangent");
+                dcoef_out[(size_t)p * ntab + i] = d;
+            }
+        const bolt::DualP s[BOLT_NSCAL] = {bg.H0, bg.eta0, bg.rho_crit, bg.Omega_L, bg.tau0};
+        for (int p = 0; p < bolt::NPAR; ++p)
+            for (int i = 0; i < BOLT_NSCAL; ++i) dscal_out[p * BOLT_NSCAL + i] = s[i].d[p];
+    } catch (const std::exception& e) {
+        return set_error(BOLT_E_NUMERIC, std::string("bolt_host_tables_jacobian: ") + e.what());
     }
     return BOLT_OK;
 }

@@ -256,8 +256,6 @@ R CosmoT<R>::eta(R x) const {
 // transient that the explicit steps, sized for the values, resolve only
 // roughly: the first few outputs of the helium stage carry tangents good to
 // about 1e-4 relative; elsewhere they follow the values' accuracy.
-static inline void add_tangent(double&, double, double) {}
-static inline void add_tangent(Dual& y, double slope, double dz) { y.d += slope * dz; }
 
 template <class R, int N, class F>
 static void dopri_outputs(F&& f, R z0_, R* y, const std::vector<R>& zout, std::vector<std::array<R, N>>& yout,
@@ -280,8 +278,8 @@ static void dopri_outputs(F&& f, R z0_, R* y, const std::vector<R>& zout, std::v
     double h = (zout.empty() ? 0.0 : (val(zout.back()) < z0 ? -1e-3 : 1e-3)) * std::max(1.0, std::fabs(z0));
     R k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], yt[N], yn[N];
     f(R(z), y, k1);
-    if (tangent(z0_) != 0.0) {
-        for (int i = 0; i < N; ++i) add_tangent(y[i], -val(k1[i]), tangent(z0_));
+    if (any_tangent(z0_)) {
+        for (int i = 0; i < N; ++i) add_tangent(y[i], -val(k1[i]), z0_);
         f(R(z), y, k1);
     }
     for (size_t io = 0; io < zout.size(); ++io) {
@@ -327,7 +325,7 @@ static void dopri_outputs(F&& f, R z0_, R* y, const std::vector<R>& zout, std::v
         }
         for (int i = 0; i < N; ++i) {
             yout[io][i] = y[i];
-            add_tangent(yout[io][i], val(k1[i]), tangent(zout[io]));
+            add_tangent(yout[io][i], val(k1[i]), zout[io]);
         }
     }
 }
@@ -524,10 +522,13 @@ struct RecfastT {
             if (std::fabs(hi - lo) <= 1e-13 * std::max(1.0, std::fabs(mid))) break;
         }
         const double z0 = 0.5 * (lo + hi);
-        if constexpr (std::is_same<R, Dual>::value) {
+        if constexpr (dual_traits<R>::is_dual) {
             const double dz = 1e-6 * std::max(1.0, std::fabs(z0));
             const double gz = (val((r.*f)(R(z0 + dz))) - val((r.*f)(R(z0 - dz)))) / (2.0 * dz);
-            return Dual(z0, -tangent((r.*f)(R(z0))) / gz);
+            const R g = (r.*f)(R(z0));
+            R root(z0);
+            for (int i = 0; i < (int)(sizeof(root.d) / sizeof(double)); ++i) root.d[i] = -g.d[i] / gz;
+            return root;
         } else {
             return z0;
         }
@@ -780,8 +781,10 @@ void BackgroundT<R>::ionization(double Tnow_override) {
 
 template struct CosmoT<double>;
 template struct CosmoT<Dual>;
+template struct CosmoT<DualP>;
 template struct BackgroundT<double>;
 template struct BackgroundT<Dual>;
+template struct BackgroundT<DualP>;
 
 int recfast_xe(const bolt_cosmo_params& par, double Tnow, const double* z, int nz, double* out) {
     Background bg(par, -20.0, 0.0, 2001, 15);

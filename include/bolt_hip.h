@@ -131,6 +131,16 @@ int bolt_host_tables_jvp(const bolt_cosmo_params* par, const bolt_cosmo_params* 
                          int nx, int nq, double* coef_out, double* scal_out, double* dcoef_out,
                          double* dscal_out, double* xq_out, double* wq_out);
 
+/* The derivative of bolt_host_tables with respect to every bolt_cosmo_params
+ * field at once: ForwardDiff's chunked Duals, nine partials in one pass.
+ * dcoef_out: [9][BOLT_NTAB][nx+2] and dscal_out: [9][BOLT_NSCAL], with partial p
+ * in struct order (h, Omega_r, Omega_b, Omega_c, A, n, Y_p, N_nu, Sum_m_nu).
+ * Partial p is bitwise bolt_host_tables_jvp with dpar the p-th unit vector;
+ * coef_out, scal_out, xq_out and wq_out receive what bolt_host_tables writes. */
+int bolt_host_tables_jacobian(const bolt_cosmo_params* par, double x0, double x1, int nx, int nq,
+                              double* coef_out, double* scal_out, double* dcoef_out, double* dscal_out,
+                              double* xq_out, double* wq_out);
+
 /* RECFAST X_e(z) before reionization, Xe_RECFAST(recfastsolve(RECFAST(bg; Tnow)))
  * (src/ionization/recfast.jl:413-475).  Tnow <= 0 takes the default derived from
  * Omega_r (recfast.jl:108). */

@@ -227,3 +227,22 @@ def test_jvp_rejects_bad_arguments():
     rc = lib().bolt_host_tables_jvp(ctypes.byref(par), None, -20.0, 0.0, 201, 15, pd(c), pd(s), pd(c), pd(s),
                                     pd(q), pd(q))
     assert rc == _abi.BOLT_E_ARG
+
+
+def test_jacobian_partials_are_bitwise_the_single_direction_jvps():
+    """bolt_host_tables_jacobian carries all nine partials in one pass; each is
+    bitwise the one-direction JVP seeded with that field's unit vector."""
+    import time
+    par = boltmi.CosmoParams(**MASSIVE)
+    t0 = time.perf_counter()
+    bg, dcoef, dscal = boltmi.tables_jacobian(par, x_grid=GRID)
+    t_jac = time.perf_counter() - t0
+    c0, s0 = tables(par)
+    assert np.array_equal(bg.coef, c0) and np.array_equal(bg.scal, s0)
+    t0 = time.perf_counter()
+    for p, field in enumerate(boltmi.PARAM_FIELDS):
+        _, _, dc, ds = jvp(par, {field: 1.0})
+        assert np.array_equal(dcoef[p], dc), field
+        assert np.array_equal(dscal[p], ds), field
+    t_jvp = time.perf_counter() - t0
+    print(f"jacobian {t_jac:.2f} s, nine JVPs {t_jvp:.2f} s")
