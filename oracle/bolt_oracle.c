@@ -62,6 +62,11 @@ static lay_t layout(const orc_cfg* c) { /* perturbations.jl:114-125 */
 
 int orc_state_length(const orc_cfg* c) { return layout(c).n; }
 
+/* the per-bin scratch arrays below are sized for ORC_MAXQ bins */
+static int nq_bad(int nq) { return nq < 1 || nq > ORC_MAXQ; }
+static int cfg_bad(const orc_cfg* c) { return nq_bad(c->nq) || c->ell_g < 2 || c->ell_nu < 2 || c->ell_mnu < 2; }
+static void fill_nan(double* v, size_t n) { for (size_t i = 0; i < n; ++i) v[i] = NAN; }
+
 /* ----------------------------------------------------------------- splines */
 /* uniform cubic B-spline value/derivatives (util.jl:11-13) */
 double orc_spline(const orc_tables* t, int tab, double x, int deriv) {
@@ -110,6 +115,7 @@ static double dlnf0dlnq(double q, double Tnu) { return -q / Tnu / (1.0 + exp(-q 
 
 void orc_rho_sigma(const orc_params* p, const orc_tables* t, int nq, double a, const double* M0,
                    const double* M2, double* rho, double* sigma) { /* perturbations.jl:127-145 */
+    if (nq_bad(nq)) { *rho = *sigma = NAN; return; }
     qgrid_t g = qgrid(p, t);
     double m = p->Sum_m_nu, sr = 0.0, ss = 0.0;
     for (int i = 0; i < nq; ++i) {
@@ -125,6 +131,7 @@ void orc_rho_sigma(const orc_params* p, const orc_tables* t, int nq, double a, c
 
 double orc_theta(const orc_params* p, const orc_tables* t, int nq, double a, const double* M1) {
     (void)a; /* perturbations.jl:148-158 */
+    if (nq_bad(nq)) return NAN;
     qgrid_t g = qgrid(p, t);
     double s = 0.0;
     for (int i = 0; i < nq; ++i) {
@@ -163,7 +170,7 @@ static void metric(const orc_params* p, const orc_tables* t, const lay_t* L, dou
                    const double* u, double* Psi, double* Phip) {
     double a = b->a, H02 = t->H0 * t->H0, Onu = omega_nu_rel(p);
     double rho, sig;
-    double M0[64], M2[64];
+    double M0[ORC_MAXQ], M2[ORC_MAXQ];
     for (int i = 0; i < L->nq; ++i) { M0[i] = u[L->iM + i]; M2[i] = u[L->iM + 2 * L->nq + i]; }
     orc_rho_sigma(p, t, L->nq, a, M0, M2, &rho, &sig);
     double Phi = u[L->iPhi];
@@ -182,13 +189,14 @@ static void metric(const orc_params* p, const orc_tables* t, const lay_t* L, dou
 void orc_rhs(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x, double* u,
              double* du, int mutate) {
     lay_t L = layout(c);
+    if (cfg_bad(c)) { fill_nan(du, L.n > 0 ? (size_t)L.n : 0); return; }
     bgx_t b = bgx(t, x);
     qgrid_t g = qgrid(p, t);
     double a = b.a, Hc = b.Hc, eta = b.eta, tp = b.tp, tpp = b.tpp, csb2 = b.csb2;
     double R = 4.0 * p->Omega_r / (3.0 * p->Omega_b * a);
     double Psi, Phip;
     metric(p, t, &L, k, &b, u, &Psi, &Phip);
-    double Phi = u[L.iPhi], del = u[L.idel], v = u[L.iv], db = u[L.idb], vb = u[L.ivb];
+    double Phi = u[L.iPhi], v = u[L.iv], db = u[L.idb], vb = u[L.ivb];
     const double* T = u + L.iT;
     const double* P = u + L.iP;
     const double* N = u + L.iN;
@@ -261,8 +269,8 @@ static void jac_entries(const orc_params* p, const orc_tables* t, const orc_cfg*
     int ng = L.ng, nn = L.nn, nm = L.nm, nq = L.nq;
     /* gradients of the Ψ and Φ′ functionals */
     int nidx = 0;
-    int idx[128];
-    double dPsi[128], dPhip[128];
+    int idx[8 + 2 * ORC_MAXQ];
+    double dPsi[8 + 2 * ORC_MAXQ], dPhip[8 + 2 * ORC_MAXQ];
     double c12 = 12.0 * H02 / (k * k) / (a * a);
     double gg = H02 / (2.0 * Hc * Hc);
 #define PUSH(col, vpsi, vphi) do { idx[nidx] = (col); dPsi[nidx] = (vpsi); dPhip[nidx] = (vphi); ++nidx; } while (0)
@@ -273,7 +281,7 @@ static void jac_entries(const orc_params* p, const orc_tables* t, const orc_cfg*
     PUSH(L.idb, 0.0, gg * p->Omega_b / a);
     PUSH(L.iT + 0, 0.0, gg * 4.0 * p->Omega_r / (a * a));
     PUSH(L.iN + 0, 0.0, gg * 4.0 * Onu / (a * a));
-    double qv[64], epsv[64], df0v[64];
+    double qv[ORC_MAXQ], epsv[ORC_MAXQ], df0v[ORC_MAXQ];
     for (int i = 0; i < nq; ++i) {
         double q = xq2q(t->xq[i], g.lqmin, g.lqmax);
         double eps = sqrt(q * q + (a * p->Sum_m_nu) * (a * p->Sum_m_nu));
@@ -356,6 +364,7 @@ static void add_dense(void* c, int r, int col, double v) { densectx* d = (densec
 
 void orc_jacobian(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x, double* J) {
     int n = orc_state_length(c);
+    if (cfg_bad(c)) { fill_nan(J, n > 0 ? (size_t)n * n : 0); return; }
     memset(J, 0, sizeof(double) * (size_t)n * n);
     densectx d = {J, n};
     jac_entries(p, t, c, k, x, add_dense, &d);
@@ -364,6 +373,7 @@ void orc_jacobian(const orc_params* p, const orc_tables* t, const orc_cfg* c, do
 /* ------------------------------------------------------ initial_conditions */
 void orc_initial_conditions(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* u) {
     lay_t L = layout(c);
+    if (cfg_bad(c)) { fill_nan(u, L.n > 0 ? (size_t)L.n : 0); return; }
     qgrid_t g = qgrid(p, t);
     memset(u, 0, sizeof(double) * L.n);
     double xi = t->x0;
@@ -406,13 +416,14 @@ void orc_initial_conditions(const orc_params* p, const orc_tables* t, const orc_
 double orc_source(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
                   const double* u, const double* du) {
     lay_t L = layout(c);
+    if (cfg_bad(c)) return NAN;
     double H02 = t->H0 * t->H0;
     double Hc = orc_spline(t, T_HC, x, 0), Hcp = orc_spline(t, T_HC_P, x, 0), Hcpp = orc_spline(t, T_HC_PP, x, 0);
     double tau = orc_spline(t, T_TAU, x, 0), tp = orc_spline(t, T_TAU_P, x, 0), tpp = orc_spline(t, T_TAU_PP, x, 0);
     double gt = orc_spline(t, T_G, x, 0), gtp = orc_spline(t, T_G_P, x, 0), gtpp = orc_spline(t, T_G_PP, x, 0);
     double a = exp(x);
     double Onu = omega_nu_rel(p);
-    double M0[64], M2[64], dM0[64], dM2[64];
+    double M0[ORC_MAXQ], M2[ORC_MAXQ], dM0[ORC_MAXQ], dM2[ORC_MAXQ];
     for (int i = 0; i < L.nq; ++i) {
         M0[i] = u[L.iM + i]; M2[i] = u[L.iM + 2 * L.nq + i];
         dM0[i] = du[L.iM + i]; dM2[i] = du[L.iM + 2 * L.nq + i];
@@ -610,6 +621,11 @@ static double wrms(const double* e, const double* u0, const double* u1, int n, d
 static int integrate(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, obs_t* obs,
                      orc_stats* st) {
     lay_t L = layout(c);
+    if (cfg_bad(c)) {
+        memset(st, 0, sizeof(*st));
+        st->status = 4;
+        return 4;
+    }
     int n = L.n;
     double rt = c->reltol, at = c->abstol;
     int maxsteps = c->max_steps > 0 ? c->max_steps : 100000;
@@ -865,35 +881,57 @@ int orc_boltsolve_rsa(const orc_params* p, const orc_tables* t, const orc_cfg* c
     return r;
 }
 
-double orc_plin(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x, orc_stats* st) {
+/* Ω_ν of the massive neutrino in plin's matter sum (spectra.jl:182-186):
+ * νfac = 90ζ(3)/(11π⁴) · Ω_r h²/T_γ · (N_ν/3)^(3/4), Ω_ν = Σm_ν νfac / h² */
+double orc_omega_nu_planted(const orc_params* p, const orc_tables* t, int plant) {
+    double Tg = pow(15.0 / (M_PI * M_PI) * t->rho_crit * p->Omega_r, 0.25);
+    double zeta = 1.2020569;
+    double nufac = (90.0 * zeta / (11.0 * pow(M_PI, 4))) * (p->Omega_r * p->h * p->h / Tg) * pow(p->N_nu / 3.0, 0.75);
+    if (plant & 2) nufac *= 1.1;
+    return p->Sum_m_nu * nufac / (p->h * p->h);
+}
+
+double orc_omega_nu(const orc_params* p, const orc_tables* t) { return orc_omega_nu_planted(p, t, 0); }
+
+/* plin (spectra.jl:163-198): one solve to x, the massive-ν density and velocity
+ * moments normalised by ρ₀ℳ(x), the comoving-gauge densities δ − 3ℋv/k, their
+ * Ω-weighted sum, and P = 2π²/k³ δ_m² A (k/0.05)^(n−1) */
+double orc_plin_planted(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                        orc_stats* st, int plant) {
     lay_t L = layout(c);
+    if (cfg_bad(c)) {
+        if (st) { memset(st, 0, sizeof(*st)); st->status = 4; }
+        return NAN;
+    }
     double* u = (double*)malloc(sizeof(double) * L.n);
     orc_boltsolve(p, t, c, k, &x, 1, u, st);
     double a = exp(x);
     double rho0 = orc_spline(t, T_RHO0M, x, 0);
-    double M0[64], M1[64], M2[64];
+    double M0[ORC_MAXQ], M1[ORC_MAXQ], M2[ORC_MAXQ];
     for (int i = 0; i < L.nq; ++i) {
         M0[i] = u[L.iM + i]; M1[i] = u[L.iM + L.nq + i]; M2[i] = u[L.iM + 2 * L.nq + i];
     }
     double rho, sig;
     orc_rho_sigma(p, t, L.nq, a, M0, M2, &rho, &sig);
     double Mrho = rho / rho0;
-    double Mtheta = k * orc_theta(p, t, L.nq, a, M1) / rho0;
+    double Mtheta = k * orc_theta(p, t, L.nq, a, M1) / rho0 * ((plant & 8) ? 2.0 : 1.0);
     double dcN = u[L.idel], dbN = u[L.idb], vcN = u[L.iv], vbN = u[L.ivb];
     double vmN = -Mtheta / k;
-    double Tg = pow(15.0 / (M_PI * M_PI) * t->rho_crit * p->Omega_r, 0.25);
-    double zeta = 1.2020569;
-    double nufac = (90.0 * zeta / (11.0 * pow(M_PI, 4))) * (p->Omega_r * p->h * p->h / Tg) * pow(p->N_nu / 3.0, 0.75);
-    double Onu = p->Sum_m_nu * nufac / (p->h * p->h);
+    double Onu = orc_omega_nu_planted(p, t, plant);
     double Om = p->Omega_c + p->Omega_b + Onu;
     double Hc = orc_spline(t, T_HC, x, 0);
-    double dc = dcN - 3.0 * Hc * vcN / k;
-    double db = dbN - 3.0 * Hc * vbN / k;
-    double dmn = Mrho - 3.0 * Hc * vmN / k;
+    double gs = (plant & 1) ? -3.0 : 3.0;
+    double dc = dcN - gs * Hc * vcN / k;
+    double db = dbN - gs * Hc * vbN / k;
+    double dmn = Mrho - gs * Hc * vmN / k;
     double dm = (p->Omega_c * dc + p->Omega_b * db + Onu * dmn) / Om;
     double Pprim = p->A * pow(k / 0.05, p->n - 1.0);
     free(u);
-    return (2.0 * M_PI * M_PI / (k * k * k)) * dm * dm * Pprim;
+    return (((plant & 4) ? 1.0 : 2.0) * M_PI * M_PI / (k * k * k)) * dm * dm * Pprim;
+}
+
+double orc_plin(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x, orc_stats* st) {
+    return orc_plin_planted(p, t, c, k, x, st, 0);
 }
 
 /* ------------------------------------------------------- spherical Bessel */

@@ -37,8 +37,12 @@ typedef struct {
 
 typedef struct {
     int64_t accepted, rejected, rhs_evals, factorizations;
-    int status; /* 0 ok, 1 max steps, 2 dt underflow, 3 non-finite */
+    int status; /* 0 ok, 1 max steps, 2 dt underflow, 3 non-finite, 4 bad configuration */
 } orc_stats;
+
+/* largest momentum-bin count the oracle's fixed per-bin arrays hold; calls with
+ * nq outside [1, ORC_MAXQ] or a cut-off below 2 return status 4 / NaN outputs */
+#define ORC_MAXQ 60
 
 int orc_state_length(const orc_cfg* c);
 /* KenCarp4 ESDIRK tableau used by the integrator: A [6][6] (diagonal γ), b, b̂, c */
@@ -58,6 +62,15 @@ double orc_source(const orc_params* p, const orc_tables* t, const orc_cfg* c, do
 void orc_rho_sigma(const orc_params* p, const orc_tables* t, int nq, double a, const double* M0,
                    const double* M2, double* rho, double* sigma);
 double orc_theta(const orc_params* p, const orc_tables* t, int nq, double a, const double* M1);
+/* Ω_ν of plin's non-relativistic massive neutrino (spectra.jl:182-186) */
+double orc_omega_nu(const orc_params* p, const orc_tables* t);
+/* plin with a deliberately planted mistake, for tests that show a pin can
+ * fail: plant bit 1 flips the sign of the 3ℋv/k gauge terms, bit 2 scales
+ * νfac by 1.1, bit 4 drops the factor 2 of 2π²/k³, bit 8 doubles θ.
+ * plant = 0 is orc_plin. */
+double orc_plin_planted(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double x,
+                        orc_stats* st, int plant);
+double orc_omega_nu_planted(const orc_params* p, const orc_tables* t, int plant);
 
 /* boltsolve + dense output at sorted x_save; out [nsave][n] */
 int orc_boltsolve(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,

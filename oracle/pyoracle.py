@@ -79,6 +79,10 @@ def lib():
         L.orc_rho_sigma.argtypes = [P, T, _I, _D, _PD, _PD, _PD, _PD]
         L.orc_theta.argtypes = [P, T, _I, _D, _PD]
         L.orc_theta.restype = _D
+        L.orc_omega_nu_planted.argtypes = [P, T, _I]
+        L.orc_omega_nu_planted.restype = _D
+        L.orc_plin_planted.argtypes = [P, T, C, _D, _D, ctypes.POINTER(OrcStats), _I]
+        L.orc_plin_planted.restype = _D
         _lib = L
     return _lib
 
@@ -187,11 +191,33 @@ class Oracle:
                               _pd(S), status.ctypes.data_as(_PI), nthreads)
         return S, status
 
-    def plin(self, cfg, k, x=0.0):
+    def plin(self, cfg, k, x=0.0, plant=0):
+        """P(k) at x (spectra.jl:163-198).  plant != 0 selects a deliberately
+        wrong variant (orc_plin_planted), used only to show that a pin fails."""
         st = OrcStats()
-        v = lib().orc_plin(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, x,
-                           ctypes.byref(st))
+        if plant:
+            v = lib().orc_plin_planted(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, x,
+                                       ctypes.byref(st), plant)
+        else:
+            v = lib().orc_plin(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, x,
+                               ctypes.byref(st))
         return v, st
+
+    def omega_nu(self, plant=0):
+        """Ω_ν of plin's massive neutrino (spectra.jl:182-186)"""
+        return lib().orc_omega_nu_planted(ctypes.byref(self.par), ctypes.byref(self.tab), plant)
+
+    def theta(self, M1, a=1.0):
+        M1 = np.ascontiguousarray(M1, dtype=np.float64)
+        return lib().orc_theta(ctypes.byref(self.par), ctypes.byref(self.tab), len(M1), a, _pd(M1))
+
+    def rho_sigma(self, M0, M2, a=1.0):
+        M0 = np.ascontiguousarray(M0, dtype=np.float64)
+        M2 = np.ascontiguousarray(M2, dtype=np.float64)
+        rho, sig = _D(), _D()
+        lib().orc_rho_sigma(ctypes.byref(self.par), ctypes.byref(self.tab), len(M0), a, _pd(M0), _pd(M2),
+                            ctypes.byref(rho), ctypes.byref(sig))
+        return rho.value, sig.value
 
     def cltt(self, S, k_src, kgrid, ells, bessel_mode=0, nthreads=0, kbeg=0, kend=None):
         S = np.ascontiguousarray(S, dtype=np.float64)

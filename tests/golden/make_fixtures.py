@@ -6,6 +6,8 @@ default /root/reference) and writes compact .npz files next to this script:
   recfast_xe.npz      Fortran RECFAST X_e(z) (runtests.jl:39-47), every row
   class_kXXX.npz      CLASS Newtonian-gauge x, Φ, δ_b (runtests.jl:103-145)
   camb_cl.npz         CAMB unlensed D_ℓ TT/TE/EE (runtests.jl:168)
+  camb_pk.npz         CAMB linear P(k) at z = 0, k in h/Mpc, P in (Mpc/h)³
+                      (test/data/camb_pk_x0.dat, test/camb_v_class_2022.ipynb)
 
 Only data values are stored (inputs and expected outputs).
 """
@@ -30,6 +32,8 @@ def main(ref="/root/reference"):
                             delta_b=rows[2][::-1], phi=rows[6][::-1])
     camb = np.loadtxt(os.path.join(d, "camb_rough_ttteee_unlensed.dat"))
     np.savez_compressed(os.path.join(HERE, "camb_cl.npz"), ell=camb[0], tt=camb[1], te=camb[2], ee=camb[3])
+    pk = np.loadtxt(os.path.join(d, "camb_pk_x0.dat"))
+    np.savez_compressed(os.path.join(HERE, "camb_pk.npz"), k_h=pk[0], P_h=pk[1])
 
 
 if __name__ == "__main__":
