@@ -76,46 +76,67 @@ def ode_flops(n, nq, steps, accepted, nmodes, nodes):
     return steps * per_step + accepted * rhs + nmodes * nodes * (8 * n + rhs + 120)
 
 
-def cpu_baseline(par, bg, ks, ells, dense, threads):
-    """The CPU FP64 oracle on a bounded sample of the same workload, run on
-    `threads` host threads, extrapolated to the full workload."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(par, bg, ks, ells, dense, threads, S_gpu, cl_gpu):
+    """The CPU FP64 oracle beside the GPU, on the same inputs, and the GPU's
+    error against it.
+
+    Source grid: the oracle solves a bounded sample of the modes (32 of 800,
+    spread over k), each on one thread of a `threads`-wide pool; per-mode times
+    are summed and scaled to all modes.  The sampled columns are compared with
+    the GPU's (largest |difference| / largest |value| per column).
+    C_ℓ^TT: the oracle sums the GPU's whole source grid with exact j_ℓ for all
+    2499 ℓ and 4999 dense k on `threads` threads, timed in full, and is compared
+    with the GPU's C_ℓ ℓ by ℓ.  A 1-core figure comes from single-thread timings
+    (the per-mode solves, and the LOS of 50 dense-k midpoints on one thread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from concurrent.futures import ThreadPoolExecutor
     o = pyoracle.Oracle(par.as_tuple(), bg.tables_dict())
     cfg = pyoracle.Oracle.cfg(LG, LN, LM, NQ, RELTOL, ABSTOL, 0, 1)
     mode_idx = list(range(12, NK, 25))           # 32 of 800 modes, spread over k
-    kmid_idx = list(range(25, NKD - 1, 50))      # 100 of 4999 k midpoints
-    S_full = np.zeros((NK, bg.nx))
+    mode_idx[-1] = NK - 1                        # and the longest, k = 0.5
+    cols = {}
 
     def one_mode(i):
         t0 = time.perf_counter()
         S, st = o.source_column(cfg, float(ks[i]))
-        S_full[i] = S
+        cols[i] = S
         return time.perf_counter() - t0
 
-    # the LOS sample needs a source grid: use the sampled columns' neighbours
     with ThreadPoolExecutor(threads) as ex:
         t_modes = list(ex.map(one_mode, mode_idx))
-    # fill the remaining columns by nearest sampled column (sample only)
-    si = np.array(mode_idx)
-    for i in range(NK):
-        S_full[i] = S_full[si[np.argmin(np.abs(si - i))]]
-
-    def one_k(ik):
-        t0 = time.perf_counter()
-        o.cltt(S_full, ks, dense, ells, 0, nthreads=1, kbeg=ik, kend=ik + 1)
-        return time.perf_counter() - t0
-
-    with ThreadPoolExecutor(threads) as ex:
-        t_ks = list(ex.map(one_k, kmid_idx))
-    t_sg = sum(t_modes) * (NK / len(mode_idx)) / threads
-    t_cl = sum(t_ks) * ((NKD - 1) / len(kmid_idx)) / threads
+    err_src = max(float(np.max(np.abs(S_gpu[i] - cols[i])) / np.max(np.abs(cols[i]))) for i in mode_idx)
+    t0 = time.perf_counter()
+    cl_ref = o.cltt(S_gpu, ks, dense, ells, 0, nthreads=threads)
+    t_cl = time.perf_counter() - t0
+    err_cl = float(np.max(np.abs(cl_gpu / cl_ref - 1)))
+    kmid_idx = list(range(50, NKD - 1, 100))     # 50 of 4999 midpoints, one thread
+    t0 = time.perf_counter()
+    for ik in kmid_idx:
+        o.cltt(S_gpu, ks, dense, ells, 0, nthreads=1, kbeg=ik, kend=ik + 1)
+    t_cl1 = (time.perf_counter() - t0) * (NKD - 1) / len(kmid_idx)
+    t_sg1 = sum(t_modes) * (NK / len(mode_idx))
+    t_sg = t_sg1 / threads
     return {"value": 1.0 / (t_sg + t_cl), "unit": "spectra/s", "cores": threads, "kind": "port",
-            "sample": f"oracle source_grid on {len(mode_idx)} of {NK} k-modes and cltt on {len(kmid_idx)} of "
-                      f"{NKD - 1} dense-k midpoints (all 2499 ℓ), {threads} threads; per-item times summed and "
-                      f"scaled to the full workload",
-            "est_source_grid_s": t_sg, "est_cltt_s": t_cl, "sample_cpu_s": sum(t_modes) + sum(t_ks)}
+            "cpu_model": cpu_model(), "one_core_value": 1.0 / (t_sg1 + t_cl1),
+            "sample": f"oracle source_grid on {len(mode_idx)} of {NK} k-modes (one thread each, {threads} at a "
+                      f"time; per-mode times summed and scaled to {NK}) + oracle cltt on the full GPU source grid "
+                      f"(2499 l x 4999 k, {threads} threads, timed in full); one_core_value from single-thread "
+                      f"timings (the sampled solves and 50 dense-k midpoints of the LOS)",
+            "est_source_grid_s": t_sg, "cltt_s": t_cl, "one_core_source_grid_s": t_sg1, "one_core_cltt_s": t_cl1,
+            "sample_cpu_s": sum(t_modes) + t_cl * threads,
+            "gpu_max_err": {"source_columns_scaled": err_src, "cltt_rel": err_cl,
+                            "columns_compared": len(mode_idx), "ells_compared": len(ells)}}
 
 
 def plain_gpu_baseline(lane, L, cfg, d_k, value):
@@ -291,6 +312,10 @@ def batch(args):
 
 
 def serve(args):
+    print(json.dumps(serve_run(args.rate, args.requests, args.slots, args.hbm_gib, args.seed)), flush=True)
+
+
+def serve_run(rate, nreq, slots, hbm_gib, seed):
     """Serving workload: seeded Poisson arrivals of spectrum requests, each with
     its own cosmology drawn from the box h ∈ [0.6, 0.8], Ω_b h² ∈ [0.020, 0.024],
     Ω_c h² ∈ [0.10, 0.14], n_s ∈ [0.92, 1.0], ln 10¹⁰A_s ∈ [2.9, 3.2], 30 % with
@@ -299,10 +324,10 @@ def serve(args):
     {500, 1500, 2500} (ℓᵧ = ℓ_ν = 50, reltol 1e-11, 5000 dense k).  Latency is
     submit to result, host tables included."""
     import boltmi
-    rng = np.random.default_rng(args.seed)
-    srv = boltmi.Server(slots=args.slots, hbm_gib=args.hbm_gib)
+    rng = np.random.default_rng(seed)
+    srv = boltmi.Server(slots=slots, hbm_gib=hbm_gib)
     reqs = []
-    for _ in range(args.requests):
+    for _ in range(nreq):
         h = rng.uniform(0.6, 0.8)
         massive = rng.uniform() < 0.3
         par = boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h**2, Ω_c=rng.uniform(0.10, 0.14) / h**2,
@@ -314,7 +339,7 @@ def serve(args):
         else:
             reqs.append(("cltt", par, dict(nk=nk, ell_max=int(rng.choice([500, 1500, 2500])),
                                            ℓ_mν=20 if massive else 10)))
-    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
     try:
         # one small request of each kind first (not timed): loads the kernels
         warm = [srv.submit_pk(boltmi.CosmoParams(), nk=8), srv.submit_cltt(boltmi.CosmoParams(), nk=8, ell_max=50,
@@ -339,9 +364,9 @@ def serve(args):
         "metric": METRIC, "value": len(reqs) / (t1 - t0), "unit": "spectra/s", "n_gpus": 1, "steps": len(reqs),
         "warmup": 2, "ms_per_step": (t1 - t0) * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"serving: Poisson arrivals at {args.rate}/s, {len(reqs)} requests, "
-                               f"{args.slots} slots; half P(k), half C_l^TT; 30% massive nu",
-                   "seed": args.seed},
+        "config": {"workload": f"serving: Poisson arrivals at {rate}/s, {len(reqs)} requests, "
+                               f"{slots} slots; half P(k), half C_l^TT; 30% massive nu",
+                   "seed": seed, "rate_per_s": rate},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "queue_ms_p50": float(np.percentile([i["queue_ms"] for i in infos], 50)),
@@ -349,7 +374,7 @@ def serve(args):
         "device_ms_p50": float(np.percentile([i["device_ms"] for i in infos], 50)),
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -361,6 +386,9 @@ def main():
                     help="independent spectra in flight, each on its own context (HIP stream) and host thread")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-mode-per-launch GPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default solver and LOS shapes)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the short serving leg (p50/p99 latency at two arrival rates) after the timed region")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the workload (reference "
@@ -474,6 +502,13 @@ def main():
     def split(k):
         return [k // len(lanes) + (1 if j < k % len(lanes) else 0) for j in range(len(lanes))]
 
+    # the project's autotuner picks the solver and LOS shapes for this workload
+    # (untimed; the choice holds for every context on the device)
+    tune = None
+    if not args.no_tune:
+        tune = boltmi.autotune(par, bg, cfg, ks, dense, LMAX, ctx=lanes[0].ctx)
+        print(f"# autotune: solver SR {tune['solver_sr']} {tune['solver_ms']}, LOS slots/wave {tune['los_spw']} "
+              f"{tune['los_ms']}", file=sys.stderr, flush=True)
     # every lane runs at least one untimed step (its context's first call sizes its cached buffers)
     run([max(c, 1) for c in split(args.warmup)], "warmup")
     torch.cuda.synchronize()
@@ -501,12 +536,15 @@ def main():
     los_tf = f_los_nominal / (ms_los * 1e-3) / 1e12
     los_tf_exec = 4.0 * los_exec / (ms_los * 1e-3) / 1e12
     if ms_ode >= ms_los:
-        roof = {"kernel": "k_boltsolve", "bound": "mfma", "achieved": ode_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                "frac": ode_tf / PEAK_FP64, "traffic": measured_traffic("k_boltsolve"),
-                "note": "FP64 vector peak (equal to the FP64 MFMA figure); algorithmic flops of the structured "
-                        "ESDIRK solve per step/RHS/node, from the kernel's own step counters"}
+        roof = {"kernel": "k_boltsolve", "bound": "fp64-vector", "achieved": ode_tf, "peak": PEAK_FP64,
+                "unit": "TFLOP/s", "frac": ode_tf / PEAK_FP64, "traffic": measured_traffic("k_boltsolve"),
+                "chip_level_tflops": f_ode / dt / 1e12,
+                "note": "FP64 vector peak (no MFMA instruction is used); algorithmic flops of the structured "
+                        "ESDIRK solve per step/RHS/node, from the kernel's own step counters, over the kernel's "
+                        "average HIP-event duration (which with spectra in flight includes waiting for CUs); "
+                        "chip_level_tflops divides the same flops by the wall time per spectrum"}
     else:
-        roof = {"kernel": "k_los", "bound": "mfma", "achieved": los_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
+        roof = {"kernel": "k_los", "bound": "fp64-vector", "achieved": los_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": los_tf / PEAK_FP64, "traffic": measured_traffic("k_los")}
     line = {
         "metric": METRIC, "value": 1.0 / dt, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
@@ -524,14 +562,33 @@ def main():
         "los": {"nominal_tflops": los_tf, "executed_tflops": los_tf_exec, "executed_steps": los_exec,
                 "nominal_steps": f_los_nominal / 4.0},
         "roofline": roof,
+        "autotune": tune,
     }
     if not args.no_plain:
         line["plain_gpu_baseline"] = plain_gpu_baseline(lanes[0], L, cfg, d_k, value=1.0 / dt)
     if not args.no_cpu:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(par, bg, ks, ells, dense, thr)
+        cb = cpu_baseline(par, bg, ks, ells, dense, thr, lanes[0].d_S.cpu().numpy(), lanes[0].d_cl.cpu().numpy())
+        line["max_rel_err_vs_oracle"] = cb.pop("gpu_max_err")
+        line["cpu_baseline"] = cb
     else:
         line["cpu_baseline"] = None
+    if not args.no_serve:
+        # after the timed region: p50/p99 of a seeded Poisson request stream at
+        # the benchmark's arrival rate and at one below the device's capacity
+        legs = {}
+        for rate in (50.0, 4.0):
+            r = serve_run(rate, 24, args.slots, args.hbm_gib, args.seed)
+            print(f"# serving at {rate}/s: p50 {r['latency_ms']['p50']:.0f} ms, p99 {r['latency_ms']['p99']:.0f} ms, "
+                  f"{r['value']:.2f} spectra/s", file=sys.stderr, flush=True)
+            legs[f"rate_{rate:g}_per_s"] = {"spectra_per_s": r["value"], "latency_ms": r["latency_ms"],
+                                            "queue_ms_p50": r["queue_ms_p50"], "device_ms_p50": r["device_ms_p50"],
+                                            "requests": r["steps"]}
+        line["serving"] = {"workload": "24 requests per rate: cosmologies from the parameter box (30% massive nu), "
+                                       "nk in {50,200,800}, half P(k), half C_l^TT to l_max in {500,1500,2500}; "
+                                       "latency from submit to result, host tables included",
+                           **legs}
+        line["p99_latency_ms"] = legs["rate_50_per_s"]["latency_ms"]["p99"]
     print(json.dumps(line), flush=True)
 
 

@@ -1,5 +1,5 @@
 """boltmi -- Bolt.jl's perturbation hot path on MI355X (libbolt_hip.so)."""
-from .api import (MASS_NATURAL, Background, backgrounds, BasicNewtonian, Context, CosmoParams, HIPNewtonian, Hierarchy,
+from .api import (MASS_NATURAL, Background, autotune, backgrounds, BasicNewtonian, Context, CosmoParams, HIPNewtonian, Hierarchy,
                   IonizationHistory, PerturbationIntegrator, RECFAST, Server, Solution, SourceGrid, boltsolve,
                   boltsolve_batch, boltsolve_rsa, cl_on_grid, clee, clee_on_grid, clte, clte_on_grid, cltt,
                   cltt_on_grid, default_context, hier_cfg,

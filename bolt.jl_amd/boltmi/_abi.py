@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libbolt_hip.so")
+# BOLT_HIP_LIB selects another in-tree build of the same library (the
+# NaN-filled-workspace debug build, libbolt_hip_nanfill.so)
+LIB_PATH = os.environ.get("BOLT_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "libbolt_hip.so")
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -45,6 +47,11 @@ class StatsC(ctypes.Structure):
                 ("linear_solves", _I64), ("kernel_ms", _D)]
 
 
+class TuneReportC(ctypes.Structure):
+    _fields_ = [("solver_sr", _I), ("solver_ms", ctypes.c_float * 4), ("los_spw", _I),
+                ("los_ms", ctypes.c_float * 3)]
+
+
 REQ_PK, REQ_CLTT = 0, 1
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -75,6 +82,7 @@ SIGNATURES = {
     "bolt_ctx_create": (_I, [_I, ctypes.POINTER(_VP)]),
     "bolt_ctx_destroy": (_I, [_VP]),
     "bolt_ctx_sync": (_I, [_VP]),
+    "bolt_ws_layout": (_I, [_I, _I, _I, _PI64, _PI, _PI64, _PI64, _PI64]),
     "bolt_cosmo_upload": (_I, [_VP, ctypes.POINTER(CosmoParamsC), ctypes.POINTER(TablesC), _PI]),
     "bolt_cosmo_release": (_I, [_VP, _I]),
     "bolt_rhs_batch": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PD, _I]),
@@ -100,6 +108,11 @@ SIGNATURES = {
     "bolt_cl_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, _VP, _VP,
                          ctypes.POINTER(ctypes.c_float)]),
     "bolt_cltt_dev": (_I, [_VP, _I, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, ctypes.POINTER(ctypes.c_float)]),
+    "bolt_autotune": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _I, _I, ctypes.POINTER(TuneReportC)]),
+    "bolt_graph_spectrum_create": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _VP, _I, _VP, _I, _PI, _I, _VP, _VP, _VP,
+                                        _VP, ctypes.POINTER(_VP)]),
+    "bolt_graph_launch": (_I, [_VP]),
+    "bolt_graph_destroy": (_I, [_VP]),
     "bolt_plin": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PI, ctypes.POINTER(StatsC)]),
     "bolt_serve_start": (_I, [_VP, _I, _D]),
     "bolt_serve_slots": (_I, [_VP]),

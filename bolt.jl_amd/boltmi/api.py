@@ -17,8 +17,10 @@ bg.rho_crit, bg.Omega_L; ih.Xe, ih.tau, ih.tau_p, ih.tau_pp, ih.g, ih.g_p,
 ih.g_pp, ih.Tmat, ih.csb2, ih.tau0).
 """
 import ctypes
+import itertools
 import math
 import threading
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -125,7 +127,10 @@ class Background:
         self.rho0M = _Spline(self, _abi.T_RHO0M)
         self._ct = None
         self._ct_tangent = None
+        # uploads of these tables: Context serial -> (weak reference to the
+        # Context, cosmology id); released when this Background is collected
         self._uploads = {}
+        weakref.finalize(self, _release_uploads, self._uploads)
 
     def _scalar(self, i):
         return float(self.scal[i]) if self.dscal is None else Dual(float(self.scal[i]), float(self.dscal[i]))
@@ -150,13 +155,28 @@ class Background:
         if self.dcoef is not None:
             raise TypeError("a Background built from Dual parameters cannot be uploaded: derivatives stop at the "
                             "host tables (the tangent-linear perturbation solve is not built)")
-        key = id(ctx)
-        if key not in self._uploads:
+        if not ctx.handle:
+            raise ValueError("the context is closed")
+        ent = self._uploads.get(ctx.serial)
+        if ent is None:
             cid = ctypes.c_int(-1)
             check(lib().bolt_cosmo_upload(ctx.handle, ctypes.byref(self.par.c_struct()),
                                           ctypes.byref(self.c_tables()), ctypes.byref(cid)))
-            self._uploads[key] = cid.value
-        return self._uploads[key]
+            ent = (weakref.ref(ctx), cid.value)
+            self._uploads[ctx.serial] = ent
+        return ent[1]
+
+
+def _release_uploads(uploads):
+    """free a collected Background's device tables on every live context"""
+    for ref, cid in list(uploads.values()):
+        ctx = ref()
+        if ctx is not None and ctx.handle:
+            try:
+                lib().bolt_cosmo_release(ctx.handle, cid)
+            except Exception:
+                pass
+    uploads.clear()
 
 
 def backgrounds(pars, x_grid=(-20.0, 0.0, 2001), nq=15, threads=0):
@@ -223,6 +243,8 @@ class Context:
         check(lib().bolt_ctx_create(device, ctypes.byref(h)))
         self.handle = h
         self.lock = threading.Lock()
+        # never reused, unlike id(): keys the uploads a Background keeps
+        self.serial = next(_ctx_serial)
 
     def close(self):
         if self.handle:
@@ -236,6 +258,7 @@ class Context:
             pass
 
 
+_ctx_serial = itertools.count(1)
 _default_ctx = None
 
 
@@ -458,19 +481,37 @@ def source_grid_batch(cosmos, k_grids, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν
     for c, (g, n_, m_, q_, rt, at) in enumerate(zip(_per(ℓᵧ, nc), _per(ℓ_ν, nc), _per(ℓ_mν, nc), _per(nq, nc),
                                                     _per(reltol, nc), _per(abstol, nc))):
         cfgs[c] = hier_cfg(g, n_, m_, q_, rt, at)
-    ids = np.array([bg.cosmo_id(ctx) for (_, bg, _) in cosmos], dtype=np.int32)
     nx = cosmos[0][1].nx
     S = np.zeros((nc, nk, nx)) if want_t else None
     SP = np.zeros((nc, nk, nx)) if want_p else None
     status = np.zeros((nc, nk), dtype=np.int32)
     st = _abi.StatsC()
     with ctx.lock:
+        ids = np.array([bg.cosmo_id(ctx) for (_, bg, _) in cosmos], dtype=np.int32)
         check(lib().bolt_source_grid_batch(ctx.handle, nc, pi(ids), cfgs, pd(kg), nk, pd(S) if want_t else None,
                                            pd(SP) if want_p else None, pi(status), ctypes.byref(st)))
     stats = _stats_dict(st)
     out_t = [SourceGrid(cosmos[c][1].x_grid, kg[c], S[c], stats) for c in range(nc)] if want_t else None
     out_p = [SourceGrid(cosmos[c][1].x_grid, kg[c], SP[c], stats) for c in range(nc)] if want_p else None
     return out_t, out_p
+
+
+def autotune(par, bg, cfg, k_grid, kgrid=None, ell_max=0, ctx=None):
+    """Time every solver shape (rows per lane of the substitution scans, 5..8)
+    on source_grid over k_grid with hierarchy cfg, and, given a dense kgrid
+    and ell_max, every LOS shape (x slots per wavefront 1, 2, 4) on cltt for
+    ℓ = 2..ell_max; later calls on this device use the fastest.  Returns the
+    choices and the per-shape times (ms)."""
+    ctx = ctx or default_context()
+    ks = np.ascontiguousarray(k_grid, dtype=np.float64)
+    kg = None if kgrid is None else np.ascontiguousarray(kgrid, dtype=np.float64)
+    rep = _abi.TuneReportC()
+    with ctx.lock:
+        check(lib().bolt_autotune(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks),
+                                  None if kg is None else pd(kg), 0 if kg is None else len(kg), int(ell_max),
+                                  ctypes.byref(rep)))
+    return {"solver_sr": rep.solver_sr, "solver_ms": {sr: float(rep.solver_ms[sr - 5]) for sr in range(5, 9)},
+            "los_spw": rep.los_spw, "los_ms": {spw: float(rep.los_ms[i]) for i, spw in enumerate((1, 2, 4))}}
 
 
 def quadratic_k(kmin, kmax, nk):

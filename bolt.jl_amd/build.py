@@ -27,13 +27,17 @@ def hip_flags():
             "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 
 
-def build(verbose=False):
+def build(verbose=False, nanfill=False):
+    """libbolt_hip.so; nanfill=True builds the debug variant
+    libbolt_hip_nanfill.so, whose solver starts every k-mode from a workspace
+    filled with NaNs (load it with BOLT_HIP_LIB=<path>)."""
     objs = []
-    tmp = os.path.join(HERE, "_build")
+    tmp = os.path.join(HERE, "_build_nanfill" if nanfill else "_build")
     os.makedirs(tmp, exist_ok=True)
     for src in SOURCES:
         obj = os.path.join(tmp, src + ".o")
-        cmd = [HIPCC] + hip_flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + hip_flags() + (["-DBOLT_DEBUG_NANFILL"] if nanfill else []) + \
+              ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC, "-x", "c++"] + [f for f in hip_flags() if not f.startswith("--offload-arch")
                                            and f not in ("-Xclang", "-target-feature", "-packed-fp32-ops")] + \
@@ -43,14 +47,15 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    out_tmp = LIB + ".tmp"
+    lib = LIB.replace(".so", "_nanfill.so") if nanfill else LIB
+    out_tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out_tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(out_tmp, LIB)
-    return LIB
+    os.replace(out_tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, nanfill="--nanfill" in sys.argv))

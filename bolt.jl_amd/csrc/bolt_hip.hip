@@ -21,12 +21,16 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -276,7 +280,14 @@ struct Bands {
 };
 
 #define CB 8      // rows per block of the banded sweeps
-#define SR 6      // rows per lane of the substitution scans (64·SR rows per pass)
+// rows per lane of the substitution scans (64·SR rows per pass): a template
+// parameter of the solver, SR_MIN..SR_MAX, picked per hierarchy size by the
+// autotuner (SR_DEFAULT until tuned).  Lane strides of 5 or 7 doubles spread a
+// wavefront's 8-byte LDS accesses over all banks, 6 over half of them; 8 covers
+// n = 473 in one pass.
+#define SR_MIN 5
+#define SR_MAX 8
+#define SR_DEFAULT 6
 #define NXC 24    // doubles per stored XC
 
 // LDS workspace of one k-mode.  Vectors of n doubles: u, f, u_{n+1}, f_{n+1}, ψ,
@@ -290,58 +301,58 @@ struct Bands {
 struct WS {
     double* b;
     int n, nq;
-    __device__ double* u() const { return b; }
-    __device__ double* f() const { return b + n; }
-    __device__ double* un() const { return b + 2 * n; }
-    __device__ double* fn() const { return b + 3 * n; }
-    __device__ double* arg() const { return b + 4 * n; }
-    __device__ double* w() const { return b + 5 * n; }
-    __device__ double* Z(int j) const { return b + (6 + j) * n; }
-    __device__ double* cS() const { return b + 12 * n; }
-    __device__ double* cP() const { return b + 13 * n; }
-    __device__ double* cPi() const { return b + 14 * n; }
-    __device__ double* rcp_(int j) const { return b + (15 + j) * n; }  // recipe vector j (RC_*)
-    __device__ Bands bands(int s) const {
+    __host__ __device__ double* u() const { return b; }
+    __host__ __device__ double* f() const { return b + n; }
+    __host__ __device__ double* un() const { return b + 2 * n; }
+    __host__ __device__ double* fn() const { return b + 3 * n; }
+    __host__ __device__ double* arg() const { return b + 4 * n; }
+    __host__ __device__ double* w() const { return b + 5 * n; }
+    __host__ __device__ double* Z(int j) const { return b + (6 + j) * n; }
+    __host__ __device__ double* cS() const { return b + 12 * n; }
+    __host__ __device__ double* cP() const { return b + 13 * n; }
+    __host__ __device__ double* cPi() const { return b + 14 * n; }
+    __host__ __device__ double* rcp_(int j) const { return b + (15 + j) * n; }  // recipe vector j (RC_*)
+    __host__ __device__ Bands bands(int s) const {
         Bands B;
         B.l1 = b + (21 + 3 * s) * n; B.d = b + (22 + 3 * s) * n; B.u1 = b + (23 + 3 * s) * n;
         B.l2c = b + 36 * n + 27 * nq + 2 * CB * s;
         B.u2c = B.l2c + CB;
         return B;
     }
-    __device__ double* q() const { return b + 36 * n; }
-    __device__ double* df0() const { return b + 36 * n + nq; }
-    __device__ double* base() const { return b + 36 * n + 2 * nq; }
-    __device__ double* qe() const { return b + 36 * n + 3 * nq; }
-    __device__ double* eqdf() const { return b + 36 * n + 4 * nq; }
-    __device__ double* sw() const { return b + 36 * n + 5 * nq; }
-    __device__ double* rw() const { return b + 36 * n + 6 * nq; }
-    __device__ double* qst_base(int s) const { return b + 36 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
-    __device__ QSet qcur() const { QSet Q; Q.qe = qe(); Q.eqdf = eqdf(); Q.sw = sw(); Q.rw = rw(); return Q; }
-    __device__ QSet qst(int s) const {
+    __host__ __device__ double* q() const { return b + 36 * n; }
+    __host__ __device__ double* df0() const { return b + 36 * n + nq; }
+    __host__ __device__ double* base() const { return b + 36 * n + 2 * nq; }
+    __host__ __device__ double* qe() const { return b + 36 * n + 3 * nq; }
+    __host__ __device__ double* eqdf() const { return b + 36 * n + 4 * nq; }
+    __host__ __device__ double* sw() const { return b + 36 * n + 5 * nq; }
+    __host__ __device__ double* rw() const { return b + 36 * n + 6 * nq; }
+    __host__ __device__ double* qst_base(int s) const { return b + 36 * n + (7 + 4 * s) * nq; }  // qe, eqdf, sw, rw
+    __host__ __device__ QSet qcur() const { QSet Q; Q.qe = qe(); Q.eqdf = eqdf(); Q.sw = sw(); Q.rw = rw(); return Q; }
+    __host__ __device__ QSet qst(int s) const {
         const double* p = qst_base(s);
         QSet Q; Q.qe = p; Q.eqdf = p + nq; Q.sw = p + 2 * nq; Q.rw = p + 3 * nq;
         return Q;
     }
-    __device__ double* xcs(int s) const { return b + 36 * n + 27 * nq + 10 * CB + NXC * s; }
+    __host__ __device__ double* xcs(int s) const { return b + 36 * n + 27 * nq + 10 * CB + NXC * s; }
     // scalar table of stage point s (s < 5) or of the current x (s = 5)
-    __device__ double* ktab(int s) const { return b + 36 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
-    __device__ double* kcur() const { return ktab(5); }
-    __device__ double* red() const { return ktab(6); }  // [16]
-    __device__ double* dump() const { return red() + 15; }  // store slot nobody reads
-    __device__ int* desc() const { return (int*)(red() + 16); }
-    __device__ int* kinds() const { return desc() + n; }
-    __device__ int* seg() const { return desc() + 2 * n; }          // [MAXSEG][2]: c0, m
-    __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg
+    __host__ __device__ double* ktab(int s) const { return b + 36 * n + 27 * nq + 10 * CB + 5 * NXC + (K_Q0 + 2 * nq) * s; }
+    __host__ __device__ double* kcur() const { return ktab(5); }
+    __host__ __device__ double* red() const { return ktab(6); }  // [16]
+    __host__ __device__ double* dump() const { return red() + 15; }  // store slot nobody reads
+    __host__ __device__ int* desc() const { return (int*)(red() + 16); }
+    __host__ __device__ int* kinds() const { return desc() + n; }
+    __host__ __device__ int* seg() const { return desc() + 2 * n; }          // [MAXSEG][2]: c0, m
+    __host__ __device__ int* misc() const { return seg() + 2 * MAXSEG; }     // nseg
 };
 
-__device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
+__host__ __device__ inline void ws_bind(WS& W, double* lds, int n, int nq) {
     W.b = lds;
     W.n = n;
     W.nq = nq;
 }
 __host__ inline size_t ws_bytes(int n, int nq) {
     return sizeof(double) * ((size_t)36 * n + 27 * nq + 10 * CB + 5 * NXC + 6 * (K_Q0 + 2 * nq) + 16) +
-           sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4) + sizeof(double) * WAVE * SR;  // scan read padding
+           sizeof(int) * (size_t)(2 * n + 2 * MAXSEG + 4) + sizeof(double) * WAVE * SR_MAX;  // scan read padding
 }
 
 __device__ inline void store_xc(double* p, const XC& X) {
@@ -772,6 +783,18 @@ __device__ inline void chain_factor_all(const Geo& G, const WS& W, int lane) {
     }
 }
 
+// Makes LDS stores of any lane of this wavefront visible to later LDS loads of
+// any other lane of the same wavefront.  The hardware completes one
+// wavefront's LDS operations in issue order, so what is needed is that the
+// compiler keep them in program order across this point: a wavefront-scope
+// fence (no instruction is emitted for it) and a wave barrier, neither of which
+// LDS accesses may cross.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- substitution by wavefront scans
 // With every chain's first row at l1 = 0 and last row at u1 = 0, the forward
 // and back substitutions over all n rows are each one first-order affine
@@ -822,10 +845,11 @@ __device__ inline void affine_scan(double& a, double& b) {
     affine_step<0x143, 0xC>(a, b);
 }
 
-static_assert(SR >= 4, "rows 0..3 must sit in one lane");
+static_assert(SR_MIN >= 4, "rows 0..3 must sit in one lane");
 
 // the lane maps of one pass: al/be hold f_i ∘ ... ∘ f_0 of the lane's rows;
 // returns the lane's incoming value and advances carry past the pass
+template <int SR>
 __device__ inline double pass_scan(double (&al)[SR], double (&be)[SR], double& carry) {
 #pragma unroll
     for (int i = 1; i < SR; ++i) { be[i] = fma(al[i], be[i - 1], be[i]); al[i] = al[i] * al[i - 1]; }
@@ -844,6 +868,7 @@ __device__ inline double pass_scan(double (&al)[SR], double (&be)[SR], double& c
 // forward pass rows past n come after every real row, and the scan carries
 // values only toward later rows, so they need no masking; in the backward
 // pass they come first and are masked to the identity map.
+template <int SR>
 __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* dump, int lane) {
     constexpr int span = WAVE * SR;
     double carry = 0.0;
@@ -866,7 +891,8 @@ __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* du
         }
     }
     // backward: pass p covers rows p·span .. p·span + span - 1, lane order
-    // running down the rows
+    // running down the rows; it reads rows other lanes stored above
+    wave_lds_sync();
     carry = 0.0;
     for (int p = (n - 1) / span; p >= 0; --p) {
         double al[SR], be[SR];
@@ -897,22 +923,26 @@ __device__ inline void wave_subst(const Bands& Bd, int n, double* bv, double* du
             *dst = x[i];
         }
     }
+    // the caller's next reads of bv (the border functionals) come from other lanes
+    wave_lds_sync();
 }
 
 // the four right-hand sides w, c_S, c_P, c_Π after chain_factor: wavefront r
 // takes right-hand side r
+template <int SR>
 __device__ inline void chain_solve4(const Geo& G, const WS& W, int st, const Thr& T) {
     const Bands Bd = W.bands(st);
     for (int r = T.wv; r < 4; r += T.nt / WAVE) {
         double* bv = r == 0 ? W.w() : r == 1 ? W.cS() : r == 2 ? W.cP() : W.cPi();
-        wave_subst(Bd, G.n, bv, W.dump(), T.lane);
+        wave_subst<SR>(Bd, G.n, bv, W.dump(), T.lane);
     }
 }
 
 // one right-hand side with the stored factors, by the first wavefront
+template <int SR>
 __device__ inline void chain_solve1(const Geo& G, const WS& W, int st, double* bvec, const Thr& T) {
     if (T.wv != 0) return;
-    wave_subst(W.bands(st), G.n, bvec, W.dump(), T.lane);
+    wave_subst<SR>(W.bands(st), G.n, bvec, W.dump(), T.lane);
 }
 
 // 3x3 capacitance matrix M = I - hγ D Y (D: Ψ, Φ′, Π rows; Y = W_L⁻¹[c_S c_P c_Π])
@@ -1165,6 +1195,7 @@ struct SolveArgs {
     int* status;           // [out]
     long long* counters;   // [out][4]: accepted, rejected, rhs evals, solves
     int n_state_stride;    // n of the output layout (mode 0)
+    int lds_doubles;       // dynamic LDS of the launch, in doubles
 };
 
 
@@ -1245,6 +1276,7 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
 }
 
 // one workgroup of NWAVE wavefronts per k-mode
+template <int SR>
 __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
     extern __shared__ double lds[];
     __shared__ int s_mode;
@@ -1254,6 +1286,12 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
         __syncthreads();
         const int mi = s_mode;
         if (mi >= A.nmodes) return;
+#ifdef BOLT_DEBUG_NANFILL
+        // debug build: each mode starts from a workspace full of NaNs, so a read
+        // of anything this mode has not written itself shows as a wrong value
+        for (int i = T.tid; i < A.lds_doubles; i += T.nt) lds[i] = __builtin_nan("");
+        __syncthreads();
+#endif
         ModeDesc md = A.modes[mi];
         const CosmoDev C = A.cosmos[md.cosmo];
         const CfgDev cf = A.cfgs[md.cfg];
@@ -1381,7 +1419,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                     ++nrhs;
                     __syncthreads();
                 }
-                chain_solve4(G, W, s - 1, T);
+                chain_solve4<SR>(G, W, s - 1, T);
                 ++nsol;
                 // no barrier: wavefront r's border functionals read only the
                 // vector it has just solved (a wavefront's LDS accesses stay in
@@ -1406,7 +1444,7 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             const bool spec = !last;
             Fun Fn;
             if (T.wv == 0) {
-                chain_solve1(G, W, 4, W.arg(), T);
+                chain_solve1<SR>(G, W, 4, W.arg(), T);
             } else if (spec) {
                 Thr Tf = T;
                 Tf.tid -= WAVE;
@@ -1629,8 +1667,12 @@ __global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const Mo
 // NS = 1: one source (C_ℓ^TT, or C_ℓ^EE with the spin factor); NS = 2: the
 // temperature and polarization sources share each j_ℓ and give TT, TE, EE.
 #define LOS_SLOTS 16   // x slots of 64 nodes (up to 1024 nodes after x = -8)
-#define LOS_WAVES 4
-#define LOS_SPW (LOS_SLOTS / LOS_WAVES)
+// blocks of one LOS launch: block b takes the dense-k midpoints b, b + grid,
+// b + 2·grid, ... (a spread over k, so blocks cost about the same) and sums
+// their C_ℓ contributions into a row of its own.  Fixed, so that the
+// summation order, and with it every bit of C_ℓ, does not depend on the device
+#define LOS_GRID 512
+#define LOS_SPW_DEFAULT 4  // x slots per wavefront (LOS_SLOTS/SPW wavefronts per block): 1, 2 or 4
 #define LOS_TAU 1e-7
 // lanes with z < LOS_ZS take j_ℓ from its ascending series
 //   j_ℓ(z) = z^ℓ Σ_m c_{ℓ,m} z^{2m},  c_{ℓ,m} = (-1/2)^m / (m! (2ℓ+2m+1)!!),
@@ -1641,7 +1683,7 @@ __global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const Mo
 #define LOS_TAU_SMALL 1e-30
 #define JSER_L 48
 #define JSER_M 14
-__constant__ double c_jser[JSER_L][JSER_M];
+__device__ double c_jser[JSER_L][JSER_M];  // global, not constant: see jser
 
 static void fill_jser(double* c) {
     for (int l = 0; l < JSER_L; ++l)
@@ -1653,9 +1695,9 @@ static void fill_jser(double* c) {
         }
 }
 
-// out of line: inlined into the unrolled ℓ loop its scalar coefficient loads
-// are hoisted together and exhaust the scalar registers
-__device__ __attribute__((noinline)) double jser(int l, double w, double zl) {
+// inlined; in the unrolled ℓ loop each ℓ's coefficient loads stay behind a
+// compiler barrier (los_slot_series), so they are not all hoisted together
+__device__ inline double jser(int l, double w, double zl) {
     const double* c = c_jser[l];
     double F = c[JSER_M - 1];
 #pragma unroll
@@ -1671,14 +1713,14 @@ struct LosArgs {
     int nk_src;
     const double* kgrid;  // [nkd]
     int nkd;
-    const int* lidx;      // [lmax+1] output slot of ℓ or -1
+    const int* lidx;      // [lmax+1] column of ℓ among the distinct requested ℓ, or -1
     int lmax;
-    int nell;
+    int nu;               // distinct requested ℓ
     const double* etaj;   // [nxi] η at the integration nodes
     const double* dxj;    // [nxi] x_{j+1}-x_j
     int i0, nxi;
     int spin;             // NS = 1: multiply T_ℓ by ℓð (E modes)
-    double* contrib;      // [nout][nell][nkd-1], nout = 1 (NS = 1) or 3 (TT, TE, EE)
+    double* partial;      // [grid][nout][nu] per-block sums over its midpoints; nout = 1 (NS = 1) or 3 (TT, TE, EE)
     long long* steps;     // executed recurrence steps (per k), may be NULL
 };
 
@@ -1735,6 +1777,7 @@ __device__ inline void los_slot_series(double& jp, double& jc, double iz, double
     const double base = (double)(2 * L0 - 1), w = z * z;
 #pragma unroll
     for (int i = I0; i < CH; ++i) {
+        asm volatile("" ::: "memory");  // keeps this ℓ's series coefficients' loads here
         const int l = L0 + i;
         double t = (base + 2.0 * i) * iz;
         double jn;
@@ -1765,8 +1808,13 @@ __device__ inline double los_reduce(double* acc, int lane) {
         bool up = (lane & w) != 0;
 #pragma unroll
         for (int i = 0; i < w; ++i) {
-            double send = up ? acc[i] : acc[i + w];
-            double keep = up ? acc[i + w] : acc[i];
+            // the two values pass through an empty asm so that the selects act
+            // on registers: a select between two elements of acc would otherwise
+            // become a lane-dependent index into the whole array
+            double a = acc[i], b = acc[i + w];
+            asm volatile("" : "+v"(a), "+v"(b));
+            const double send = up ? a : b;
+            const double keep = up ? b : a;
             acc[i] = keep + __shfl_xor(send, w, WAVE);
         }
     }
@@ -1776,176 +1824,200 @@ __device__ inline double los_reduce(double* acc, int lane) {
     return T;
 }
 
-template <int NS>
-__global__ void __launch_bounds__(LOS_WAVES * WAVE) k_los(LosArgs A) {
+// j₀(z), j₁(z): the ascending series below LOS_ZS, sin/cos above.  Out of
+// line: inlined, the polynomial constants of sincos stay live across the
+// block's whole midpoint loop and crowd out the recurrence's registers.
+__device__ __attribute__((noinline)) double2 los_j01(double z) {
+    if (z < LOS_ZS) return make_double2(jser(0, z * z, 1.0), jser(1, z * z, z));
+    double sn, cs;
+    sincos(z, &sn, &cs);
+    const double j0 = sn / z;
+    return make_double2(j0, (j0 - cs) / z);
+}
+
+// A (k/0.05)^(n−1) Δk/k, out of line for the same reason
+__device__ __attribute__((noinline)) double los_pw(double A_s, double n_s, double k, double dk) {
+    return A_s * pow(k / 0.05, n_s - 1.0) * dk / k;
+}
+
+// T_ℓ(k) = Σ_x j_ℓ(k(η₀−η)) S(x, k) Δx for every requested ℓ, and the
+// C_ℓ contributions T_ℓ² P(k) Δk/k summed over the block's midpoints.
+// A block of LOS_SLOTS/SPW wavefronts; wavefront w owns x slots w, w + W, ...
+// (SPW of them).  C_ℓ of ℓ is accumulated by lane ℓ mod 16 of wavefront 0
+// only (its partial-row entry is read and written by no other thread).
+template <int NS, int SPW>
+__global__ void __launch_bounds__(LOS_SLOTS / SPW * WAVE) k_los(LosArgs A) {
     constexpr int CH = 16;  // ℓ per chunk (NS·CH accumulators per lane)
-    // one block (LOS_WAVES wavefronts) per dense-k midpoint; wave w owns x slots w, w + LOS_WAVES, ...
+    constexpr int NW = LOS_SLOTS / SPW;
+    constexpr int NOUT = NS == 1 ? 1 : 3;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
-    const int ik = blockIdx.x;
-    if (ik >= A.nkd - 1) return;
-    __shared__ double part[LOS_WAVES][NS][CH];
+    __shared__ double part[NW][NS][CH];
+    __shared__ double nsum[NW];
     const double eta0 = A.cosmo->eta0;
     const int nx = A.cosmo->nx;
     const int nkm = A.nkd - 1;
-    double k = (A.kgrid[ik] + A.kgrid[ik + 1]) / 2.0;
-    double dk = A.kgrid[ik + 1] - A.kgrid[ik];
-    // Gridded-linear interpolation in k with Line extrapolation
-    int lo = 0, hi = A.nk_src - 2;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (A.k_src[mid] <= k) lo = mid; else hi = mid - 1;
-    }
-    int ia = lo;
-    double tt = (k - A.k_src[ia]) / (A.k_src[ia + 1] - A.k_src[ia]);
-    const double* Sa = A.S + (size_t)ia * nx + A.i0;
-    const double* Sb = A.S + (size_t)(ia + 1) * nx + A.i0;
-    const double* Pa = NS == 2 ? A.SP + (size_t)ia * nx + A.i0 : nullptr;
-    const double* Pb = NS == 2 ? A.SP + (size_t)(ia + 1) * nx + A.i0 : nullptr;
+    double* row = A.partial + (size_t)blockIdx.x * NOUT * A.nu;
+    bool first = true;  // the block's first midpoint stores, later ones add
+    for (int ik = blockIdx.x; ik < nkm; ik += gridDim.x) {
+        const double k = (A.kgrid[ik] + A.kgrid[ik + 1]) / 2.0;
+        const double dk = A.kgrid[ik + 1] - A.kgrid[ik];
+        // Gridded-linear interpolation in k with Line extrapolation
+        int lo = 0, hi = A.nk_src - 2;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (A.k_src[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const int ia = lo;
+        const double tt = (k - A.k_src[ia]) / (A.k_src[ia + 1] - A.k_src[ia]);
+        const double* Sa = A.S + (size_t)ia * nx + A.i0;
+        const double* Sb = A.S + (size_t)(ia + 1) * nx + A.i0;
+        const double* Pa = NS == 2 ? A.SP + (size_t)ia * nx + A.i0 : nullptr;
+        const double* Pb = NS == 2 ? A.SP + (size_t)(ia + 1) * nx + A.i0 : nullptr;
 
-    double jp[LOS_SPW], jc[LOS_SPW], iz[LOS_SPW], sv[LOS_SPW], spv[LOS_SPW], mx[LOS_SPW], zv[LOS_SPW], zl[LOS_SPW];
-    bool alive[LOS_SPW], small[LOS_SPW];
+        double jp[SPW], jc[SPW], iz[SPW], sv[SPW], spv[SPW], mx[SPW], zv[SPW], zl[SPW];
+        bool alive[SPW], small[SPW];
 #pragma unroll
-    for (int sl = 0; sl < LOS_SPW; ++sl) {
-        int j = (wv + LOS_WAVES * sl) * WAVE + lane;
-        if (j < A.nxi) {
-            double z = k * (eta0 - A.etaj[j]);
-            double s = ((1.0 - tt) * Sa[j] + tt * Sb[j]) * A.dxj[j];
-            double sp = NS == 2 ? ((1.0 - tt) * Pa[j] + tt * Pb[j]) * A.dxj[j] : 0.0;
-            double j0, j1;
-            small[sl] = z < LOS_ZS;
-            if (small[sl]) {
-                j0 = jser(0, z * z, 1.0);
-                j1 = jser(1, z * z, z);
+        for (int sl = 0; sl < SPW; ++sl) {
+            const int j = (wv + NW * sl) * WAVE + lane;
+            if (j < A.nxi) {
+                const double z = k * (eta0 - A.etaj[j]);
+                const double sx = ((1.0 - tt) * Sa[j] + tt * Sb[j]) * A.dxj[j];
+                const double sp = NS == 2 ? ((1.0 - tt) * Pa[j] + tt * Pb[j]) * A.dxj[j] : 0.0;
+                small[sl] = z < LOS_ZS;
+                const double2 j01 = los_j01(z);
+                const double j0 = j01.x, j1 = j01.y;
+                zv[sl] = z;
+                zl[sl] = z;
+                jp[sl] = j0; jc[sl] = j1;
+                iz[sl] = 1.0 / z;
+                sv[sl] = sx;
+                spv[sl] = sp;
+                mx[sl] = fmax(fabs(j0), fabs(j1));
+                alive[sl] = true;
             } else {
-                double sn, cs;
-                sincos(z, &sn, &cs);
-                j0 = sn / z;
-                j1 = (j0 - cs) / z;
+                jp[sl] = jc[sl] = iz[sl] = sv[sl] = spv[sl] = mx[sl] = 0.0;
+                zv[sl] = zl[sl] = 0.0;
+                alive[sl] = false;
+                small[sl] = false;
             }
-            zv[sl] = z;
-            zl[sl] = z;
-            jp[sl] = j0; jc[sl] = j1;
-            iz[sl] = 1.0 / z;
-            sv[sl] = s;
-            spv[sl] = sp;
-            mx[sl] = fmax(fabs(j0), fabs(j1));
-            alive[sl] = true;
-        } else {
-            jp[sl] = jc[sl] = iz[sl] = sv[sl] = spv[sl] = mx[sl] = 0.0;
-            zv[sl] = zl[sl] = 0.0;
-            alive[sl] = false;
-            small[sl] = false;
         }
-    }
-    long long nstep = 0;
-    const double Pw = A.cosmo->Pprim_A * pow(k / 0.05, A.cosmo->Pprim_n - 1.0) * dk / k;
-    for (int L0 = 0; L0 <= A.lmax; L0 += CH) {
-        double acc[CH], accp[NS == 2 ? CH : 1];
+        long long nstep = 0;
+        const double Pw = los_pw(A.cosmo->Pprim_A, A.cosmo->Pprim_n, k, dk);
+        for (int L0 = 0; L0 <= A.lmax; L0 += CH) {
+            double acc[CH], accp[NS == 2 ? CH : 1];
 #pragma unroll
-        for (int i = 0; i < CH; ++i) acc[i] = 0.0;
+            for (int i = 0; i < CH; ++i) acc[i] = 0.0;
 #pragma unroll
-        for (int i = 0; i < (NS == 2 ? CH : 1); ++i) accp[i] = 0.0;
-        bool any_live = false;
-        if (L0 == 0) {
+            for (int i = 0; i < (NS == 2 ? CH : 1); ++i) accp[i] = 0.0;
+            bool any_live = false;
+            if (L0 == 0) {
 #pragma unroll
-            for (int sl = 0; sl < LOS_SPW; ++sl) {
-                acc[0] += jp[sl] * sv[sl];
-                acc[1] += jc[sl] * sv[sl];
-                if constexpr (NS == 2) { accp[0] += jp[sl] * spv[sl]; accp[1] += jc[sl] * spv[sl]; }
-                if (__any(alive[sl] && small[sl]))
-                    los_slot_series<NS, CH, 2>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
-                                               mx[sl], alive[sl], 0, acc, accp, nstep);
-                else
-                    los_slot<NS, CH, 2>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], 0, acc, accp,
-                                        nstep);
-            }
-            any_live = true;
-        } else {
-#pragma unroll
-            for (int sl = 0; sl < LOS_SPW; ++sl) {
-                if (!__any(alive[sl])) continue;
+                for (int sl = 0; sl < SPW; ++sl) {
+                    acc[0] += jp[sl] * sv[sl];
+                    acc[1] += jc[sl] * sv[sl];
+                    if constexpr (NS == 2) { accp[0] += jp[sl] * spv[sl]; accp[1] += jc[sl] * spv[sl]; }
+                    if (__any(alive[sl] && small[sl]))
+                        los_slot_series<NS, CH, 2>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
+                                                   mx[sl], alive[sl], 0, acc, accp, nstep);
+                    else
+                        los_slot<NS, CH, 2>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], 0, acc, accp,
+                                            nstep);
+                }
                 any_live = true;
-                if (__any(alive[sl] && small[sl]))
-                    los_slot_series<NS, CH, 0>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
-                                               mx[sl], alive[sl], L0, acc, accp, nstep);
-                else
-                    los_slot<NS, CH, 0>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], L0, acc, accp,
-                                        nstep);
-            }
-        }
-        double Tv = 0.0, Ev = 0.0;
-        if (any_live) {  // wave-uniform
-            Tv = los_reduce<CH>(acc, lane);
-            if constexpr (NS == 2) Ev = los_reduce<CH>(accp, lane);
-        }
-        if (lane < CH) {
-            part[wv][0][lane] = Tv;
-            if (NS == 2) part[wv][NS - 1][lane] = Ev;
-        }
-        int live_block = __syncthreads_or(any_live ? 1 : 0);
-        if (wv == 0 && lane < CH) {
-            double Tt = part[0][0][lane], Et = NS == 2 ? part[0][NS - 1][lane] : 0.0;
+            } else {
 #pragma unroll
-            for (int w = 1; w < LOS_WAVES; ++w) {
-                Tt += part[w][0][lane];
-                if (NS == 2) Et += part[w][NS - 1][lane];
+                for (int sl = 0; sl < SPW; ++sl) {
+                    if (!__any(alive[sl])) continue;
+                    any_live = true;
+                    if (__any(alive[sl] && small[sl]))
+                        los_slot_series<NS, CH, 0>(jp[sl], jc[sl], iz[sl], zv[sl], zl[sl], small[sl], sv[sl], spv[sl],
+                                                   mx[sl], alive[sl], L0, acc, accp, nstep);
+                    else
+                        los_slot<NS, CH, 0>(jp[sl], jc[sl], iz[sl], sv[sl], spv[sl], mx[sl], alive[sl], L0, acc, accp,
+                                            nstep);
+                }
             }
-            int l = L0 + lane;
-            if (l <= A.lmax) {
-                int e = A.lidx[l];
-                if (e >= 0) {
+            double Tv = 0.0, Ev = 0.0;
+            if (any_live) {  // wave-uniform
+                Tv = los_reduce<CH>(acc, lane);
+                if constexpr (NS == 2) Ev = los_reduce<CH>(accp, lane);
+            }
+            if (lane < CH) {
+                part[wv][0][lane] = Tv;
+                if (NS == 2) part[wv][NS - 1][lane] = Ev;
+            }
+            const int live_block = __syncthreads_or(any_live ? 1 : 0);
+            if (wv == 0 && lane < CH) {
+                double Tt = part[0][0][lane], Et = NS == 2 ? part[0][NS - 1][lane] : 0.0;
+#pragma unroll
+                for (int w = 1; w < NW; ++w) {
+                    Tt += part[w][0][lane];
+                    if (NS == 2) Et += part[w][NS - 1][lane];
+                }
+                const int l = L0 + lane;
+                const int u = l <= A.lmax ? A.lidx[l] : -1;
+                if (u >= 0) {
                     if (NS == 1) {
-                        double v = A.spin ? Tt * spin_factor(l) : Tt;
-                        A.contrib[(size_t)e * nkm + ik] = v * v * Pw;
+                        const double v = A.spin ? Tt * spin_factor(l) : Tt;
+                        const double c = v * v * Pw;
+                        row[u] = first ? c : row[u] + c;
                     } else {
-                        double ep = Et * spin_factor(l);
-                        A.contrib[(size_t)e * nkm + ik] = Tt * Tt * Pw;
-                        A.contrib[((size_t)A.nell + e) * nkm + ik] = Tt * ep * Pw;
-                        A.contrib[((size_t)2 * A.nell + e) * nkm + ik] = ep * ep * Pw;
+                        const double ep = Et * spin_factor(l);
+                        const double c0 = Tt * Tt * Pw, c1 = Tt * ep * Pw, c2 = ep * ep * Pw;
+                        row[u] = first ? c0 : row[u] + c0;
+                        row[A.nu + u] = first ? c1 : row[A.nu + u] + c1;
+                        row[2 * A.nu + u] = first ? c2 : row[2 * A.nu + u] + c2;
+                    }
+                }
+                // every term is past its cut: the remaining T_ℓ are zero, so
+                // the block's first midpoint stores zeros for them (same lane
+                // ownership: ℓ ≡ lane mod 16) and later midpoints add nothing
+                if (!live_block && first) {
+                    for (int l2 = L0 + CH + lane; l2 <= A.lmax; l2 += CH) {
+                        const int u2 = A.lidx[l2];
+                        if (u2 < 0) continue;
+                        for (int o = 0; o < NOUT; ++o) row[o * A.nu + u2] = 0.0;
                     }
                 }
             }
+            __syncthreads();
+            if (!live_block) break;
         }
-        __syncthreads();
-        if (!live_block) {  // every term is past its cut: the remaining T_ℓ are zero
-            for (int l = L0 + CH + threadIdx.x; l <= A.lmax; l += blockDim.x) {
-                int e = A.lidx[l];
-                if (e < 0) continue;
-                for (int o = 0; o < (NS == 1 ? 1 : 3); ++o) A.contrib[((size_t)o * A.nell + e) * nkm + ik] = 0.0;
+        if (A.steps) {
+            const double ns = wave_sum((double)nstep);
+            if (lane == 0) nsum[wv] = ns;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0;
+                for (int w = 0; w < NW; ++w) t += nsum[w];
+                A.steps[ik] = (long long)t;
             }
-            break;
+            __syncthreads();
         }
-    }
-    if (A.steps) {
-        double ns = wave_sum((double)nstep);
-        if (lane == 0) part[wv][0][0] = ns;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0;
-            for (int w = 0; w < LOS_WAVES; ++w) t += part[w][0][0];
-            A.steps[ik] = (long long)t;
-        }
+        first = false;
     }
 }
 
-// C_ℓ = 4π Σ_k contrib (fixed order: per-thread strided sums, then a tree);
-// block o·nell + e reduces row e of output o into cl_o (skipped when NULL)
-__global__ void __launch_bounds__(256) k_los_reduce(const double* contrib, int nkm, int nell, double* cl0,
-                                                    double* cl1, double* cl2) {
-    __shared__ double red[256];
-    const int o = blockIdx.x / nell, e = blockIdx.x % nell;
+// C_ℓ = 4π Σ_blocks partial, in a fixed order: output o, positions
+// 64·blockIdx.x .. +63 (one lane each, reading column pmap[i]); wavefront w
+// sums the blocks w·nb/4 .. (w+1)·nb/4 - 1 in turn, then the four wavefront
+// sums are added in order.  Repeated ℓ read the same column and get the same bits.
+__global__ void __launch_bounds__(256) k_los_reduce(const double* partial, int nb, int nout, int nu, const int* pmap,
+                                                    int nell, double* cl0, double* cl1, double* cl2) {
+    __shared__ double red[4][WAVE];
+    const int o = blockIdx.y;
     double* cl = o == 0 ? cl0 : o == 1 ? cl1 : cl2;
-    if (o > 2 || !cl) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int i = blockIdx.x * WAVE + lane;
+    const int u = i < nell ? pmap[i] : 0;
+    const int b0 = (int)((long long)wv * nb / 4), b1 = (int)((long long)(wv + 1) * nb / 4);
     double s = 0.0;
-    for (int i = threadIdx.x; i < nkm; i += blockDim.x) s += contrib[((size_t)o * nell + e) * nkm + i];
-    red[threadIdx.x] = s;
+    if (i < nell)
+        for (int b = b0; b < b1; ++b) s += partial[((size_t)b * nout + o) * nu + u];
+    red[wv][lane] = s;
     __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) cl[e] = 4.0 * M_PI * red[0];
+    if (wv == 0 && i < nell && cl) cl[i] = 4.0 * M_PI * (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
 }
 
 // ------------------------------------------------------------------ host side
@@ -1996,11 +2068,14 @@ struct bolt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int n_cu = 0;
+    int los_spw = 0;    // LOS slots per wavefront pinned for this context (0: tuned or default)
+    int solver_sr = 0;  // scan rows per lane pinned for this context (0: tuned or default)
+
     int lds_max = 64 * 1024;
-    std::vector<CosmoSlot> cosmos;
+    std::deque<CosmoSlot> cosmos;  // stable slot addresses
     std::mutex mu;
     // launch descriptors and LOS work arrays of calls on `stream` (under mu)
-    GBuf c_modes, c_cfgs, c_cosmos, c_counter, c_lidx, c_eta, c_dx, c_contrib;
+    GBuf c_modes, c_cfgs, c_cosmos, c_counter, c_lidx, c_pmap, c_eta, c_dx, c_contrib;
     Server* server = nullptr;  // request server (bolt_serve_start), if running
     std::mutex server_mu;
 };
@@ -2023,6 +2098,7 @@ static CfgDev to_cfgdev(const bolt_hier_cfg* c) {
     return d;
 }
 
+// callers hold ctx->mu: uploads append to ctx->cosmos under it
 static int get_cosmo(bolt_ctx* ctx, int id, CosmoSlot** out) {
     if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
     if (id < 0 || id >= (int)ctx->cosmos.size() || !ctx->cosmos[id].used)
@@ -2052,6 +2128,14 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return set_error(BOLT_E_DEVICE, "hipGetDeviceProperties failed"); }
     c->n_cu = prop.multiProcessorCount;
+    if (const char* e = std::getenv("BOLT_LOS_SPW")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4) c->los_spw = v;
+    }
+    if (const char* e = std::getenv("BOLT_SOLVER_SR")) {
+        const int v = std::atoi(e);
+        if (v >= SR_MIN && v <= SR_MAX) c->solver_sr = v;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return set_error(BOLT_E_DEVICE, "hipStreamCreate failed");
@@ -2061,7 +2145,9 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
         int lds_max = (int)prop.sharedMemPerBlockOptin > 0 ? (int)prop.sharedMemPerBlockOptin
                                                             : (int)prop.sharedMemPerBlock;
         c->lds_max = lds_max - 1024;  // headroom for static __shared__
-        (void)hipFuncSetAttribute((const void*)k_boltsolve, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+        for (const void* f : {(const void*)k_boltsolve<5>, (const void*)k_boltsolve<6>, (const void*)k_boltsolve<7>,
+                              (const void*)k_boltsolve<8>})
+            (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipFuncSetAttribute((const void*)k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipFuncSetAttribute((const void*)k_ics, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipGetLastError();  // an unsupported opt-in must not surface at the next launch check
@@ -2093,12 +2179,71 @@ int bolt_ctx_destroy(bolt_ctx* ctx) {
         if (s.d_q) (void)hipFree(s.d_q);
         if (s.d_self) (void)hipFree(s.d_self);
     }
-    for (GBuf* g : {&ctx->c_modes, &ctx->c_cfgs, &ctx->c_cosmos, &ctx->c_counter, &ctx->c_lidx, &ctx->c_eta,
-                    &ctx->c_dx, &ctx->c_contrib})
+    for (GBuf* g : {&ctx->c_modes, &ctx->c_cfgs, &ctx->c_cosmos, &ctx->c_counter, &ctx->c_lidx, &ctx->c_pmap,
+                    &ctx->c_eta, &ctx->c_dx, &ctx->c_contrib})
         g->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BOLT_OK;
+}
+
+// The LDS workspace of one k-mode of the solver, laid out on the host by the
+// same accessors the kernel uses: every region as a [begin, end) byte range,
+// the dump slot (stores of rows past n), the workspace size the launch
+// reserves, and the farthest byte the branch-free reads past n (substitution
+// scans, banded LU tails) can touch.
+int bolt_ws_layout(int n, int nq, int cap, long long* regions, int* nregions, long long* dump_off,
+                   long long* total_bytes, long long* read_reach) {
+    if (n < 1 || nq < 1 || cap < 0 || !regions || !nregions || !dump_off || !total_bytes || !read_reach)
+        return set_error(BOLT_E_ARG, "bad workspace layout query");
+    const size_t tot = ws_bytes(n, nq);
+    std::vector<double> mem(tot / sizeof(double) + 1);
+    WS W;
+    ws_bind(W, mem.data(), n, nq);
+    const char* b0 = (const char*)W.b;
+    int cnt = 0;
+    auto add = [&](const void* p, size_t bytes) {
+        if (cnt < cap) {
+            regions[2 * cnt] = (const char*)p - b0;
+            regions[2 * cnt + 1] = (const char*)p - b0 + (long long)bytes;
+        }
+        ++cnt;
+    };
+    const size_t vb = sizeof(double) * (size_t)n, qb = sizeof(double) * (size_t)nq;
+    add(W.u(), vb); add(W.f(), vb); add(W.un(), vb); add(W.fn(), vb); add(W.arg(), vb); add(W.w(), vb);
+    for (int j = 0; j < 6; ++j) add(W.Z(j), vb);
+    add(W.cS(), vb); add(W.cP(), vb); add(W.cPi(), vb);
+    for (int j = 0; j < RC_N; ++j) add(W.rcp_(j), vb);
+    long long reach = 0;
+    long long rows = 0;
+    for (int sr = SR_MIN; sr <= SR_MAX; ++sr) {
+        const long long span = (long long)WAVE * sr;
+        rows = std::max(rows, ((n + span - 1) / span) * span);
+    }
+    auto reach_of = [&](const double* v) {  // scans read whole passes; the LU tail up to n + CB - 1
+        const long long r = (const char*)v - b0 + sizeof(double) * std::max<long long>(rows, n + CB);
+        reach = std::max(reach, r);
+    };
+    for (int st = 0; st < 5; ++st) {
+        const Bands B = W.bands(st);
+        add(B.l1, vb); add(B.d, vb); add(B.u1, vb);
+        add(B.l2c, sizeof(double) * CB); add(B.u2c, sizeof(double) * CB);
+        reach_of(B.l1); reach_of(B.d); reach_of(B.u1);
+    }
+    reach_of(W.w()); reach_of(W.cS()); reach_of(W.cP()); reach_of(W.cPi()); reach_of(W.arg());
+    add(W.q(), qb); add(W.df0(), qb); add(W.base(), qb);
+    add(W.qe(), qb); add(W.eqdf(), qb); add(W.sw(), qb); add(W.rw(), qb);
+    for (int st = 0; st < 5; ++st) add(W.qst_base(st), 4 * qb);
+    for (int st = 0; st < 5; ++st) add(W.xcs(st), sizeof(double) * NXC);
+    for (int st = 0; st < 6; ++st) add(W.ktab(st), sizeof(double) * (K_Q0 + 2 * nq));
+    add(W.red(), sizeof(double) * 12);  // stage_border's 4 x 3 functionals; block_sum uses the first 4
+    add(W.desc(), sizeof(int) * (size_t)n); add(W.kinds(), sizeof(int) * (size_t)n);
+    add(W.seg(), sizeof(int) * 2 * MAXSEG); add(W.misc(), sizeof(int) * 4);
+    *nregions = cnt;
+    *dump_off = (const char*)W.dump() - b0;
+    *total_bytes = (long long)tot;
+    *read_reach = reach;
+    return cnt <= cap ? BOLT_OK : set_error(BOLT_E_ARG, "region buffer too small");
 }
 
 int bolt_ctx_sync(bolt_ctx* ctx) {
@@ -2183,10 +2328,11 @@ int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_ta
 }
 
 int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
     CosmoSlot* s;
     int r = get_cosmo(ctx, cosmo_id, &s);
     if (r) return r;
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     (void)hipFree(s->d_coef); (void)hipFree(s->d_q); (void)hipFree(s->d_self);
@@ -2206,6 +2352,23 @@ struct SolveBufs {
     int cap_modes = 0, cap_cfgs = 0, cap_cosmos = 0;
 };
 
+// shapes chosen by bolt_autotune, shared by every context on a device: scan
+// rows per lane by (device, state length), LOS slots per wavefront by
+// (device, dense k, ℓ_max, two sources)
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, int> g_tuned_sr;
+static std::map<std::tuple<int, int, int, int>, int> g_tuned_spw;
+
+// rows per lane of the substitution scans for state length n: the value tuned
+// for n on this device, else SR_DEFAULT
+static int solver_sr(bolt_ctx* ctx, int n) {
+    if (!ctx) return SR_DEFAULT;
+    if (ctx->solver_sr > 0) return ctx->solver_sr;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned_sr.find(std::make_pair(ctx->device, n));
+    return it != g_tuned_sr.end() ? it->second : SR_DEFAULT;
+}
+
 // the integrator over a mode list whose modes may differ in cosmology and
 // hierarchy; modes are sorted by k, largest (most steps) first.  persistent:
 // a grid of at most one resident workgroup per slot pulls modes from the work
@@ -2214,7 +2377,8 @@ struct SolveBufs {
 static int launch_modes(bolt_ctx* ctx, hipStream_t stream, const SolveBufs* pre, bool persistent,
                         const std::vector<CosmoDev>& cosmos, const std::vector<CfgDev>& cfgs,
                         std::vector<ModeDesc>& md, int out_mode, const double* d_xsave, int nsave, double* d_out,
-                        double* d_out2, int* d_status, long long* d_counters, int n_stride, double* kernel_ms) {
+                        double* d_out2, int* d_status, long long* d_counters, int n_stride, double* kernel_ms,
+                        int force_sr = 0) {
     const int B = (int)md.size();
     std::stable_sort(md.begin(), md.end(), [](const ModeDesc& a, const ModeDesc& b) { return a.k > b.k; });
     size_t lds = 0;
@@ -2251,10 +2415,11 @@ static int launch_modes(bolt_ctx* ctx, hipStream_t stream, const SolveBufs* pre,
     A.status = d_status;
     A.counters = d_counters;
     A.n_state_stride = n_stride;
+    A.lds_doubles = (int)(lds / sizeof(double));
     int grid = B;
     if (persistent) {
         int per_cu = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve, NTHR, lds));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve<SR_DEFAULT>, NTHR, lds));
         if (per_cu < 1) per_cu = 1;
         grid = std::min(B, per_cu * ctx->n_cu);
     }
@@ -2262,7 +2427,14 @@ static int launch_modes(bolt_ctx* ctx, hipStream_t stream, const SolveBufs* pre,
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, stream));
-    hipLaunchKernelGGL(k_boltsolve, dim3(grid), dim3(NTHR), lds, stream, A);
+    int nmax = 0;
+    for (const CfgDev& c : cfgs) nmax = std::max(nmax, c.n);
+    switch (force_sr ? force_sr : solver_sr(ctx, nmax)) {
+        case 5: hipLaunchKernelGGL(k_boltsolve<5>, dim3(grid), dim3(NTHR), lds, stream, A); break;
+        case 7: hipLaunchKernelGGL(k_boltsolve<7>, dim3(grid), dim3(NTHR), lds, stream, A); break;
+        case 8: hipLaunchKernelGGL(k_boltsolve<8>, dim3(grid), dim3(NTHR), lds, stream, A); break;
+        default: hipLaunchKernelGGL(k_boltsolve<6>, dim3(grid), dim3(NTHR), lds, stream, A); break;
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -2291,14 +2463,14 @@ static int read_positive_k(bolt_ctx* ctx, const double* k, int B, bool on_device
 static int launch_solve(bolt_ctx* ctx, CosmoSlot* cs, const bolt_hier_cfg* cfg, const double* d_k_or_host, int B,
                         bool k_on_device, int out_mode, const double* d_xsave, int nsave, double* d_out,
                         int* d_status, long long* d_counters, int n_stride, double* kernel_ms,
-                        double* d_out2 = nullptr) {
+                        double* d_out2 = nullptr, int force_sr = 0) {
     std::vector<double> kh;
     int r = read_positive_k(ctx, d_k_or_host, B, k_on_device, kh);
     if (r) return r;
     std::vector<ModeDesc> md(B);
     for (int i = 0; i < B; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
     return launch_modes(ctx, ctx->stream, nullptr, true, {cs->host}, {to_cfgdev(cfg)}, md, out_mode, d_xsave, nsave,
-                        d_out, d_out2, d_status, d_counters, n_stride, kernel_ms);
+                        d_out, d_out2, d_status, d_counters, n_stride, kernel_ms, force_sr);
 }
 
 static void fill_stats(bolt_stats* st, const std::vector<long long>& ctr, int B, double ms) {
@@ -2317,6 +2489,8 @@ extern "C" {
 
 int bolt_rhs_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B, double x,
                    double* u, double* du, int mutate) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2324,7 +2498,6 @@ int bolt_rhs_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const 
     if (!k || !u || !du || B <= 0) return set_error(BOLT_E_ARG, "bolt_rhs_batch: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
     if (!(x >= cs->host.x0 && x <= cs->host.x1)) return set_error(BOLT_E_ARG, "x outside the table grid");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     CfgDev cd = to_cfgdev(cfg);
     std::vector<ModeDesc> md(B);
@@ -2349,13 +2522,14 @@ int bolt_rhs_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const 
 
 int bolt_initial_conditions(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
                             double* u0) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((r = check_cfg(cfg))) return r;
     if (!k || !u0 || B <= 0) return set_error(BOLT_E_ARG, "bolt_initial_conditions: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     CfgDev cd = to_cfgdev(cfg);
     std::vector<ModeDesc> md(B);
@@ -2377,6 +2551,8 @@ int bolt_initial_conditions(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cf
 
 int bolt_boltsolve_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
                          const double* x_save, int nsave, double* u_out, int* status, bolt_stats* stats) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2389,7 +2565,6 @@ int bolt_boltsolve_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, 
             return set_error(BOLT_E_ARG, "x_save outside [x0, x1]");
         if (i && !(x_save[i] >= x_save[i - 1])) return set_error(BOLT_E_ARG, "x_save must be sorted ascending");
     }
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int n = bolt_state_length(cfg);
     DBuf dk, dx, dout, dst, dctr;
@@ -2415,13 +2590,14 @@ int bolt_boltsolve_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, 
 
 int bolt_boltsolve_rsa_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
                              double* out, int* status, bolt_stats* stats) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((r = check_cfg(cfg))) return r;
     if (!k || !out || !status || B <= 0) return set_error(BOLT_E_ARG, "bolt_boltsolve_rsa_batch: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int n = bolt_state_length(cfg), nx = cs->host.nx;
     std::vector<double> xs(nx);
@@ -2459,6 +2635,8 @@ int bolt_boltsolve_rsa_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* c
 
 int bolt_source_grid_tp_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
                             double* d_ST, double* d_SP, int* d_status, int64_t* d_counters, float* ms) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2466,7 +2644,6 @@ int bolt_source_grid_tp_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cf
     if (!d_k || (!d_ST && !d_SP) || !d_status || nk <= 0)
         return set_error(BOLT_E_ARG, "bolt_source_grid_tp_dev: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     double dms = 0;
     const int mode = (d_ST ? 1 : 0) | (d_SP ? 2 : 0);
@@ -2484,13 +2661,14 @@ int bolt_source_grid_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, 
 
 int bolt_source_grid_tp(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double* S_T,
                         double* S_P, int* status, bolt_stats* stats) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((r = check_cfg(cfg))) return r;
     if (!k || (!S_T && !S_P) || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_source_grid: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int nx = cs->host.nx;
     DBuf dS, dP, dst, dctr;
@@ -2613,6 +2791,8 @@ int bolt_source_grid_batch_dev(bolt_ctx* ctx, int ncosmo, const int* cosmo_ids, 
 
 int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk, double x, double* pk,
               int* status, bolt_stats* stats) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2620,7 +2800,6 @@ int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const doubl
     if (!k || !pk || !status || nk <= 0) return set_error(BOLT_E_ARG, "bolt_plin: bad argument");
     if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
     if (!(x >= cs->host.x0 && x <= cs->host.x1)) return set_error(BOLT_E_ARG, "x outside the table grid");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int n = bolt_state_length(cfg);
     DBuf dx, dout, dst, dctr, dm, dc, dpk;
@@ -2661,25 +2840,53 @@ int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const doubl
 // d_tt (needs d_S), d_te (needs both), d_ee (needs d_SP), each may be NULL.
 struct LosBufs {
     int* lidx = nullptr;
+    int* pmap = nullptr;
     double* etaj = nullptr;
     double* dxj = nullptr;
-    double* contrib = nullptr;
-    size_t cap_lidx = 0, cap_nxi = 0, cap_contrib = 0;
+    double* partial = nullptr;
+    size_t cap_lidx = 0, cap_pmap = 0, cap_nxi = 0, cap_partial = 0;
+};
+
+template <int NS>
+static void launch_los(int spw, int grid, hipStream_t stream, const LosArgs& A) {
+    if (spw == 1) hipLaunchKernelGGL((k_los<NS, 1>), dim3(grid), dim3(LOS_SLOTS * WAVE), 0, stream, A);
+    else if (spw == 2) hipLaunchKernelGGL((k_los<NS, 2>), dim3(grid), dim3(LOS_SLOTS / 2 * WAVE), 0, stream, A);
+    else hipLaunchKernelGGL((k_los<NS, 4>), dim3(grid), dim3(LOS_SLOTS / 4 * WAVE), 0, stream, A);
+}
+
+// slots per wavefront of the LOS kernel for this shape: the value pinned on
+// the context (BOLT_LOS_SPW at context creation), else the default
+static int los_shape(bolt_ctx* ctx, int nkd, int lmax, bool two) {
+    if (!ctx) return LOS_SPW_DEFAULT;
+    if (ctx->los_spw > 0) return ctx->los_spw;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned_spw.find(std::make_tuple(ctx->device, nkd, lmax, two ? 1 : 0));
+    return it != g_tuned_spw.end() ? it->second : LOS_SPW_DEFAULT;
+}
+
+// host-side work space of one C_ℓ call that must not allocate: the ℓ -> column
+// table and the column map (caller's arrays), and two events for the timing
+struct ClHost {
+    int* lidx;
+    int* pmap;
+    int cap_lidx, cap_pmap;
+    hipEvent_t ev0, ev1;
 };
 
 static int cl_core(bolt_ctx* ctx, hipStream_t stream, const LosBufs* pre, const CosmoDev& host,
                    const CosmoDev* d_cosmo, const double* d_S, const double* d_SP, const double* d_k_src, int nk_src,
-                   const double* d_kgrid, int nkd, const std::vector<int>& ells, int bessel_mode, double* d_tt,
-                   double* d_te, double* d_ee, long long* d_steps, float* ms_out) {
+                   const double* d_kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* d_tt,
+                   double* d_te, double* d_ee, long long* d_steps, float* ms_out, const ClHost* hb = nullptr,
+                   int force_spw = 0) {
     if (bessel_mode != 0) return set_error(BOLT_E_ARG, "bessel_mode 1 (spline table) is not implemented on device");
     if ((d_tt && !d_S) || (d_te && (!d_S || !d_SP)) || (d_ee && !d_SP))
         return set_error(BOLT_E_ARG, "a requested spectrum lacks its source grid");
     if (!d_tt && !d_te && !d_ee) return set_error(BOLT_E_ARG, "no spectrum requested");
     // two sources share the recurrence only when a cross spectrum or both autos are wanted
     const bool two = d_te || (d_tt && d_ee);
-    int nell = (int)ells.size();
     int lmax = 0;
-    for (int l : ells) {
+    for (int e = 0; e < nell; ++e) {
+        const int l = ells[e];
         if (l < 0 || l > 20000) return set_error(BOLT_E_ARG, "ℓ out of range [0, 20000]");
         lmax = std::max(lmax, l);
     }
@@ -2692,24 +2899,39 @@ static int cl_core(bolt_ctx* ctx, hipStream_t stream, const LosBufs* pre, const 
     if (i0 < 0 || i0 >= nx - 1) return set_error(BOLT_E_ARG, "x grid has no node after x = -8");
     int nxi = nx - 1 - i0;
     if (nxi > LOS_SLOTS * WAVE) return set_error(BOLT_E_ARG, "too many integration nodes for the LOS kernel");
-    std::vector<int> lidx(lmax + 1, -1);
-    for (int e = 0; e < nell; ++e) lidx[ells[e]] = e;
+    // distinct ℓ, ascending: column u of the partial rows; pmap[i] = column of ells[i]
+    std::vector<int> lidx_v, pmap_v;
+    if (hb && (lmax + 1 > hb->cap_lidx || nell > hb->cap_pmap))
+        return set_error(BOLT_E_ARG, "C_l request exceeds the host work space");
+    if (!hb) { lidx_v.resize(lmax + 1); pmap_v.resize(nell); }
+    int* lidx = hb ? hb->lidx : lidx_v.data();
+    int* pmap = hb ? hb->pmap : pmap_v.data();
+    for (int l = 0; l <= lmax; ++l) lidx[l] = -1;
+    for (int e = 0; e < nell; ++e) lidx[ells[e]] = 0;
+    int nu = 0;
+    for (int l = 0; l <= lmax; ++l) if (lidx[l] >= 0) lidx[l] = nu++;
+    for (int e = 0; e < nell; ++e) pmap[e] = lidx[ells[e]];
     const int nout = two ? 3 : 1;
+    const int nkm = nkd - 1;
+    const int grid = std::min(LOS_GRID, nkm);
     LosBufs lb;
-    const size_t ncon = (size_t)nout * nell * (nkd - 1);
+    const size_t npart = (size_t)grid * nout * nu;
     if (pre) {
-        if ((size_t)(lmax + 1) > pre->cap_lidx || (size_t)nxi > pre->cap_nxi || ncon > pre->cap_contrib)
+        if ((size_t)(lmax + 1) > pre->cap_lidx || (size_t)nell > pre->cap_pmap || (size_t)nxi > pre->cap_nxi ||
+            npart > pre->cap_partial)
             return set_error(BOLT_E_ARG, "C_l request exceeds the preallocated LOS buffers");
         lb = *pre;
     } else {  // the context's own stream, under its lock: its cached buffers
         HIPCHK(ctx->c_lidx.ensure(sizeof(int) * (lmax + 1)));
+        HIPCHK(ctx->c_pmap.ensure(sizeof(int) * nell));
         HIPCHK(ctx->c_eta.ensure(sizeof(double) * nxi));
         HIPCHK(ctx->c_dx.ensure(sizeof(double) * nxi));
-        HIPCHK(ctx->c_contrib.ensure(sizeof(double) * ncon));
-        lb.lidx = ctx->c_lidx.as<int>(); lb.etaj = ctx->c_eta.as<double>(); lb.dxj = ctx->c_dx.as<double>();
-        lb.contrib = ctx->c_contrib.as<double>();
+        HIPCHK(ctx->c_contrib.ensure(sizeof(double) * npart));
+        lb.lidx = ctx->c_lidx.as<int>(); lb.pmap = ctx->c_pmap.as<int>(); lb.etaj = ctx->c_eta.as<double>();
+        lb.dxj = ctx->c_dx.as<double>(); lb.partial = ctx->c_contrib.as<double>();
     }
-    HIPCHK(hipMemcpyAsync(lb.lidx, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(lb.lidx, lidx, sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(lb.pmap, pmap, sizeof(int) * nell, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, stream, d_cosmo, i0, nxi, lb.etaj, lb.dxj);
     HIPCHK(hipGetLastError());
     LosArgs A;
@@ -2717,28 +2939,32 @@ static int cl_core(bolt_ctx* ctx, hipStream_t stream, const LosBufs* pre, const 
     A.S = two ? d_S : (d_tt ? d_S : d_SP);
     A.SP = two ? d_SP : nullptr;
     A.spin = (!two && !d_tt) ? 1 : 0;
-    A.lidx = lb.lidx; A.lmax = lmax; A.nell = nell; A.etaj = lb.etaj; A.dxj = lb.dxj;
-    A.i0 = i0; A.nxi = nxi; A.contrib = lb.contrib; A.steps = d_steps;
+    A.lidx = lb.lidx; A.lmax = lmax; A.nu = nu; A.etaj = lb.etaj; A.dxj = lb.dxj;
+    A.i0 = i0; A.nxi = nxi; A.partial = lb.partial; A.steps = d_steps;
     hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    int nkm = nkd - 1;
+    if (hb) {
+        e0 = hb->ev0; e1 = hb->ev1;
+    } else {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+    }
     HIPCHK(hipEventRecord(e0, stream));
-    if (two) hipLaunchKernelGGL(k_los<2>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, stream, A);
-    else hipLaunchKernelGGL(k_los<1>, dim3(nkm), dim3(LOS_WAVES * WAVE), 0, stream, A);
+    const int spw = force_spw ? force_spw : los_shape(ctx, nkd, lmax, two);
+    if (two) launch_los<2>(spw, grid, stream, A);
+    else launch_los<1>(spw, grid, stream, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, stream));
-    if (two)
-        hipLaunchKernelGGL(k_los_reduce, dim3(3 * nell), dim3(256), 0, stream, lb.contrib, nkm, nell, d_tt, d_te, d_ee);
-    else
-        hipLaunchKernelGGL(k_los_reduce, dim3(nell), dim3(256), 0, stream, lb.contrib, nkm, nell, d_tt ? d_tt : d_ee,
-                           (double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL(k_los_reduce, dim3((nell + WAVE - 1) / WAVE, nout), dim3(256), 0, stream, lb.partial, grid, nout,
+                       nu, lb.pmap, nell, two ? d_tt : (d_tt ? d_tt : d_ee), two ? d_te : (double*)nullptr,
+                       two ? d_ee : (double*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (!hb) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
     if (ms_out) *ms_out = ms;
     HIPCHK(hipStreamSynchronize(stream));
     return BOLT_OK;
@@ -2748,6 +2974,8 @@ extern "C" {
 
 int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src, int nk_src, const double* kgrid,
               int nkd, const int* ells, int nell, int bessel_mode, double* cl) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2755,7 +2983,6 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
         return set_error(BOLT_E_ARG, "bolt_cltt: bad argument");
     for (int i = 1; i < nk_src; ++i) if (!(k_src[i] > k_src[i - 1])) return set_error(BOLT_E_ARG, "k_src must increase");
     for (int i = 1; i < nkd; ++i) if (!(kgrid[i] > kgrid[i - 1])) return set_error(BOLT_E_ARG, "kgrid must increase");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int nx = cs->host.nx;
     DBuf dS, dks, dkg, dcl;
@@ -2768,7 +2995,7 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
     HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
     std::vector<int> ev(ells, ells + nell);
     if ((r = cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, dS.as<double>(), nullptr, dks.as<double>(),
-                     nk_src, dkg.as<double>(), nkd, ev,
+                     nk_src, dkg.as<double>(), nkd, ev.data(), (int)ev.size(),
                      bessel_mode, dcl.as<double>(), nullptr, nullptr, nullptr, nullptr)))
         return r;
     HIPCHK(hipMemcpy(cl, dcl.p, sizeof(double) * nell, hipMemcpyDeviceToHost));
@@ -2780,22 +3007,236 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
 int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
                        const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl, int64_t* d_steps,
                        float* ms) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if (!d_S || !d_k_src || !d_kgrid || !ells || !d_cl || nk_src < 2 || nkd < 2 || nell < 1)
         return set_error(BOLT_E_ARG, "bolt_cltt_dev: bad argument");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<int> ev(ells, ells + nell);
-    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd, ev,
-                   0, d_cl, nullptr, nullptr,
+    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd,
+                   ev.data(), (int)ev.size(), 0, d_cl, nullptr, nullptr,
                    (long long*)d_steps, ms);
+}
+
+// The project's autotuner: times every solver shape (rows per lane of the
+// substitution scans, SR = 5..8) on the caller's k grid and hierarchy, and
+// every LOS shape (x slots per wavefront, 1/2/4) on the resulting source grid,
+// dense k grid and ℓ range; keeps the fastest of each on the context for that
+// state length and that (dense k, ℓ_max).  Each shape gives bitwise
+// reproducible results; different shapes add in different orders.
+int bolt_autotune(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk,
+                  const double* kgrid, int nkd, int lmax, bolt_tune_report* rep) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!k || nk < 2 || !rep || (kgrid && (nkd < 2 || lmax < 2 || lmax > 20000)))
+        return set_error(BOLT_E_ARG, "bolt_autotune: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    for (int i = 1; i < nk; ++i) if (!(k[i] > k[i - 1])) return set_error(BOLT_E_ARG, "k must increase");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::memset(rep, 0, sizeof(*rep));
+    const int nx = cs->host.nx;
+    DBuf dk, dS, dst, dctr;
+    HIPCHK(dk.alloc(sizeof(double) * nk));
+    HIPCHK(dS.alloc(sizeof(double) * (size_t)nk * nx));
+    HIPCHK(dst.alloc(sizeof(int) * nk));
+    HIPCHK(dctr.alloc(sizeof(long long) * 4 * nk));
+    HIPCHK(hipMemcpyAsync(dk.p, k, sizeof(double) * nk, hipMemcpyHostToDevice, ctx->stream));
+    float best = 0;
+    for (int sr = SR_MIN; sr <= SR_MAX; ++sr) {
+        double ms = 0;
+        for (int rep_i = 0; rep_i < 2; ++rep_i)  // the first launch of a shape loads its code
+            if ((r = launch_solve(ctx, cs, cfg, dk.as<double>(), nk, true, 1, nullptr, 0, dS.as<double>(),
+                                  dst.as<int>(), dctr.as<long long>(), 0, &ms, nullptr, sr)))
+                return r;
+        rep->solver_ms[sr - SR_MIN] = (float)ms;
+        if (sr == SR_MIN || ms < best) { best = (float)ms; rep->solver_sr = sr; }
+    }
+    {
+        std::lock_guard<std::mutex> lt(g_tune_mu);
+        g_tuned_sr[std::make_pair(ctx->device, bolt_state_length(cfg))] = rep->solver_sr;
+    }
+    if (kgrid) {
+        DBuf dkg, dcl;
+        HIPCHK(dkg.alloc(sizeof(double) * nkd));
+        HIPCHK(dcl.alloc(sizeof(double) * (lmax - 1)));
+        HIPCHK(hipMemcpyAsync(dkg.p, kgrid, sizeof(double) * nkd, hipMemcpyHostToDevice, ctx->stream));
+        std::vector<int> ells(lmax - 1);
+        for (int l = 2; l <= lmax; ++l) ells[l - 2] = l;
+        const int spws[3] = {1, 2, 4};
+        for (int c = 0; c < 3; ++c) {
+            float ms = 0;
+            for (int rep_i = 0; rep_i < 2; ++rep_i)
+                if ((r = cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, dS.as<double>(), nullptr,
+                                 dk.as<double>(), nk, dkg.as<double>(), nkd, ells.data(), (int)ells.size(), 0,
+                                 dcl.as<double>(), nullptr, nullptr, nullptr, &ms, nullptr, spws[c])))
+                    return r;
+            rep->los_ms[c] = ms;
+            if (c == 0 || ms < best) { best = ms; rep->los_spw = spws[c]; }
+        }
+        std::lock_guard<std::mutex> lt(g_tune_mu);
+        g_tuned_spw[std::make_tuple(ctx->device, nkd, lmax, 0)] = rep->los_spw;
+    }
+    return BOLT_OK;
+}
+
+}  // extern "C"
+
+// the solver's work counter, zeroed by a kernel (a graph replays it; memset
+// nodes are avoided in captured graphs)
+__global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// A captured spectrum: source_grid + cltt for one cosmology, hierarchy and pair
+// of k grids, with its descriptors and LOS work space owned by the graph
+struct bolt_graph {
+    bolt_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> allocs;
+    ~bolt_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) allocs.push_back((void*)*p);
+        return e;
+    }
+};
+
+extern "C" {
+
+int bolt_graph_spectrum_create(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                               const double* d_kgrid, int nkd, const int* ells, int nell, double* d_S,
+                               int* d_status, int64_t* d_counters, double* d_cl, bolt_graph** out) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if ((r = check_cfg(cfg))) return r;
+    if (!d_k || nk < 2 || !d_kgrid || nkd < 2 || !ells || nell < 1 || !d_S || !d_status || !d_cl || !out)
+        return set_error(BOLT_E_ARG, "bolt_graph_spectrum_create: bad argument");
+    if (cfg->nq != cs->host.nq) return set_error(BOLT_E_ARG, "cfg.nq differs from the tables' nq");
+    HIPCHK(hipSetDevice(ctx->device));
+    const CosmoDev& host = cs->host;
+    // the solver launch, as launch_modes sets it up
+    std::vector<double> kh;
+    if ((r = read_positive_k(ctx, d_k, nk, true, kh))) return r;
+    std::vector<ModeDesc> md(nk);
+    for (int i = 0; i < nk; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
+    std::stable_sort(md.begin(), md.end(), [](const ModeDesc& a, const ModeDesc& b) { return a.k > b.k; });
+    const CfgDev cd = to_cfgdev(cfg);
+    const size_t lds = ws_bytes(cd.n, cd.nq);
+    if ((long long)lds > ctx->lds_max) return set_error(BOLT_E_ARG, "hierarchy too large for one workgroup's LDS");
+    // the LOS launch, as cl_core sets it up
+    int lmax = 0;
+    for (int e = 0; e < nell; ++e) {
+        if (ells[e] < 0 || ells[e] > 20000) return set_error(BOLT_E_ARG, "ℓ out of range [0, 20000]");
+        lmax = std::max(lmax, ells[e]);
+    }
+    const int nx = host.nx;
+    int i0 = -1;
+    for (int j = 0; j < nx; ++j)
+        if ((host.x0 * (nx - 1 - j) + host.x1 * j) / (nx - 1) > -8.0) { i0 = j; break; }
+    if (i0 < 0 || i0 >= nx - 1) return set_error(BOLT_E_ARG, "x grid has no node after x = -8");
+    const int nxi = nx - 1 - i0;
+    if (nxi > LOS_SLOTS * WAVE) return set_error(BOLT_E_ARG, "too many integration nodes for the LOS kernel");
+    std::vector<int> lidx(lmax + 1, -1), pmap(nell);
+    for (int e = 0; e < nell; ++e) lidx[ells[e]] = 0;
+    int nu = 0;
+    for (int l = 0; l <= lmax; ++l) if (lidx[l] >= 0) lidx[l] = nu++;
+    for (int e = 0; e < nell; ++e) pmap[e] = lidx[ells[e]];
+    const int nkm = nkd - 1, lgrid = std::min(LOS_GRID, nkm);
+
+    std::unique_ptr<bolt_graph> g(new bolt_graph);
+    g->ctx = ctx;
+    ModeDesc* d_md; CfgDev* d_cfg; int* d_ctr; int* d_lidx; int* d_pmap; double *d_eta, *d_dx, *d_part;
+    HIPCHK(g->alloc(&d_md, nk));
+    HIPCHK(g->alloc(&d_cfg, 1));
+    HIPCHK(g->alloc(&d_ctr, 1));
+    HIPCHK(g->alloc(&d_lidx, lmax + 1));
+    HIPCHK(g->alloc(&d_pmap, nell));
+    HIPCHK(g->alloc(&d_eta, nxi));
+    HIPCHK(g->alloc(&d_dx, nxi));
+    HIPCHK(g->alloc(&d_part, (size_t)lgrid * nu));
+    HIPCHK(hipMemcpyAsync(d_md, md.data(), sizeof(ModeDesc) * nk, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_cfg, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_lidx, lidx.data(), sizeof(int) * (lmax + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_pmap, pmap.data(), sizeof(int) * nell, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    SolveArgs A;
+    A.cosmos = cs->d_self; A.cfgs = d_cfg; A.modes = d_md; A.nmodes = nk; A.work_counter = d_ctr;
+    A.out_mode = 1; A.x_save = nullptr; A.nsave = 0; A.out = d_S; A.out2 = nullptr; A.status = d_status;
+    A.counters = (long long*)d_counters; A.n_state_stride = 0; A.lds_doubles = (int)(lds / sizeof(double));
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_boltsolve<SR_DEFAULT>, NTHR, lds));
+    const int sgrid = std::min(nk, std::max(per_cu, 1) * ctx->n_cu);
+    const int sr = solver_sr(ctx, cd.n);
+    LosArgs L;
+    L.cosmo = cs->d_self; L.S = d_S; L.SP = nullptr; L.k_src = d_k; L.nk_src = nk; L.kgrid = d_kgrid; L.nkd = nkd;
+    L.lidx = d_lidx; L.lmax = lmax; L.nu = nu; L.etaj = d_eta; L.dxj = d_dx; L.i0 = i0; L.nxi = nxi; L.spin = 0;
+    L.partial = d_part; L.steps = nullptr;
+    const int spw = los_shape(ctx, nkd, lmax, false);
+    // capture: counter reset, solver, LOS node table, LOS, reduction
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, ctx->stream, d_ctr);
+    switch (sr) {
+        case 5: hipLaunchKernelGGL(k_boltsolve<5>, dim3(sgrid), dim3(NTHR), lds, ctx->stream, A); break;
+        case 7: hipLaunchKernelGGL(k_boltsolve<7>, dim3(sgrid), dim3(NTHR), lds, ctx->stream, A); break;
+        case 8: hipLaunchKernelGGL(k_boltsolve<8>, dim3(sgrid), dim3(NTHR), lds, ctx->stream, A); break;
+        default: hipLaunchKernelGGL(k_boltsolve<6>, dim3(sgrid), dim3(NTHR), lds, ctx->stream, A); break;
+    }
+    hipLaunchKernelGGL(k_los_prep, dim3((nxi + 255) / 256), dim3(256), 0, ctx->stream, cs->d_self, i0, nxi, d_eta,
+                       d_dx);
+    launch_los<1>(spw, lgrid, ctx->stream, L);
+    hipLaunchKernelGGL(k_los_reduce, dim3((nell + WAVE - 1) / WAVE, 1), dim3(256), 0, ctx->stream, d_part, lgrid, 1,
+                       nu, d_pmap, nell, d_cl, (double*)nullptr, (double*)nullptr);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (le != hipSuccess || ce != hipSuccess)
+        return set_error(BOLT_E_DEVICE, std::string("capturing the spectrum graph failed: ") +
+                                            hipGetErrorString(le != hipSuccess ? le : ce));
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return BOLT_OK;
+}
+
+int bolt_graph_launch(bolt_graph* g) {
+    if (!g || !g->ctx) return set_error(BOLT_E_ARG, "bolt_graph_launch: NULL graph");
+    std::lock_guard<std::mutex> lk(g->ctx->mu);
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipGraphLaunch(g->exec, g->ctx->stream));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    return BOLT_OK;
+}
+
+int bolt_graph_destroy(bolt_graph* g) {
+    if (!g) return BOLT_OK;
+    if (g->ctx) {
+        std::lock_guard<std::mutex> lk(g->ctx->mu);
+        (void)hipSetDevice(g->ctx->device);
+        (void)hipStreamSynchronize(g->ctx->stream);
+    }
+    delete g;
+    return BOLT_OK;
 }
 
 int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, const double* k_src, int nk_src,
             const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode, double* cltt, double* clte,
             double* clee) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
@@ -2803,7 +3244,6 @@ int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, c
         return set_error(BOLT_E_ARG, "bolt_cl: bad argument");
     for (int i = 1; i < nk_src; ++i) if (!(k_src[i] > k_src[i - 1])) return set_error(BOLT_E_ARG, "k_src must increase");
     for (int i = 1; i < nkd; ++i) if (!(kgrid[i] > kgrid[i - 1])) return set_error(BOLT_E_ARG, "kgrid must increase");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     int nx = cs->host.nx;
     size_t sb = sizeof(double) * (size_t)nk_src * nx;
@@ -2818,7 +3258,7 @@ int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, c
     std::vector<int> ev(ells, ells + nell);
     double* d = dcl.as<double>();
     if ((r = cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, dS.as<double>(), dP.as<double>(),
-                     dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev,
+                     dks.as<double>(), nk_src, dkg.as<double>(), nkd, ev.data(), (int)ev.size(),
                      bessel_mode, cltt ? d : nullptr, clte ? d + nell : nullptr, clee ? d + 2 * nell : nullptr,
                      nullptr, nullptr)))
         return r;
@@ -2831,16 +3271,17 @@ int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, c
 int bolt_cl_dev(bolt_ctx* ctx, int cosmo_id, const double* d_ST, const double* d_SP, const double* d_k_src,
                 int nk_src, const double* d_kgrid, int nkd, const int* ells, int nell, double* d_tt, double* d_te,
                 double* d_ee, int64_t* d_steps, float* ms) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
     CosmoSlot* cs;
     int r = get_cosmo(ctx, cosmo_id, &cs);
     if (r) return r;
     if ((!d_ST && !d_SP) || !d_k_src || !d_kgrid || !ells || nk_src < 2 || nkd < 2 || nell < 1)
         return set_error(BOLT_E_ARG, "bolt_cl_dev: bad argument");
-    std::lock_guard<std::mutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<int> ev(ells, ells + nell);
-    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_ST, d_SP, d_k_src, nk_src, d_kgrid, nkd, ev, 0,
-                   d_tt, d_te, d_ee, (long long*)d_steps,
+    return cl_core(ctx, ctx->stream, nullptr, cs->host, cs->d_self, d_ST, d_SP, d_k_src, nk_src, d_kgrid, nkd,
+                   ev.data(), (int)ev.size(), 0, d_tt, d_te, d_ee, (long long*)d_steps,
                    ms);
 }
 
@@ -2937,11 +3378,12 @@ static int slot_create(ServeSlot& s) {
     HIPCHK(slot_alloc(s, &s.sb.counter, 1));
     s.sb.cap_modes = SV_MAX_NK; s.sb.cap_cfgs = 1; s.sb.cap_cosmos = 1;
     HIPCHK(slot_alloc(s, &s.lb.lidx, SV_MAX_ELL + 1));
+    HIPCHK(slot_alloc(s, &s.lb.pmap, SV_MAX_ELL + 1));
     HIPCHK(slot_alloc(s, &s.lb.etaj, LOS_SLOTS * WAVE));
     HIPCHK(slot_alloc(s, &s.lb.dxj, LOS_SLOTS * WAVE));
-    HIPCHK(slot_alloc(s, &s.lb.contrib, (size_t)(SV_MAX_ELL - 1) * (SV_MAX_NKD - 1)));
-    s.lb.cap_lidx = SV_MAX_ELL + 1; s.lb.cap_nxi = LOS_SLOTS * WAVE;
-    s.lb.cap_contrib = (size_t)(SV_MAX_ELL - 1) * (SV_MAX_NKD - 1);
+    HIPCHK(slot_alloc(s, &s.lb.partial, (size_t)LOS_GRID * (SV_MAX_ELL + 1)));
+    s.lb.cap_lidx = SV_MAX_ELL + 1; s.lb.cap_pmap = SV_MAX_ELL + 1; s.lb.cap_nxi = LOS_SLOTS * WAVE;
+    s.lb.cap_partial = (size_t)LOS_GRID * (SV_MAX_ELL + 1);
     return BOLT_OK;
 }
 
@@ -3046,7 +3488,8 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
         std::vector<int> ells(rq.ell_max - 1);
         for (int l = 2; l <= rq.ell_max; ++l) ells[l - 2] = l;
         float ms2 = 0;
-        if ((r = cl_core(ctx, st, &sl.lb, C, sl.d_self, sl.d_S, nullptr, sl.d_k, nk, sl.d_dense, nkd, ells, 0, sl.d_cl,
+        if ((r = cl_core(ctx, st, &sl.lb, C, sl.d_self, sl.d_S, nullptr, sl.d_k, nk, sl.d_dense, nkd, ells.data(),
+                         (int)ells.size(), 0, sl.d_cl,
                          nullptr, nullptr, nullptr, &ms2)))
             return r;
         ms += ms2;

@@ -158,6 +158,15 @@ int bolt_ctx_create(int device, bolt_ctx** out);
 int bolt_ctx_destroy(bolt_ctx* ctx);
 int bolt_ctx_sync(bolt_ctx* ctx);
 
+/* Layout introspection (no reference counterpart; host only, no GPU needed):
+ * the solver's per-k-mode LDS workspace for state length n and nq momentum
+ * bins, as `*nregions` [begin, end) byte ranges written to regions[2*i..] (at
+ * most cap of them), the byte offset of the slot that absorbs stores of rows
+ * past n, the bytes a launch reserves per workgroup, and the farthest byte
+ * the solver's branch-free reads past n may touch.  For tests of the layout. */
+int bolt_ws_layout(int n, int nq, int cap, long long* regions, int* nregions, long long* dump_off,
+                   long long* total_bytes, long long* read_reach);
+
 /* Upload one cosmology's parameters and tables; returns its id. */
 int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_tables* t,
                       int* cosmo_id);
@@ -223,6 +232,43 @@ int bolt_cltt(bolt_ctx* ctx, int cosmo_id, const double* S, const double* k_src,
 int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
                   const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
                   int64_t* d_steps, float* ms);
+
+/* Autotuner (no reference counterpart: Bolt's only tunable is the @qthreads
+ * loop over k, spectra.jl:10).  Times every shape of the solver -- rows per
+ * lane of its substitution scans, SR = 5..8 -- on source_grid over the host
+ * array k[nk] (increasing) with hierarchy cfg, and, when kgrid is not NULL,
+ * every shape of the LOS kernel -- x slots per wavefront, 1, 2, 4 -- on cltt
+ * for ℓ = 2..lmax over kgrid[nkd] and the source grid just computed.  The
+ * fastest shapes are kept for the context's device, that state length and
+ * that (nkd, lmax), and used by later calls of every context on the device;
+ * the timings come back in *rep.
+ * A given shape gives bitwise reproducible results; two shapes agree to
+ * roundoff (their sums run in different orders). */
+typedef struct bolt_tune_report {
+    int solver_sr;      /* chosen rows per lane (5..8) */
+    float solver_ms[4]; /* one source_grid launch per SR = 5, 6, 7, 8 */
+    int los_spw;        /* chosen x slots per wavefront (1, 2, 4); 0 when kgrid is NULL */
+    float los_ms[3];    /* one LOS launch per SPW = 1, 2, 4 */
+} bolt_tune_report;
+int bolt_autotune(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk,
+                  const double* kgrid, int nkd, int lmax, bolt_tune_report* rep);
+
+/* A whole spectrum as one HIP graph (no reference counterpart: Bolt calls
+ * source_grid then cltt, spectra.jl:6-23, 84-96).  Captures, on the context's
+ * stream: the solver's work-counter reset (a kernel, not a memset), the
+ * persistent source_grid launch over d_k[nk] with hierarchy cfg into
+ * d_S[nk][nx] (status d_status[nk], counters d_counters[nk][4] or NULL), and the
+ * LOS for C_ℓ^TT at ells[nell] over d_kgrid[nkd] into d_cl[nell], with the
+ * solver and LOS shapes in force at creation.  The graph owns its descriptors
+ * and LOS work space; the caller's device buffers must outlive it.
+ * bolt_graph_launch replays it and waits; each replay recomputes the spectrum
+ * bit for bit as source_grid_dev + cltt_dev do. */
+typedef struct bolt_graph bolt_graph;
+int bolt_graph_spectrum_create(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                               const double* d_kgrid, int nkd, const int* ells, int nell, double* d_S,
+                               int* d_status, int64_t* d_counters, double* d_cl, bolt_graph** out);
+int bolt_graph_launch(bolt_graph* g);
+int bolt_graph_destroy(bolt_graph* g);
 
 /* source_grid for many cosmologies at once: the @qthreads loop over k_grid
  * (spectra.jl:10) widened to every (cosmology, k) pair of the batch, run by one

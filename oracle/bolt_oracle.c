@@ -28,6 +28,17 @@
  * The linear solve is Gaussian elimination with partial pivoting on W
  * (symmetrically permuted so hierarchy tails come first), skipping exact-zero
  * multipliers and pivot-row entries.
+ *
+ * What pins each function (tests/test_oracle.py, tests/test_oracle_plin.py):
+ *   orc_rhs, orc_jacobian       exact Jacobian/linearity identities; CLASS Φ, δ_b fixture 1e-3
+ *   orc_initial_conditions      closed forms (Φ, δ = 3Θ₀, v = −3kΘ₁), super-horizon stasis
+ *   integrate (KenCarp4)        tableau order conditions in rational arithmetic; CLASS fixture
+ *   orc_source*, orc_cl*        CAMB C_ℓ fixture 1.1e-1; single-shell closed form; TE/EE spin factors
+ *   orc_sph_bessel, tables      scipy/mpmath j_ℓ 1e-12; scipy natural spline
+ *   orc_rho_sigma, orc_theta    massless closed form 4·(7/8)(π²/15)T_ν⁴; Poisson constraint
+ *   orc_plin, orc_omega_nu      CAMB P(k) fixture 5e-3; Poisson constraint 1e-4; Ω_ν h² = Σm_ν/93.14 eV
+ * orc_plin_planted / orc_omega_nu_planted exist only so that the tests can
+ * show each of those pins failing on a planted mistake.
  */
 #include "bolt_oracle.h"
 

@@ -4,6 +4,10 @@ import sys
 import numpy as np
 import pytest
 
+# torch ships its own HIP runtime; loading it before libbolt_hip.so makes both
+# use that one (loaded the other way round, torch finds no GPU)
+import torch  # noqa: F401
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "bolt.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
     if p not in sys.path:
