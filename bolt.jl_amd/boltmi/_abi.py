@@ -113,6 +113,8 @@ SIGNATURES = {
                                         _VP, ctypes.POINTER(_VP)]),
     "bolt_graph_launch": (_I, [_VP]),
     "bolt_graph_destroy": (_I, [_VP]),
+    "bolt_cltt_dev_noalloc": (_I, [_VP, _I, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP,
+                                   ctypes.POINTER(ctypes.c_float)]),
     "bolt_plin": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PI, ctypes.POINTER(StatsC)]),
     "bolt_serve_start": (_I, [_VP, _I, _D]),
     "bolt_serve_slots": (_I, [_VP]),

@@ -2020,6 +2020,21 @@ __global__ void __launch_bounds__(256) k_los_reduce(const double* partial, int n
     if (wv == 0 && i < nell && cl) cl[i] = 4.0 * M_PI * (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
 }
 
+// ------------------------------------------ no-allocation C_ℓ^TT work space
+// Device work space of bolt_cltt_dev_noalloc, for callers that cannot allocate
+// (code under HIP graph capture, serving loops, harnesses that forbid
+// hipMalloc): the LOS partial rows for ℓ <= NOALLOC_MAX_L, η and Δx at the
+// integration nodes, the ℓ -> column table and the column map.  One copy per
+// GPU (a file-scope device array, 12.3 MB), held until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns.
+#define NOALLOC_MAX_L 3000
+#define NOALLOC_MAX_NELL 3001
+constexpr size_t NOALLOC_PART = (size_t)LOS_GRID * (NOALLOC_MAX_L + 1);  // doubles
+constexpr size_t NOALLOC_NODES = (size_t)LOS_SLOTS * WAVE;               // doubles, for each of η and Δx
+constexpr size_t NOALLOC_INTS = (size_t)(NOALLOC_MAX_L + 1) + NOALLOC_MAX_NELL;
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_PART + 2 * NOALLOC_NODES + (NOALLOC_INTS + 1) / 2;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
 // ------------------------------------------------------------------ host side
 #define HIPCHK(expr)                                                                                   \
     do {                                                                                               \
@@ -2071,6 +2086,7 @@ struct bolt_ctx {
     int los_spw = 0;    // LOS slots per wavefront pinned for this context (0: tuned or default)
     int solver_sr = 0;  // scan rows per lane pinned for this context (0: tuned or default)
 
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing of the no-allocation entry points
     int lds_max = 64 * 1024;
     std::deque<CosmoSlot> cosmos;  // stable slot addresses
     std::mutex mu;
@@ -2096,6 +2112,23 @@ static CfgDev to_cfgdev(const bolt_hier_cfg* c) {
     d.n = 2 * (d.ng + 1) + (d.nn + 1) + (d.nm + 1) * d.nq + 5;
     d.reltol = c->reltol; d.abstol = c->abstol; d.max_steps = c->max_steps;
     return d;
+}
+
+// device address of `scratch` on each GPU, set once per device with it current
+#define SCRATCH_MAX_DEV 64
+static std::mutex g_scratch_mu;
+static double* g_scratch[SCRATCH_MAX_DEV] = {};
+
+static int scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEV) return set_error(BOLT_E_ARG, "device index beyond the scratch table");
+    if (g_scratch[device]) return BOLT_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(BOLT_E_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch[device] = (double*)p;
+    return BOLT_OK;
 }
 
 // callers hold ctx->mu: uploads append to ctx->cosmos under it
@@ -2140,6 +2173,20 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
         delete c;
         return set_error(BOLT_E_DEVICE, "hipStreamCreate failed");
     }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_error(BOLT_E_DEVICE, "hipEventCreate failed");
+    }
+    {
+        const int r = scratch_init(device);
+        if (r) {
+            (void)hipEventDestroy(c->ev0); (void)hipEventDestroy(c->ev1);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return r;
+        }
+    }
     {
         // opt in to the device's full LDS for the one-wavefront-per-mode kernels
         int lds_max = (int)prop.sharedMemPerBlockOptin > 0 ? (int)prop.sharedMemPerBlockOptin
@@ -2182,6 +2229,8 @@ int bolt_ctx_destroy(bolt_ctx* ctx) {
     for (GBuf* g : {&ctx->c_modes, &ctx->c_cfgs, &ctx->c_cosmos, &ctx->c_counter, &ctx->c_lidx, &ctx->c_pmap,
                     &ctx->c_eta, &ctx->c_dx, &ctx->c_contrib})
         g->release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BOLT_OK;
@@ -3230,6 +3279,41 @@ int bolt_graph_destroy(bolt_graph* g) {
     }
     delete g;
     return BOLT_OK;
+}
+
+// bolt_cltt_dev without any allocation: the device work space is this GPU's
+// `scratch` array and the host work space sits on the stack; calls that
+// overlap on one GPU share the array and race
+int bolt_cltt_dev_noalloc(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
+                          const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl, int64_t* d_steps,
+                          float* ms) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(ctx->mu);  // the cosmology table and the device work
+    CosmoSlot* cs;
+    int r = get_cosmo(ctx, cosmo_id, &cs);
+    if (r) return r;
+    if (!d_S || !d_k_src || !d_kgrid || !ells || !d_cl || nk_src < 2 || nkd < 2 || nell < 1)
+        return set_error(BOLT_E_ARG, "bolt_cltt_dev_noalloc: bad argument");
+    if (nell > NOALLOC_MAX_NELL)
+        return set_error(BOLT_E_ARG, "bolt_cltt_dev_noalloc: more than 3001 multipoles requested");
+    for (int e = 0; e < nell; ++e)
+        if (ells[e] < 0 || ells[e] > NOALLOC_MAX_L)
+            return set_error(BOLT_E_ARG, "bolt_cltt_dev_noalloc: ℓ outside [0, 3000]");
+    double* base = ctx->device >= 0 && ctx->device < SCRATCH_MAX_DEV ? g_scratch[ctx->device] : nullptr;
+    if (!base) return set_error(BOLT_E_STATE, "bolt_cltt_dev_noalloc: no work space on this device");
+    LosBufs lb;
+    lb.partial = base;
+    lb.etaj = base + NOALLOC_PART;
+    lb.dxj = lb.etaj + NOALLOC_NODES;
+    lb.lidx = (int*)(lb.dxj + NOALLOC_NODES);
+    lb.pmap = lb.lidx + NOALLOC_MAX_L + 1;
+    lb.cap_partial = NOALLOC_PART; lb.cap_nxi = NOALLOC_NODES;
+    lb.cap_lidx = NOALLOC_MAX_L + 1; lb.cap_pmap = NOALLOC_MAX_NELL;
+    int h_lidx[NOALLOC_MAX_L + 1], h_pmap[NOALLOC_MAX_NELL];
+    const ClHost hb{h_lidx, h_pmap, NOALLOC_MAX_L + 1, NOALLOC_MAX_NELL, ctx->ev0, ctx->ev1};
+    HIPCHK(hipSetDevice(ctx->device));
+    return cl_core(ctx, ctx->stream, &lb, cs->host, cs->d_self, d_S, nullptr, d_k_src, nk_src, d_kgrid, nkd, ells,
+                   nell, 0, d_cl, nullptr, nullptr, (long long*)d_steps, ms, &hb);
 }
 
 int bolt_cl(bolt_ctx* ctx, int cosmo_id, const double* S_T, const double* S_P, const double* k_src, int nk_src,

@@ -233,6 +233,17 @@ int bolt_cltt_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* 
                   const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
                   int64_t* d_steps, float* ms);
 
+/* bolt_cltt_dev that allocates nothing (no reference counterpart; for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results, bit for bit,
+ * as bolt_cltt_dev; the LOS work space is a library-owned device array of
+ * 12.3 MB per GPU (held until the process ends), so ℓ <= 3000 and at most 3001
+ * entries in ells; larger requests return BOLT_E_ARG.  Calls that overlap on
+ * one GPU (from different contexts) share that array and race. */
+int bolt_cltt_dev_noalloc(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_k_src, int nk_src,
+                          const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
+                          int64_t* d_steps, float* ms);
+
 /* Autotuner (no reference counterpart: Bolt's only tunable is the @qthreads
  * loop over k, spectra.jl:10).  Times every shape of the solver -- rows per
  * lane of its substitution scans, SR = 5..8 -- on source_grid over the host
