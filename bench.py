@@ -311,6 +311,218 @@ def batch(args):
     print(json.dumps(line), flush=True)
 
 
+# hierarchy tuples (ℓᵧ, ℓ_ν, ℓ_mν, nq) of the precision sweep: n = 128, 224, 233, 256, 317
+C4_TUPLES = [(20, 20, 5, 10), (12, 12, 8, 20), (40, 40, 6, 15), (16, 16, 4, 40), (30, 25, 6, 32)]
+
+
+def c4(args):
+    """Precision sweep: --ncosmo cosmologies (256) from the parameter box, half
+    with one massive ν; hierarchy tuple c mod 5 of C4_TUPLES (n = 128..317);
+    a k grid of 200, 400 or 800 log points in [1e-4, 0.5] Mpc⁻¹ drawn per
+    cosmology.  Every cosmology's source grid (reltol 1e-11), C_ℓ^TT for
+    ℓ = 2..2500 (5000 dense k) and P(k) at 200 log k (plin, reltol 1e-5) are
+    computed with FP64 stage solves and with FP32 solves + FP64 refinement
+    (the default: until every correction is below 1e-14 of its component, at
+    most 12 sweeps, else the step is rejected).  Errors: FP32 against FP64 on
+    every output of every cosmology; both against the CPU FP64 oracle on samples
+    (P(k) at 6 k of 10 cosmologies, C_ℓ^TT at all ℓ for the two smallest
+    hierarchies from the oracle's own source grids); and, for scale, P(k) at
+    plin's reltol 1e-5 against FP64 P(k) at reltol 1e-9 (the integration's own
+    error).  Solver counters (steps, rejections, RHS evaluations, linear solves)
+    are summed over the source-grid modes.  FP16 matrix-core solves are not
+    offered: the stage solve is banded plus rank 3, with no dense block."""
+    import torch
+    import boltmi
+    from boltmi import _abi
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(args.seed)
+    nc = args.ncosmo
+    pars, tup, nks = [], [], []
+    for c in range(nc):
+        h = rng.uniform(0.6, 0.8)
+        pars.append(boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h ** 2, Ω_c=rng.uniform(0.10, 0.14) / h ** 2,
+                                       n=rng.uniform(0.92, 1.0), A=math.exp(rng.uniform(2.9, 3.2)) / 1e10,
+                                       Σm_ν=(rng.uniform(0.06, 0.3) if c % 2 else 0.0) * boltmi.MASS_NATURAL))
+        tup.append(C4_TUPLES[c % len(C4_TUPLES)])
+        nks.append(int(rng.choice([200, 400, 800])))
+    thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cosmos = [None] * nc
+    for t in C4_TUPLES:  # tables per momentum-bin count
+        idx = [c for c in range(nc) if tup[c] == t]
+        for c, cb in zip(idx, boltmi.backgrounds([pars[c] for c in idx], nq=t[3], threads=thr)):
+            cosmos[c] = cb
+    ctx = boltmi.default_context()
+    L = _abi.lib()
+    dev = torch.device("cuda:0")
+    ells = np.arange(2, LMAX + 1, dtype=np.int32)
+    kpk = np.logspace(-4, math.log10(0.5), 200)
+    modes = {"fp64": 0, "fp32_refined": boltmi.precision_flags("fp32")}
+    counters = {m: np.zeros(4, dtype=np.int64) for m in modes}
+    out = {m: {"cl": [None] * nc, "pk": [None] * nc, "S": {}} for m in modes}
+    timing = {}
+    groups = sorted({(tup[c], nks[c]) for c in range(nc)})
+    with ctx.lock:
+        ids = [bg.cosmo_id(ctx) for _, bg, _ in cosmos]
+    cl_pool = set()  # massless cosmologies of the two smallest hierarchies at nk = 200: the C_ℓ oracle samples
+    for mname, flags in modes.items():
+        t_sg = t_los = t_pk = 0.0
+        nmodes = 0
+        for t, nk in groups:
+            idx = [c for c in range(nc) if tup[c] == t and nks[c] == nk]
+            ks = np.logspace(-4, math.log10(0.5), nk)
+            cfgs = (_abi.HierCfgC * len(idx))(*[boltmi.hier_cfg(t[0], t[1], t[2], t[3], 1e-11, 1e-6, flags=flags)
+                                                for _ in idx])
+            d_k = torch.tensor(np.tile(ks, (len(idx), 1)), dtype=torch.float64, device=dev)
+            d_S = torch.empty((len(idx), nk, cosmos[idx[0]][1].nx), dtype=torch.float64, device=dev)
+            d_st = torch.zeros((len(idx), nk), dtype=torch.int32, device=dev)
+            d_ctr = torch.zeros((len(idx), nk, 4), dtype=torch.int64, device=dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ms = ctypes.c_float()
+            with ctx.lock:
+                _abi.check(L.bolt_source_grid_batch_dev(ctx.handle, len(idx), _abi.pi(np.array([ids[c] for c in idx],
+                                                                                            dtype=np.int32)),
+                                                        cfgs, d_k.data_ptr(), nk, d_S.data_ptr(), None, d_st.data_ptr(),
+                                                        d_ctr.data_ptr(), ctypes.byref(ms)))
+            torch.cuda.synchronize()
+            t_sg += time.perf_counter() - t0
+            counters[mname] += d_ctr.sum(dim=(0, 1)).cpu().numpy()
+            nmodes += len(idx) * nk
+            if int(d_st.max().item()) != 0:
+                raise RuntimeError(f"a mode failed in the {mname} sweep")
+            dense = None
+            for j, c in enumerate(idx):
+                bg = cosmos[c][1]
+                dense = torch.tensor(boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, NKD), device=dev)
+                d_cl = torch.empty(len(ells), dtype=torch.float64, device=dev)
+                t0 = time.perf_counter()
+                with ctx.lock:
+                    _abi.check(L.bolt_cltt_dev(ctx.handle, ids[c], d_S[j].data_ptr(), d_k[j].data_ptr(), nk,
+                                               dense.data_ptr(), NKD, _abi.pi(ells), len(ells), d_cl.data_ptr(), None,
+                                               None))
+                t_los += time.perf_counter() - t0
+                out[mname]["cl"][c] = d_cl.cpu().numpy()
+                if nk == 200 and t in C4_TUPLES[:2] and c % 2 == 0:
+                    cl_pool.add(c)
+            del d_S
+        for c in range(nc):
+            par, bg, ih = cosmos[c]
+            t = tup[c]
+            t0 = time.perf_counter()
+            out[mname]["pk"][c] = boltmi.plin(kpk, par, bg, ih, n_q=t[3], ℓᵧ=t[0], ℓ_ν=t[1], ℓ_mν=t[2], flags=flags)
+            t_pk += time.perf_counter() - t0
+        timing[mname] = {"source_grids_s": t_sg, "kmode_solves_per_s": nmodes / t_sg, "los_s": t_los, "plin_s": t_pk}
+        print(f"# c4 {mname}: source grids {t_sg:.2f} s ({nmodes / t_sg:.0f} k-mode solves/s), LOS {t_los:.2f} s, "
+              f"plin {t_pk:.2f} s", file=sys.stderr, flush=True)
+
+    def rel(a, b):
+        return float(np.max(np.abs(np.asarray(a) / np.asarray(b) - 1)))
+
+    # the integration's own error at plin's reltol: FP64 P(k) at reltol 1e-9
+    t0 = time.perf_counter()
+    pk_tight = []
+    for c in range(nc):
+        par, bg, ih = cosmos[c]
+        t = tup[c]
+        pk_tight.append(boltmi.plin(kpk, par, bg, ih, n_q=t[3], ℓᵧ=t[0], ℓ_ν=t[1], ℓ_mν=t[2], reltol=1e-9))
+    print(f"# c4 fp64 P(k) at reltol 1e-9: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    vs_tight = {m: max(rel(out[m]["pk"][c], pk_tight[c]) for c in range(nc)) for m in modes}
+
+    fp32_vs_64 = {"pk": max(rel(out["fp32_refined"]["pk"][c], out["fp64"]["pk"][c]) for c in range(nc)),
+                  "cltt": max(rel(out["fp32_refined"]["cl"][c], out["fp64"]["cl"][c]) for c in range(nc))}
+    # oracle samples: P(k) at 6 k for 10 cosmologies spread over the tuples
+    pk_samples = [(c, i) for c in range(0, nc, max(1, nc // 10))[:10] for i in (0, 40, 80, 120, 160, 199)]
+
+    def pk_ref(ci):
+        c, i = ci
+        par, bg, _ = cosmos[c]
+        t = tup[c]
+        o = pyoracle.Oracle(par.as_tuple(), bg.tables_dict())
+        return o.plin(pyoracle.Oracle.cfg(t[0], t[1], t[2], t[3], 1e-5, 1e-6), float(kpk[i]))[0]
+
+    with ThreadPoolExecutor(thr) as ex:
+        pk_o = list(ex.map(pk_ref, pk_samples))
+    err_pk = {m: max(abs(out[m]["pk"][c][i] / r - 1) for (c, i), r in zip(pk_samples, pk_o)) for m in modes}
+    # oracle samples: C_ℓ^TT from the oracle's own source grid, two smallest hierarchies at nk = 200
+    cl_samples = sorted(cl_pool)[:2]
+    err_cl = {m: 0.0 for m in modes}
+    for c in cl_samples:
+        par, bg, _ = cosmos[c]
+        t = tup[c]
+        o = pyoracle.Oracle(par.as_tuple(), bg.tables_dict())
+        ks = np.logspace(-4, math.log10(0.5), 200)
+        S, st = o.source_grid(pyoracle.Oracle.cfg(t[0], t[1], t[2], t[3], 1e-11, 1e-6), ks, nthreads=thr)
+        ref = o.cltt(S, ks, boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, NKD), ells, 0, nthreads=thr)
+        for m in modes:
+            err_cl[m] = max(err_cl[m], rel(out[m]["cl"][c], ref))
+    line = {
+        "metric": "precision sweep: k-mode solves/s and per-output max rel err vs the FP64 oracle",
+        "value": timing["fp32_refined"]["kmode_solves_per_s"], "unit": "k-mode solves/s (FP32 solve + refinement)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64 state, f32 stage solves", "data": "synthetic",
+        "config": {"workload": f"{nc} cosmologies from the parameter box (half with one massive nu), hierarchy tuples "
+                               f"{C4_TUPLES} (n = 128..317), nk in {{200, 400, 800}}, source_grid reltol 1e-11, "
+                               f"C_l^TT l = 2..2500 on 5000 dense k, P(k) at 200 k (reltol 1e-5)",
+                   "n_cosmo": nc, "seed": args.seed},
+        "timing": timing,
+        "solver_counters": {m: dict(zip(("steps_accepted", "steps_rejected", "rhs_evals", "linear_solves"),
+                                        (int(v) for v in counters[m]))) for m in modes},
+        "max_rel_err": {
+            "fp32_refined_vs_fp64_all_cosmologies": fp32_vs_64,
+            "pk_reltol_1e-5_vs_fp64_reltol_1e-9_all_cosmologies": vs_tight,
+            "vs_oracle_pk": {m: float(v) for m, v in err_pk.items()},
+            "vs_oracle_cltt": err_cl,
+            "oracle_samples": {"pk": f"{len(pk_samples)} (cosmology, k) pairs", "cltt": f"cosmologies {cl_samples}, "
+                               f"all 2499 l, oracle source grid at nk = 200"}},
+        "fp16_matrix_core": "not offered: the stage solve is banded + rank-3, no dense matrix to put on MFMA",
+    }
+    print(json.dumps(line), flush=True)
+
+
+def jvp(args):
+    """Forward-mode derivative of the full-CMB spectrum on the device: source_grid
+    (800 log k, ℓᵧ = ℓ_ν = 50, ℓ_mν = 10, reltol 1e-11) and C_ℓ^TT (ℓ = 2..2500 on
+    5000 dense k) with one Dual direction (∂/∂Ω_b, as examples/plot_deriv_cl.jl),
+    against the same computation without the derivative.  Reports the kernels'
+    device times and the derivative's cost ratio."""
+    import torch
+    import boltmi
+    from boltmi.dual import Dual, value
+    kw = dict(h=0.7, Ω_b=0.046, Ω_c=0.224, n=0.96, A=2.1e-9, Σm_ν=0.06 * boltmi.MASS_NATURAL)
+    par0 = boltmi.CosmoParams(**kw)
+    bg0 = boltmi.Background(par0)
+    ih0 = boltmi.IonizationHistory(boltmi.RECFAST(bg0), par0, bg0)
+    pard = boltmi.CosmoParams(**dict(kw, Ω_b=Dual(0.046, 1.0)))
+    bgd = boltmi.Background(pard)
+    ihd = boltmi.IonizationHistory(boltmi.RECFAST(bgd), pard, bgd)
+    ks = np.logspace(-4, math.log10(0.5), NK)
+    dense = boltmi.quadratic_k(0.01 * bg0.H0, 1000 * bg0.H0, NKD)
+    ells = np.arange(2, LMAX + 1)
+    out = {}
+    for name, (par, bg, ih) in (("primal", (par0, bg0, ih0)), ("with_derivative", (pard, bgd, ihd))):
+        for rep in range(2):  # the first call loads the kernels
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sf = boltmi.source_grid(par, bg, ih, ks, None, 50, 1e-11, ℓ_ν=50, ℓ_mν=10)
+            t1 = time.perf_counter()
+            cl = boltmi.cltt_on_grid(ells, sf, dense, par, bg)
+            t2 = time.perf_counter()
+        out[name] = {"source_grid_s": t1 - t0, "solver_kernel_ms": sf.stats["kernel_ms"], "cltt_s": t2 - t1,
+                     "steps_accepted": int(sf.stats["accepted"])}
+        print(f"# jvp {name}: source_grid {t1 - t0:.3f} s (kernel {sf.stats['kernel_ms']:.1f} ms), "
+              f"cltt {t2 - t1:.3f} s", file=sys.stderr, flush=True)
+    line = {"metric": "derivative of the full-CMB spectrum along one parameter (source_grid + C_l^TT with "
+                      "d/dOmega_b), cost relative to the spectrum alone",
+            "value": out["with_derivative"]["solver_kernel_ms"] / out["primal"]["solver_kernel_ms"],
+            "unit": "solver kernel time ratio (derivative / primal)", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "800 log k in [1e-4, 0.5]/Mpc, l_g = l_nu = 50, l_mnu = 10, nq = 15, reltol 1e-11; "
+                                   "C_l^TT l = 2..2500 on 5000 dense k; one Dual direction"},
+            "timing": out}
+    print(json.dumps(line), flush=True)
+
+
 def serve(args):
     print(json.dumps(serve_run(args.rate, args.requests, args.slots, args.hbm_gib, args.seed)), flush=True)
 
@@ -394,9 +606,13 @@ def main():
                     help="run one device-to-device copy of this many MiB before the workload (reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--c1", action="store_true", help="measure the plain-method workload (100 k, l=10, states + P(k))")
+    ap.add_argument("--jvp", action="store_true",
+                    help="measure the forward-mode derivative of the full-CMB spectrum against the spectrum alone")
+    ap.add_argument("--c4", action="store_true",
+                    help="precision sweep: FP64 against FP32 + refinement stage solves over --ncosmo (256) cosmologies")
     ap.add_argument("--batch", action="store_true",
                     help="measure the many-cosmology batch (source_grid for --ncosmo cosmologies in one launch)")
-    ap.add_argument("--ncosmo", type=int, default=64, help="--batch: number of cosmologies")
+    ap.add_argument("--ncosmo", type=int, default=0, help="--batch / --c4: number of cosmologies (64 / 256)")
     ap.add_argument("--serve", action="store_true",
                     help="measure the serving workload (Poisson request stream) instead of the full CMB path")
     ap.add_argument("--rate", type=float, default=50.0, help="--serve: mean request arrivals per second")
@@ -414,10 +630,18 @@ def main():
         serve(args)
         return
     if args.batch:
+        args.ncosmo = args.ncosmo or 64
         batch(args)
+        return
+    if args.c4:
+        args.ncosmo = args.ncosmo or 256
+        c4(args)
         return
     if args.c1:
         c1(args)
+        return
+    if args.jvp:
+        jvp(args)
         return
 
     import torch

@@ -2,7 +2,7 @@
 from .api import (MASS_NATURAL, Background, autotune, backgrounds, BasicNewtonian, Context, CosmoParams, HIPNewtonian, Hierarchy,
                   IonizationHistory, PerturbationIntegrator, RECFAST, Server, Solution, SourceGrid, boltsolve,
                   boltsolve_batch, boltsolve_rsa, cl_on_grid, clee, clee_on_grid, clte, clte_on_grid, cltt,
-                  cltt_on_grid, default_context, hier_cfg,
+                  cltt_on_grid, default_context, FP32_SOLVE, hier_cfg, precision_flags,
                   initial_conditions_batch, log10_k, plin, quadratic_k, recfast_xe, rhs_batch, source_grid,
                   source_grid_P, source_grid_batch, source_grids_TP,
                   state_length)

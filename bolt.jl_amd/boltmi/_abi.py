@@ -82,9 +82,19 @@ SIGNATURES = {
     "bolt_ctx_create": (_I, [_I, ctypes.POINTER(_VP)]),
     "bolt_ctx_destroy": (_I, [_VP]),
     "bolt_ctx_sync": (_I, [_VP]),
-    "bolt_ws_layout": (_I, [_I, _I, _I, _PI64, _PI, _PI64, _PI64, _PI64]),
+    "bolt_ws_layout": (_I, [_I, _I, _I, _I, _PI64, _PI, _PI64, _PI64, _PI64]),
     "bolt_cosmo_upload": (_I, [_VP, ctypes.POINTER(CosmoParamsC), ctypes.POINTER(TablesC), _PI]),
     "bolt_cosmo_release": (_I, [_VP, _I]),
+    "bolt_cosmo_upload_jvp": (_I, [_VP, ctypes.POINTER(CosmoParamsC), ctypes.POINTER(CosmoParamsC),
+                                   ctypes.POINTER(TablesC), _PD, _PD, _PI]),
+    "bolt_rhs_jvp_batch": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PD, _PD]),
+    "bolt_initial_conditions_jvp": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _PD]),
+    "bolt_source_grid_jvp": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _PD, _PI,
+                                  ctypes.POINTER(StatsC)]),
+    "bolt_source_grid_jvp_dev": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _VP, _I, _VP, _VP, _VP, _VP,
+                                      ctypes.POINTER(ctypes.c_float)]),
+    "bolt_boltsolve_jvp_batch": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _I, _PD, _PD, _PI,
+                                      ctypes.POINTER(StatsC)]),
     "bolt_rhs_batch": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PD, _I]),
     "bolt_initial_conditions": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD]),
     "bolt_boltsolve_batch": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _PD, _I, _PD, _PI,
@@ -104,6 +114,8 @@ SIGNATURES = {
     "bolt_source_grid_batch_dev": (_I, [_VP, _I, _PI, ctypes.POINTER(HierCfgC), _VP, _I, _VP, _VP, _VP, _VP,
                                         ctypes.POINTER(ctypes.c_float)]),
     "bolt_cltt": (_I, [_VP, _I, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD]),
+    "bolt_cltt_jvp": (_I, [_VP, _I, _PD, _PD, _PD, _I, _PD, _I, _PI, _I, _PD, _PD]),
+    "bolt_cltt_jvp_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, ctypes.POINTER(ctypes.c_float)]),
     "bolt_cl": (_I, [_VP, _I, _PD, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _PD, _PD]),
     "bolt_cl_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _VP, _I, _PI, _I, _VP, _VP, _VP, _VP,
                          ctypes.POINTER(ctypes.c_float)]),

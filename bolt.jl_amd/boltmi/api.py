@@ -53,8 +53,8 @@ class CosmoParams:
 
     def c_struct(self):
         if self.is_dual():
-            raise TypeError("this entry point takes a plain cosmology; derivatives stop at the host tables "
-                            "(Background / IonizationHistory)")
+            raise TypeError("this entry point takes a plain cosmology; Dual parameters reach the device through "
+                            "Background (their tables' derivative) and cosmo_id (bolt_cosmo_upload_jvp)")
         return _abi.CosmoParamsC(*[float(v) for v in self.as_tuple()])
 
     def c_structs_dual(self):
@@ -151,17 +151,26 @@ class Background:
         return dict(nx=self.nx, nq=self.nq, x0=self.x0, x1=self.x1, H0=self.H0, eta0=self.eta0,
                     rho_crit=self.rho_crit, xq=self.quad_pts, wq=self.quad_wts, coef=self.coef)
 
+    @property
+    def is_dual(self):
+        return self.dcoef is not None
+
     def cosmo_id(self, ctx):
-        if self.dcoef is not None:
-            raise TypeError("a Background built from Dual parameters cannot be uploaded: derivatives stop at the "
-                            "host tables (the tangent-linear perturbation solve is not built)")
+        """The id of these tables on ctx (uploaded once per context); a Dual
+        background uploads its tangent too (bolt_cosmo_upload_jvp)."""
         if not ctx.handle:
             raise ValueError("the context is closed")
         ent = self._uploads.get(ctx.serial)
         if ent is None:
             cid = ctypes.c_int(-1)
-            check(lib().bolt_cosmo_upload(ctx.handle, ctypes.byref(self.par.c_struct()),
-                                          ctypes.byref(self.c_tables()), ctypes.byref(cid)))
+            if self.is_dual:
+                pv, pt = self.par.c_structs_dual()
+                check(lib().bolt_cosmo_upload_jvp(ctx.handle, ctypes.byref(pv), ctypes.byref(pt),
+                                                  ctypes.byref(self.c_tables()), pd(self.dcoef), pd(self.dscal),
+                                                  ctypes.byref(cid)))
+            else:
+                check(lib().bolt_cosmo_upload(ctx.handle, ctypes.byref(self.par.c_struct()),
+                                              ctypes.byref(self.c_tables()), ctypes.byref(cid)))
             ent = (weakref.ref(ctx), cid.value)
             self._uploads[ctx.serial] = ent
         return ent[1]
@@ -300,8 +309,24 @@ class Hierarchy:
         return 2 * (self.ℓᵧ + 1) + (self.ℓ_ν + 1) + (self.ℓ_mν + 1) * self.nq + 5
 
 
-def hier_cfg(ℓᵧ=8, ℓ_ν=8, ℓ_mν=10, nq=15, reltol=1e-6, abstol=1e-6, max_steps=0):
-    return _abi.HierCfgC(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol, max_steps, 0)
+FP32_SOLVE = 1  # bolt_hier_cfg.flags: FP32 stage solves with FP64 iterative refinement
+
+
+def precision_flags(precision="fp64", refine=0):
+    """cfg.flags for a stage-solve precision: "fp64", or "fp32" (FP32 factors and
+    substitutions, refined against the FP64 residual until every correction is
+    below 1e-14 of its component, at most `refine` sweeps per stage; 0 selects 12)"""
+    if precision == "fp64":
+        return 0
+    if precision == "fp32":
+        if not 0 <= refine <= 15:
+            raise ValueError("refine must be 0..15")
+        return FP32_SOLVE | (refine << 4)
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+def hier_cfg(ℓᵧ=8, ℓ_ν=8, ℓ_mν=10, nq=15, reltol=1e-6, abstol=1e-6, max_steps=0, flags=0):
+    return _abi.HierCfgC(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol, max_steps, flags)
 
 
 def state_length(cfg):
@@ -329,8 +354,24 @@ def _stats_dict(st):
 
 
 def rhs_batch(par, bg, cfg, ks, x, U, mutate=False, ctx=None):
-    """hierarchy! for modes ks at x; U: [n][B] (mode fastest). Returns (dU, U')."""
+    """hierarchy! for modes ks at x; U: [n][B] (mode fastest). Returns (dU, U').
+    With a Dual background and U a Dual (U, U̇): returns the Dual du (its
+    partial is J U̇ + (∂f/∂p) U) and U."""
     ctx = ctx or default_context()
+    if bg.is_dual:
+        if mutate:
+            raise ValueError("the tangent of hierarchy! is taken without the RSA write")
+        Uv, Ud = (U.v, U.d) if isinstance(U, Dual) else (U, np.zeros_like(U))
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        Uv = np.array(Uv, dtype=np.float64, order="C")
+        Ud = np.array(Ud, dtype=np.float64, order="C")
+        dU, dUd = np.zeros_like(Uv), np.zeros_like(Uv)
+        with ctx.lock:
+            cid = bg.cosmo_id(ctx)
+            check(lib().bolt_rhs_batch(ctx.handle, cid, ctypes.byref(cfg), pd(ks), len(ks), x, pd(Uv), pd(dU), 0))
+            check(lib().bolt_rhs_jvp_batch(ctx.handle, cid, ctypes.byref(cfg), pd(ks), len(ks), x, pd(Uv), pd(Ud),
+                                           pd(dUd)))
+        return Dual(dU, dUd), Uv
     ks = np.ascontiguousarray(ks, dtype=np.float64)
     U = np.array(U, dtype=np.float64, order="C")
     dU = np.zeros_like(U)
@@ -341,9 +382,16 @@ def rhs_batch(par, bg, cfg, ks, x, U, mutate=False, ctx=None):
 
 
 def initial_conditions_batch(par, bg, cfg, ks, ctx=None):
+    """initial_conditions [n][B]; a Dual with a Dual background"""
     ctx = ctx or default_context()
     ks = np.ascontiguousarray(ks, dtype=np.float64)
     U = np.zeros((state_length(cfg), len(ks)))
+    if bg.is_dual:
+        Ud = np.zeros_like(U)
+        with ctx.lock:
+            check(lib().bolt_initial_conditions_jvp(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks),
+                                                    pd(U), pd(Ud)))
+        return Dual(U, Ud)
     with ctx.lock:
         check(lib().bolt_initial_conditions(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(U)))
     return U
@@ -358,6 +406,14 @@ def boltsolve_batch(par, bg, cfg, ks, x_save, ctx=None, raise_on_fail=True):
     out = np.zeros((len(ks), len(xs), n))
     status = np.zeros(len(ks), dtype=np.int32)
     st = _abi.StatsC()
+    if bg.is_dual:  # the tangent-linear solve: states and their derivatives, as a Dual
+        outd = np.zeros_like(out)
+        with ctx.lock:
+            rc = lib().bolt_boltsolve_jvp_batch(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks),
+                                                pd(xs), len(xs), pd(out), pd(outd), pi(status), ctypes.byref(st))
+        if raise_on_fail or rc != _abi.BOLT_E_NUMERIC:
+            check(rc)
+        return Dual(out, outd), status, _stats_dict(st)
     with ctx.lock:
         rc = lib().bolt_boltsolve_batch(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(xs),
                                         len(xs), pd(out), pi(status), ctypes.byref(st))
@@ -414,16 +470,22 @@ class SourceGrid:
 
 
 def source_grid(par, bg, ih, k_grid, integrator=None, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν=10, nq=15, abstol=1e-6,
-                ctx=None):
+                ctx=None, flags=0):
     """source_grid(par, bg, ih, k_grid, BasicNewtonian(); ℓᵧ=8, reltol=1e-11)
     (spectra.jl:6-23).  As in Bolt, ℓ_ν, ℓ_mν and nq keep the Hierarchy
     defaults unless given."""
     ctx = ctx or default_context()
     ks = np.ascontiguousarray(k_grid, dtype=np.float64)
-    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol)
+    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, nq, reltol, abstol, flags=flags)
     S = np.zeros((len(ks), bg.nx))
     status = np.zeros(len(ks), dtype=np.int32)
     st = _abi.StatsC()
+    if bg.is_dual:  # ForwardDiff through source_grid: the grid is a Dual (values, derivatives)
+        dS = np.zeros_like(S)
+        with ctx.lock:
+            check(lib().bolt_source_grid_jvp(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(S),
+                                             pd(dS), pi(status), ctypes.byref(st)))
+        return SourceGrid(bg.x_grid, ks, Dual(S, dS), _stats_dict(st))
     with ctx.lock:
         check(lib().bolt_source_grid(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks), pd(S),
                                      pi(status), ctypes.byref(st)))
@@ -463,7 +525,7 @@ def _per(v, n):
 
 
 def source_grid_batch(cosmos, k_grids, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν=10, nq=15, abstol=1e-6, want_t=True,
-                      want_p=False, ctx=None):
+                      want_p=False, ctx=None, flags=0):
     """source_grid (and/or source_grid_P) for many cosmologies in one launch.
 
     cosmos: [(par, bg, ih), ...]; k_grids: one k grid for all, or one per
@@ -480,7 +542,7 @@ def source_grid_batch(cosmos, k_grids, ℓᵧ=8, reltol=1e-11, ℓ_ν=8, ℓ_mν
     cfgs = (_abi.HierCfgC * nc)()
     for c, (g, n_, m_, q_, rt, at) in enumerate(zip(_per(ℓᵧ, nc), _per(ℓ_ν, nc), _per(ℓ_mν, nc), _per(nq, nc),
                                                     _per(reltol, nc), _per(abstol, nc))):
-        cfgs[c] = hier_cfg(g, n_, m_, q_, rt, at)
+        cfgs[c] = hier_cfg(g, n_, m_, q_, rt, at, flags=flags)
     nx = cosmos[0][1].nx
     S = np.zeros((nc, nk, nx)) if want_t else None
     SP = np.zeros((nc, nk, nx)) if want_p else None
@@ -510,24 +572,31 @@ def autotune(par, bg, cfg, k_grid, kgrid=None, ell_max=0, ctx=None):
         check(lib().bolt_autotune(ctx.handle, bg.cosmo_id(ctx), ctypes.byref(cfg), pd(ks), len(ks),
                                   None if kg is None else pd(kg), 0 if kg is None else len(kg), int(ell_max),
                                   ctypes.byref(rep)))
-    return {"solver_sr": rep.solver_sr, "solver_ms": {sr: float(rep.solver_ms[sr - 5]) for sr in range(5, 9)},
+    return {"solver_sr": rep.solver_sr, "solver_ms": {sr: float(rep.solver_ms[sr - 5]) for sr in range(6, 9)},
             "los_spw": rep.los_spw, "los_ms": {spw: float(rep.los_ms[i]) for i, spw in enumerate((1, 2, 4))}}
 
 
 def quadratic_k(kmin, kmax, nk):
-    """quadratic_k (spectra.jl:60-63): k_i = kmin + (kmax-kmin)(i/nk)², i = 1..nk."""
+    """quadratic_k (spectra.jl:60-63): k_i = kmin + (kmax-kmin)(i/nk)², i = 1..nk.
+    Dual bounds are taken at their values (assume_nondual, spectra.jl:46-47, 61)."""
+    kmin, kmax = value(kmin), value(kmax)
     return np.array([kmin + (kmax - kmin) * (i / nk) ** 2 for i in range(1, nk + 1)])
 
 
 def log10_k(kmin, kmax, nk):
-    """log10_k (spectra.jl:65-68)."""
+    """log10_k (spectra.jl:65-68), Dual bounds at their values."""
+    kmin, kmax = value(kmin), value(kmax)
     return np.array([10 ** (math.log10(kmin) + math.log10(kmax / kmin) * (i - 1) / (nk - 1))
                      for i in range(1, nk + 1)])
 
 
 def cltt_on_grid(ells, sf: SourceGrid, kgrid, par, bg, bessel_mode=0, ctx=None):
-    """cltt(ℓ, s_itp, kgrid, par, bg) (spectra.jl:84-96) for a list of ℓ."""
+    """cltt(ℓ, s_itp, kgrid, par, bg) (spectra.jl:84-96) for a list of ℓ.  With a
+    Dual background (and a Dual source grid from it) the result is a Dual: C_ℓ
+    and its derivative along the parameters' partials (bolt_cltt_jvp)."""
     ctx = ctx or default_context()
+    if bg.is_dual or isinstance(sf.grid, Dual):
+        return _cltt_jvp(ells, sf, kgrid, bg, bessel_mode, ctx)
     ells_a = np.ascontiguousarray(np.atleast_1d(ells), dtype=np.int32)
     S = np.ascontiguousarray(sf.grid, dtype=np.float64)
     ksrc = np.ascontiguousarray(sf.k_grid, dtype=np.float64)
@@ -537,6 +606,25 @@ def cltt_on_grid(ells, sf: SourceGrid, kgrid, par, bg, bessel_mode=0, ctx=None):
         check(lib().bolt_cltt(ctx.handle, bg.cosmo_id(ctx), pd(S), pd(ksrc), len(ksrc), pd(kg), len(kg), pi(ells_a),
                               len(ells_a), bessel_mode, pd(cl)))
     return cl if np.ndim(ells) else float(cl[0])
+
+
+def _cltt_jvp(ells, sf, kgrid, bg, bessel_mode, ctx):
+    if not bg.is_dual:
+        raise TypeError("a Dual source grid needs the Dual background it was solved with (its η tables' derivative)")
+    if bessel_mode != 0:
+        raise ValueError("the derivative of C_ℓ is computed with the exact j_ℓ (bessel_mode 0)")
+    ells_a = np.ascontiguousarray(np.atleast_1d(ells), dtype=np.int32)
+    g = sf.grid
+    S = np.ascontiguousarray(g.v if isinstance(g, Dual) else g, dtype=np.float64)
+    dS = np.ascontiguousarray(g.d if isinstance(g, Dual) else np.zeros_like(S), dtype=np.float64)
+    ksrc = np.ascontiguousarray(sf.k_grid, dtype=np.float64)
+    kg = np.ascontiguousarray(kgrid, dtype=np.float64)
+    cl = np.zeros(len(ells_a))
+    dcl = np.zeros(len(ells_a))
+    with ctx.lock:
+        check(lib().bolt_cltt_jvp(ctx.handle, bg.cosmo_id(ctx), pd(S), pd(dS), pd(ksrc), len(ksrc), pd(kg), len(kg),
+                                  pi(ells_a), len(ells_a), pd(cl), pd(dcl)))
+    return Dual(cl, dcl) if np.ndim(ells) else Dual(float(cl[0]), float(dcl[0]))
 
 
 def cltt(ells, par, bg, ih, sf, ctx=None):
@@ -594,11 +682,11 @@ def clee(ells, par, bg, ih, sf_P, ctx=None):
     return clee_on_grid(ells, sf_P, dense, par, bg, ctx)
 
 
-def plin(k, par, bg, ih, n_q=15, ℓᵧ=50, ℓ_ν=50, ℓ_mν=20, x=0.0, reltol=1e-5, abstol=1e-6, ctx=None):
+def plin(k, par, bg, ih, n_q=15, ℓᵧ=50, ℓ_ν=50, ℓ_mν=20, x=0.0, reltol=1e-5, abstol=1e-6, ctx=None, flags=0):
     """plin (spectra.jl:163-198); k may be a scalar or an array (one batched launch)."""
     ctx = ctx or default_context()
     ks = np.ascontiguousarray(np.atleast_1d(k), dtype=np.float64)
-    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, n_q, reltol, abstol)
+    cfg = hier_cfg(ℓᵧ, ℓ_ν, ℓ_mν, n_q, reltol, abstol, flags=flags)
     pk = np.zeros(len(ks))
     status = np.zeros(len(ks), dtype=np.int32)
     st = _abi.StatsC()

@@ -7,8 +7,11 @@ a Dual; Background then computes the tables together with their derivative
 along the Dual parts (the C entry point bolt_host_tables_jvp, forward-mode AD
 through Background and RECFAST), and its splines evaluate to Duals.
 
-Derivatives stop at the host tables: the device solvers (boltsolve,
-source_grid, cltt, plin) take plain cosmologies and reject a Dual one.
+The device carries the derivative on: with a Dual background, hierarchy!
+(rhs_batch), initial_conditions_batch, boltsolve_batch, source_grid and cltt
+return Duals, computed by the tangent-linear solve and LOS (bolt_*_jvp).
+plin, the polarization sources and the batched entry points take plain
+cosmologies.
 """
 import numpy as np
 

@@ -86,9 +86,22 @@ typedef struct bolt_tables {
 typedef struct bolt_hier_cfg {
     int ell_g, ell_nu, ell_mnu, nq;
     double reltol, abstol;
-    int max_steps;      /* per mode; 0 -> 2,000,000 */
-    int flags;          /* reserved, 0 */
+    int max_steps;      /* per mode; 0 -> 100,000 */
+    int flags;          /* 0 (FP64 throughout), or BOLT_CFG_FP32_SOLVE [| refinement sweeps << 4] */
 } bolt_hier_cfg;
+
+/* cfg.flags.  BOLT_CFG_FP32_SOLVE: the stage equations (I − hγJ)z = r of the
+ * KenCarp4 Newton solve (perturbations.jl:25-33) are factored and substituted
+ * in FP32, and each stage value is refined against its FP64 residual
+ * r − z + hγJz until every correction is below 1e-14 of its component, at
+ * most (flags & BOLT_CFG_REFINE_MASK) >> 4 sweeps (0 selects 12); a stage
+ * still unconverged after the last sweep is taken as it stands.  The state,
+ * stage combinations, error estimate, step control, residuals and spline
+ * lookups stay FP64.  All modes of one launch use the same setting.  The
+ * step sequence then differs from the FP64 solve's, so outputs differ from
+ * it by up to the integration's own error at the given tolerances. */
+#define BOLT_CFG_FP32_SOLVE 1
+#define BOLT_CFG_REFINE_MASK 0xF0
 
 typedef struct bolt_stats {
     int64_t steps_accepted, steps_rejected, rhs_evals, linear_solves;
@@ -160,17 +173,70 @@ int bolt_ctx_sync(bolt_ctx* ctx);
 
 /* Layout introspection (no reference counterpart; host only, no GPU needed):
  * the solver's per-k-mode LDS workspace for state length n and nq momentum
- * bins, as `*nregions` [begin, end) byte ranges written to regions[2*i..] (at
+ * bins (fp32 != 0: as the FP32 solve mode uses it), as `*nregions` [begin, end) byte ranges written to regions[2*i..] (at
  * most cap of them), the byte offset of the slot that absorbs stores of rows
  * past n, the bytes a launch reserves per workgroup, and the farthest byte
  * the solver's branch-free reads past n may touch.  For tests of the layout. */
-int bolt_ws_layout(int n, int nq, int cap, long long* regions, int* nregions, long long* dump_off,
+int bolt_ws_layout(int n, int nq, int fp32, int cap, long long* regions, int* nregions, long long* dump_off,
                    long long* total_bytes, long long* read_reach);
 
 /* Upload one cosmology's parameters and tables; returns its id. */
 int bolt_cosmo_upload(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_tables* t,
                       int* cosmo_id);
 int bolt_cosmo_release(bolt_ctx* ctx, int cosmo_id);
+
+/* ---------------------------------------------- forward-mode derivatives ---
+ * Bolt's headline feature: ForwardDiff.Dual parameters through the whole
+ * pipeline (README.md; source_grid, cltt and plin are generic over the
+ * parameter type: spectra.jl:6-23, 84-96, 163-198; runtests.jl:54-80;
+ * examples/plot_deriv_cl.jl:100-125).  Here the derivative along one
+ * parameter direction dpar is carried beside the solution on the device.
+ *
+ * bolt_cosmo_upload_jvp: bolt_cosmo_upload plus the cosmology's tangent along
+ * dpar.  dcoef [BOLT_NTAB][nx+2] and dscal [BOLT_NSCAL] are the derivatives of
+ * t->coef and t->scal (what bolt_host_tables_jvp writes); the q grid's and
+ * every derived constant's derivatives are formed from dpar and dscal.  The id
+ * serves every other entry point as well. */
+int bolt_cosmo_upload_jvp(bolt_ctx* ctx, const bolt_cosmo_params* par, const bolt_cosmo_params* dpar,
+                          const bolt_tables* t, const double* dcoef, const double* dscal, int* cosmo_id);
+
+/* The derivative of hierarchy! at (u, u̇): du̇ = J u̇ + (∂f/∂p)·dpar u, for B modes at x
+ * (perturbations.jl:161-271 under Duals).  u, ud, dud: [n][B], Bolt's order.
+ * Checks the tangent of every row recipe, table and q weight. */
+int bolt_rhs_jvp_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B, double x,
+                       const double* u, const double* ud, double* dud);
+/* initial_conditions (perturbations.jl:274-338) and their derivative; [n][B] */
+int bolt_initial_conditions_jvp(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                                double* u0, double* ud0);
+
+/* Tangent-linear boltsolve / source_grid: the primal solve and, beside it,
+ * u̇ along the same accepted steps (each stage's tangent equation
+ * (I − hγJ)Ż = h(J ȧrg + J̇(arg + γZ)) solved with the primal's factors), so the
+ * outputs' derivatives are those of the discrete solution the primal
+ * computes.  Step control looks at the primal only (ForwardDiff under
+ * OrdinaryDiffEq may also weigh the partials in its error norm; both agree
+ * with the exact derivative to the integration's tolerance).  Temperature
+ * source only; FP64; hierarchies whose workspace plus tangent fit one
+ * workgroup's LDS (n = 323 at ℓᵧ = ℓ_ν = 50 does).  S, dS: [nk][nx]. */
+int bolt_source_grid_jvp(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int nk,
+                         double* S, double* dS, int* status, bolt_stats* stats);
+int bolt_source_grid_jvp_dev(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* d_k, int nk,
+                             double* d_S, double* d_dS, int* d_status, int64_t* d_counters, float* ms);
+/* cltt (spectra.jl:84-96) and its derivative along the cosmology's tangent
+ * direction, from S and dS = dS/dp (bolt_source_grid_jvp): the derivative of
+ * Σ_k T_ℓ² P(k) Δk/k with T_ℓ = Σ_x j_ℓ(k(η₀ − η(x))) S Δx, through S, the
+ * Bessel argument (j′_ℓ k(dη₀ − dη)) and P(k) = A (k/0.05)^(n−1); the k and x
+ * grids carry no derivative (assume_nondual, spectra.jl:46-47).  Exact j_ℓ
+ * (bessel_mode 0). */
+int bolt_cltt_jvp(bolt_ctx* ctx, int cosmo_id, const double* S, const double* dS, const double* k_src, int nk_src,
+                  const double* kgrid, int nkd, const int* ells, int nell, double* cl, double* dcl);
+int bolt_cltt_jvp_dev(bolt_ctx* ctx, int cosmo_id, const double* d_S, const double* d_dS, const double* d_k_src,
+                      int nk_src, const double* d_kgrid, int nkd, const int* ells, int nell, double* d_cl,
+                      double* d_dcl, float* ms);
+/* states and their derivatives at sorted x_save; u_out, ud_out [B][nsave][n] */
+int bolt_boltsolve_jvp_batch(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const double* k, int B,
+                             const double* x_save, int nsave, double* u_out, double* ud_out, int* status,
+                             bolt_stats* stats);
 
 /* hierarchy!(du, u, h, x) for B modes at one x (src/perturbations.jl:161-271).
  * u, du: [n][B] (mode fastest).  If mutate != 0 the RSA branch's in-place
@@ -246,7 +312,7 @@ int bolt_cltt_dev_noalloc(bolt_ctx* ctx, int cosmo_id, const double* d_S, const 
 
 /* Autotuner (no reference counterpart: Bolt's only tunable is the @qthreads
  * loop over k, spectra.jl:10).  Times every shape of the solver -- rows per
- * lane of its substitution scans, SR = 5..8 -- on source_grid over the host
+ * lane of its substitution scans, SR = 6..8 -- on source_grid over the host
  * array k[nk] (increasing) with hierarchy cfg, and, when kgrid is not NULL,
  * every shape of the LOS kernel -- x slots per wavefront, 1, 2, 4 -- on cltt
  * for ℓ = 2..lmax over kgrid[nkd] and the source grid just computed.  The
@@ -256,8 +322,9 @@ int bolt_cltt_dev_noalloc(bolt_ctx* ctx, int cosmo_id, const double* d_S, const 
  * A given shape gives bitwise reproducible results; two shapes agree to
  * roundoff (their sums run in different orders). */
 typedef struct bolt_tune_report {
-    int solver_sr;      /* chosen rows per lane (5..8) */
-    float solver_ms[4]; /* one source_grid launch per SR = 5, 6, 7, 8 */
+    int solver_sr;      /* chosen rows per lane (6..8) */
+    float solver_ms[4]; /* one source_grid launch per SR = 6, 7, 8 in entries 1..3; entry 0 (SR = 5, a shape
+                           that is not launched) stays 0 */
     int los_spw;        /* chosen x slots per wavefront (1, 2, 4); 0 when kgrid is NULL */
     float los_ms[3];    /* one LOS launch per SPW = 1, 2, 4 */
 } bolt_tune_report;

@@ -629,8 +629,14 @@ static double wrms(const double* e, const double* u0, const double* u1, int n, d
     return sqrt(s / n);
 }
 
+/* KenCarp4 with Shampine's smoothed error estimate and OrdinaryDiffEq's step
+ * control (perturbations.jl:25-33).  replay_xs (test infrastructure, may be
+ * NULL): instead of choosing steps, take exactly the accepted steps whose end
+ * points are replay_xs[0..n_replay-1] (as recorded by orc_boltsolve_steps) and
+ * accept each; the discrete solution is then a smooth function of the inputs,
+ * so central differences of it give the derivative of the fixed-step map. */
 static int integrate(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, obs_t* obs,
-                     orc_stats* st) {
+                     orc_stats* st, const double* replay_xs, int n_replay) {
     lay_t L = layout(c);
     if (cfg_bad(c)) {
         memset(st, 0, sizeof(*st));
@@ -668,6 +674,10 @@ static int integrate(const orc_params* p, const orc_tables* t, const orc_cfg* c,
     int status = 0;
     long nsteps = 0;
     while (x < xend) {
+        if (replay_xs) {
+            if (nsteps >= n_replay) { status = 1; break; }
+            h = replay_xs[nsteps] - x;
+        }
         if (++nsteps > maxsteps) { status = 1; break; }
         int last = 0;
         if (h >= xend - x) { h = xend - x; last = 1; }
@@ -705,7 +715,7 @@ static int integrate(const orc_params* p, const orc_tables* t, const orc_cfg* c,
         /* Shampine's smoothed estimate, KenCarp4's default (smooth_est):
          * est = W⁻¹ Σ (b_j - b̂_j) z_j with W of the last stage (x + h) */
         splu_solve(&lu, err, err, tmp);
-        double en = wrms(err, u, un, n, rt, at);
+        double en = replay_xs ? 0.0 : wrms(err, u, un, n, rt, at);
         if (!isfinite(en)) {
             st->rejected++;
             h *= 0.2;
@@ -713,7 +723,7 @@ static int integrate(const orc_params* p, const orc_tables* t, const orc_cfg* c,
             continue;
         }
         if (en <= 1.0) {
-            double xn = last ? xend : x + h;
+            double xn = replay_xs ? replay_xs[nsteps - 1] : (last ? xend : x + h);
             orc_rhs(p, t, c, k, xn, un, fn, 1); /* FSAL: RSA overwrite persists in un */
             st->rhs_evals++;
             st->accepted++;
@@ -792,7 +802,7 @@ int orc_boltsolve(const orc_params* p, const orc_tables* t, const orc_cfg* c, do
                   int nsave, double* out, orc_stats* st) {
     save_t s = {x_save, nsave, 0, out, p, t, c, k, 0, NULL, NULL};
     obs_t o = {save_obs, &s};
-    return integrate(p, t, c, k, &o, st);
+    return integrate(p, t, c, k, &o, st, NULL, 0);
 }
 
 typedef struct { double* xs; int max, cnt; } steplog_t;
@@ -808,25 +818,35 @@ int orc_boltsolve_steps(const orc_params* p, const orc_tables* t, const orc_cfg*
                         int max_xs, int* n_xs, orc_stats* st) {
     steplog_t s = {xs, max_xs, 0};
     obs_t o = {step_obs, &s};
-    int r = integrate(p, t, c, k, &o, st);
+    int r = integrate(p, t, c, k, &o, st, NULL, 0);
     *n_xs = s.cnt;
     return r;
 }
 
 /* one solve, temperature (spectra.jl:6-23) and/or polarization (spectra.jl:25-42)
  * source columns at every x node; either output may be NULL */
-int orc_source_column_tp(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S_T,
-                         double* S_P, orc_stats* st) {
+static int source_column_run(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S_T,
+                             double* S_P, orc_stats* st, const double* replay_xs, int n_replay) {
     int n = orc_state_length(c);
     double* xs = (double*)malloc(sizeof(double) * t->nx);
     double* work = (double*)malloc(sizeof(double) * 2 * n);
     for (int j = 0; j < t->nx; ++j) xs[j] = xnode(t, j);
     save_t s = {xs, t->nx, 0, S_T, p, t, c, k, (S_T ? 1 : 0) | (S_P ? 2 : 0), work, S_P};
     obs_t o = {save_obs, &s};
-    int r = integrate(p, t, c, k, &o, st);
+    int r = integrate(p, t, c, k, &o, st, replay_xs, n_replay);
     free(xs);
     free(work);
     return r;
+}
+
+int orc_source_column_tp(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S_T,
+                         double* S_P, orc_stats* st) {
+    return source_column_run(p, t, c, k, S_T, S_P, st, NULL, 0);
+}
+
+int orc_source_column_replay(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                             const double* step_xs, int n_steps, double* S, orc_stats* st) {
+    return source_column_run(p, t, c, k, S, NULL, st, step_xs, n_steps);
 }
 
 int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k, double* S,

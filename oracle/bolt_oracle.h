@@ -92,6 +92,12 @@ int orc_source_column_tp(const orc_params* p, const orc_tables* t, const orc_cfg
 
 int orc_source_column(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
                       double* S, orc_stats* st);
+/* the temperature column along a fixed step sequence: the accepted steps whose
+ * end points orc_boltsolve_steps recorded, each accepted without error control.
+ * Smooth in (p, t), so central differences of it are the derivative of the
+ * fixed-step discrete solution (the tangent-linear solve's reference). */
+int orc_source_column_replay(const orc_params* p, const orc_tables* t, const orc_cfg* c, double k,
+                             const double* step_xs, int n_steps, double* S, orc_stats* st);
 /* source_grid over nk modes with nthreads OpenMP threads; S [nk][nx] */
 int orc_source_grid(const orc_params* p, const orc_tables* t, const orc_cfg* c, const double* k,
                     int nk, double* S, int* status, int nthreads);

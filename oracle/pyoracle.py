@@ -64,6 +64,7 @@ def lib():
         L.orc_boltsolve_rsa.argtypes = [P, T, C, _D, _PD, ctypes.POINTER(OrcStats)]
         L.orc_source_column.argtypes = [P, T, C, _D, _PD, ctypes.POINTER(OrcStats)]
         L.orc_source_column_tp.argtypes = [P, T, C, _D, _PD, _PD, ctypes.POINTER(OrcStats)]
+        L.orc_source_column_replay.argtypes = [P, T, C, _D, _PD, _I, _PD, ctypes.POINTER(OrcStats)]
         L.orc_source_P.argtypes = [P, T, C, _D, _D, _PD]
         L.orc_source_P.restype = _D
         L.orc_source_grid.argtypes = [P, T, C, _PD, _I, _PD, _PI, _I]
@@ -168,6 +169,15 @@ class Oracle:
         st = OrcStats()
         lib().orc_source_column(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k, _pd(S),
                                 ctypes.byref(st))
+        return S, st
+
+    def source_column_replay(self, cfg, k, step_xs):
+        """source column along the accepted steps ending at step_xs (from steps())"""
+        xs = np.ascontiguousarray(step_xs, dtype=np.float64)
+        S = np.zeros(self.nx)
+        st = OrcStats()
+        lib().orc_source_column_replay(ctypes.byref(self.par), ctypes.byref(self.tab), ctypes.byref(cfg), k,
+                                       _pd(xs), len(xs), _pd(S), ctypes.byref(st))
         return S, st
 
     def source_P(self, cfg, k, x, u):

@@ -39,7 +39,7 @@ def test_solver_shapes_match_oracle(lm):
     with ThreadPoolExecutor(3) as ex:
         refs = list(ex.map(lambda k: o.source_column(cosmo.ocfg(50, 50, lm, 15, 1e-11, 1e-6), float(k)), ks))
     grids = []
-    for sr in (5, 6, 7, 8):
+    for sr in (6, 7, 8):  # five rows per lane is not launched (see SR_RUN_MIN)
         ctx = _ctx("BOLT_SOLVER_SR", sr)
         sf = boltmi.source_grid(par, bg, ih, ks, None, 50, 1e-11, ℓ_ν=50, ℓ_mν=lm, ctx=ctx)
         ctx.close()
@@ -75,7 +75,7 @@ def test_autotune_picks_a_shape_and_later_calls_use_it():
     cfg = boltmi.hier_cfg(8, 8, 10, 15, 1e-8, 1e-6)
     dense = boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 800)
     rep = boltmi.autotune(par, bg, cfg, ks, dense, 600)
-    assert rep["solver_sr"] in (5, 6, 7, 8) and rep["los_spw"] in (1, 2, 4)
+    assert rep["solver_sr"] in (6, 7, 8) and rep["los_spw"] in (1, 2, 4)
     assert all(t > 0 for t in rep["solver_ms"].values()) and all(t > 0 for t in rep["los_ms"].values())
     assert rep["solver_ms"][rep["solver_sr"]] == min(rep["solver_ms"].values())
     # the tuned shapes give the pinned shapes' results bit for bit

@@ -206,11 +206,12 @@ def test_reference_ih_fwddiff():
     assert abs(ad - rich) <= 1e-5 * abs(rich)
 
 
-def test_dual_background_is_host_only():
+def test_dual_background_carries_its_tangent():
+    """a Dual background is a Dual to the host code and is marked for the
+    device's tangent upload; a plain-struct conversion of Dual parameters is refused"""
     bg = boltmi.Background(boltmi.CosmoParams(Ω_b=boltmi.Dual(0.046, 1.0)), x_grid=(-20.0, 0.0, 201))
     assert isinstance(bg.eta0, boltmi.Dual) and isinstance(bg.H(-1.0), boltmi.Dual)
-    with pytest.raises(TypeError):
-        bg.cosmo_id(object())
+    assert bg.is_dual and not boltmi.Background(boltmi.CosmoParams(), x_grid=(-20.0, 0.0, 201)).is_dual
     with pytest.raises(TypeError):
         boltmi.CosmoParams(h=boltmi.Dual(0.7, 1.0)).c_struct()
 
