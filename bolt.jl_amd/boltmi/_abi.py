@@ -64,7 +64,13 @@ class RequestC(ctypes.Structure):
 
 class ResultC(ctypes.Structure):
     _fields_ = [("state", _I), ("status", _I), ("queue_ms", _D), ("host_ms", _D), ("device_ms", _D),
-                ("latency_ms", _D)]
+                ("latency_ms", _D), ("solve_ms", _D), ("joined_running", _I), ("error", ctypes.c_char * 160)]
+
+
+class QueueInfoC(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_longlong), ("served", ctypes.c_longlong), ("joined_running", ctypes.c_longlong),
+                ("arena_bytes", ctypes.c_longlong), ("arena_peak_bytes", ctypes.c_longlong),
+                ("solver_workgroups", _I)]
 
 
 # every exported symbol and its signature: (restype, argtypes)
@@ -130,6 +136,8 @@ SIGNATURES = {
     "bolt_plin": (_I, [_VP, _I, ctypes.POINTER(HierCfgC), _PD, _I, _D, _PD, _PI, ctypes.POINTER(StatsC)]),
     "bolt_serve_start": (_I, [_VP, _I, _D]),
     "bolt_serve_slots": (_I, [_VP]),
+    "bolt_serve_start_queue": (_I, [_VP, _D, _I]),
+    "bolt_serve_queue_info": (_I, [_VP, ctypes.POINTER(QueueInfoC)]),
     "bolt_submit": (_I, [_VP, ctypes.POINTER(RequestC), ctypes.POINTER(ctypes.c_uint64)]),
     "bolt_poll": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),
     "bolt_wait": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),

@@ -700,16 +700,33 @@ class Server:
     """Continuous batching of spectrum requests on one context.
 
     Each request carries its own cosmology; the library builds its tables on
-    the host and runs it on a free slot (one HIP stream with preallocated HBM).
-    Requests on different slots run concurrently on the device, and each
-    result is the same as the computation run alone (plin / source_grid +
-    cltt_on_grid on the same k grid)."""
+    the host.  queue=False: a request runs on a free slot (one HIP stream with
+    preallocated HBM), and requests on different slots run concurrently.
+    queue=True: every request's k-modes join one persistent solver launch
+    through a work queue the host fills while it runs, and each request's
+    device memory is taken from an HBM arena by its own size
+    (bolt_serve_start_queue).  Either way each result is the same as the
+    computation run alone (plin / source_grid + cltt_on_grid on the same k
+    grid)."""
 
-    def __init__(self, slots=4, hbm_gib=8.0, ctx=None):
+    def __init__(self, slots=4, hbm_gib=8.0, ctx=None, queue=False, host_threads=4):
         self.ctx = ctx or default_context()
-        check(lib().bolt_serve_start(self.ctx.handle, int(slots), float(hbm_gib)))
-        self.slots = lib().bolt_serve_slots(self.ctx.handle)
+        self.queue = queue
+        if queue:
+            check(lib().bolt_serve_start_queue(self.ctx.handle, float(hbm_gib), int(host_threads)))
+            self.slots = 0
+        else:
+            check(lib().bolt_serve_start(self.ctx.handle, int(slots), float(hbm_gib)))
+            self.slots = lib().bolt_serve_slots(self.ctx.handle)
         self._out = {}
+
+    def queue_info(self):
+        """launches, requests served, of those admitted into a running launch, arena bytes and peak"""
+        q = _abi.QueueInfoC()
+        check(lib().bolt_serve_queue_info(self.ctx.handle, ctypes.byref(q)))
+        return dict(launches=q.launches, served=q.served, joined_running=q.joined_running,
+                    arena_bytes=q.arena_bytes, arena_peak_bytes=q.arena_peak_bytes,
+                    solver_workgroups=q.solver_workgroups)
 
     def _submit(self, par, kind, nk, kmin, kmax, k, ell_max, nkd, cfg, nout):
         out = np.zeros(nout)
@@ -738,7 +755,8 @@ class Server:
     @staticmethod
     def _info(r):
         return dict(state=r.state, status=r.status, queue_ms=r.queue_ms, host_ms=r.host_ms, device_ms=r.device_ms,
-                    latency_ms=r.latency_ms)
+                    latency_ms=r.latency_ms, solve_ms=r.solve_ms, joined_running=bool(r.joined_running),
+                    error=r.error.decode("utf-8", "replace"))
 
     def poll(self, rid):
         r = _abi.ResultC()
@@ -752,7 +770,7 @@ class Server:
         info = self._info(r)
         out = self._out.pop(rid)[0]
         if r.status != 0:
-            raise _abi.BoltError(r.status, f"request {rid} failed")
+            raise _abi.BoltError(r.status, f"request {rid} failed: {info['error']}")
         return out, info
 
     def close(self):

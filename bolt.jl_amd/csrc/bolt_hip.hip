@@ -1567,7 +1567,58 @@ struct SolveArgs {
     // outputs' derivatives (same layout as out)
     const CosmoDev* cosmos_dot = nullptr;
     double* out_dot = nullptr;
+    // serving from a work queue the host fills while the launch runs: modes,
+    // cosmologies and configurations live in pinned host memory; published
+    // counts the modes written so far; a mode's ModeDesc pad is its output mode
+    // and out its output's offset from `out` in doubles; status and counters are
+    // indexed by the mode's queue position and done[position] receives `epoch`
+    // when the mode's outputs are in memory.  A workgroup leaves once closed is
+    // set, once drain equals the launch's epoch, or after idle_limit polls
+    // without work; the first to leave stores the epoch to leave (device
+    // memory), after which no workgroup of the launch claims another mode.
+    int serve = 0;
+    int epoch = 0;
+    int idle_limit = 0;
+    int poll_sleeps = 4;  // s_sleep 127 (~3.4 µs each) between polls
+    const int* published = nullptr;
+    const int* closed = nullptr;
+    const int* drain = nullptr;
+    int* leave = nullptr;
+    int* done = nullptr;
 };
+
+// the next published mode of a serving launch (queue position), or INT_MAX
+// once the queue is closed, the host drains this launch, no mode has arrived
+// for idle_limit polls, or another workgroup of the launch has left for one of
+// these reasons (the launch ends as a whole: a few workgroups left behind
+// never take a new request's modes on their own); one thread per workgroup.
+// Positions are claimed by compare-and-swap, so no position is taken before
+// the host has published it.  A poll reads host memory: poll_sleeps sleeps
+// between polls keep that traffic small.  The polls are relaxed loads: an acquire at system
+// scope invalidates the L2, which the workgroups still solving would pay for
+// at every poll of every idle workgroup; the caller's one acquire fence after
+// a claim orders what the host wrote before publishing.
+__device__ inline int serve_claim(const SolveArgs& A) {
+    for (int idle = 0;; ++idle) {
+        if (__hip_atomic_load(A.leave, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == A.epoch) return INT_MAX;
+        const bool stop = __hip_atomic_load(A.closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+                          __hip_atomic_load(A.drain, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == A.epoch;
+        if (!stop) {
+            const int p = __hip_atomic_load(A.published, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            int c = __hip_atomic_load(A.work_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (c < p) {
+                if (__hip_atomic_compare_exchange_strong(A.work_counter, &c, c + 1, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    return c;
+            }
+        }
+        if (stop || idle >= A.idle_limit) {
+            __hip_atomic_store(A.leave, A.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return INT_MAX;
+        }
+        for (int s = 0; s < A.poll_sleeps; ++s) __builtin_amdgcn_s_sleep(127);
+    }
+}
 
 
 // Σ over the workgroup, the same value in every thread: wavefront sums, then
@@ -1599,7 +1650,10 @@ __device__ inline double wrms(const Geo& G, const WS& W, const double* e, const 
 // output at the node/save point xv from the interval (xa, ua, fa) .. (xb, ub, fb)
 __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const SolveArgs& A, int out,
                                   double k, double xa, const double* ua, const double* fa, double xb,
-                                  const double* ub, const double* fb, double xv, int idx, const Thr& T) {
+                                  const double* ub, const double* fb, double xv, int idx, const Thr& T,
+                                  int om, double* obase) {
+    // om: the output mode (A.out_mode, or the mode's own when serving); obase:
+    // a serving mode's output, or nullptr for the launch's own layout
     double h = xb - xa;
     if (h == 0.0) {
         for (int c = T.tid; c < G.n; c += T.nt) W.arg()[c] = ua[c];
@@ -1616,15 +1670,15 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
             W.arg()[c] = v;
         }
     }
-    if (A.out_mode == 0) {
-        double* dst = A.out + ((size_t)out * A.nsave + idx) * A.n_state_stride;
+    if (om == 0) {
+        double* dst = obase ? obase + (size_t)idx * G.n : A.out + ((size_t)out * A.nsave + idx) * A.n_state_stride;
         for (int c = T.tid; c < G.n; c += T.nt) dst[chain_to_state(G, c)] = W.arg()[c];
         return;
     }
     // sources at a grid node: hierarchy!(du, u) (mutating u, spectra.jl:16, 36)
     // on the rows source_function reads, then source_function and/or
     // source_function_P
-    const bool want_t = (A.out_mode & 1) != 0, want_p = (A.out_mode & 2) != 0;
+    const bool want_t = (om & 1) != 0, want_p = (om & 2) != 0;
     XC X = make_xc(C, k, xv);
     qfill(C, G, X, W, T);
     __syncthreads();
@@ -1640,7 +1694,7 @@ __device__ inline void emit(const CosmoDev& C, const Geo& G, const WS& W, const 
     __syncthreads();
     if (want_t && T.wv == 0) {
         double s = source_fn(C, G, X, W, k, W.arg(), W.w(), F.P, T.lane);
-        if (T.lane == 0) A.out[(size_t)out * C.nx + idx] = s;
+        if (T.lane == 0) (obase ? obase + idx : A.out + (size_t)out * C.nx + idx)[0] = s;
     }
     if (want_p && T.tid == T.nt - 1) A.out2[(size_t)out * C.nx + idx] = source_fn_P(C, G, X, k, W.arg());
     __syncthreads();
@@ -1747,7 +1801,17 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
     __shared__ int s_mode;
     const Thr T = thr_self();
     for (;;) {
-        if (T.tid == 0) s_mode = atomicAdd(A.work_counter, 1);
+        if (T.tid == 0) {
+            if (A.serve) {
+                const int c = serve_claim(A);
+                // what the host wrote before publishing (tables, descriptors) is
+                // read fresh after this: system-scope acquire
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+                s_mode = c;
+            } else {
+                s_mode = atomicAdd(A.work_counter, 1);
+            }
+        }
         __syncthreads();
         const int mi = s_mode;
         if (mi >= A.nmodes) return;
@@ -1789,16 +1853,18 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
         const int nref = (cf.flags & BOLT_CFG_REFINE_MASK) ? (cf.flags & BOLT_CFG_REFINE_MASK) >> 4 : 12;
         int status = BOLT_MODE_OK;
         int next = 0;  // next output index
-        const int nout = (A.out_mode == 0) ? A.nsave : C.nx;
+        const int om = A.serve ? md.pad : A.out_mode;
+        double* const obase = A.serve ? A.out + (size_t)(unsigned)md.out : nullptr;
+        const int nout = (om == 0) ? A.nsave : C.nx;
         // outputs at the starting point take the initial condition as saved
         while (next < nout) {
-            double xv = (A.out_mode == 0) ? A.x_save[next] : node_x(C, next);
+            double xv = (om == 0) ? A.x_save[next] : node_x(C, next);
             if (!(xv <= x)) break;
             if constexpr (TAN)
                 emit_tan(C, *Cdp, G, W, Wt, A, md.out, k, x, W.u(), W.u(), Wt.ud(), Wt.ud(), x, W.u(), W.u(), Wt.ud(),
                          Wt.ud(), xv, next, T);
             else
-                emit(C, G, W, A, md.out, k, x, W.u(), W.u(), x, W.u(), W.u(), xv, next, T);
+                emit(C, G, W, A, md.out, k, x, W.u(), W.u(), x, W.u(), W.u(), xv, next, T, om, obase);
             ++next;
         }
         // FSAL start: f(u0) (the RSA overwrite, if on, persists in u)
@@ -2142,13 +2208,13 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
                 __syncthreads();
                 ++nacc;
                 while (next < nout) {
-                    double xv = (A.out_mode == 0) ? A.x_save[next] : node_x(C, next);
+                    double xv = (om == 0) ? A.x_save[next] : node_x(C, next);
                     if (!(xv <= xn)) break;
                     if constexpr (TAN)
                         emit_tan(C, *Cdp, G, W, Wt, A, md.out, k, x, W.u(), W.f(), Wt.ud(), Wt.fd(), xn, W.un(), W.fn(),
                                  Wt.und(), Wt.fnd(), xv, next, T);
                     else
-                        emit(C, G, W, A, md.out, k, x, W.u(), W.f(), xn, W.un(), W.fn(), xv, next, T);
+                        emit(C, G, W, A, md.out, k, x, W.u(), W.f(), xn, W.un(), W.fn(), xv, next, T, om, obase);
                     ++next;
                 }
                 take = true;  // u, f <- u_{n+1}, f_{n+1} in the next step's first pass
@@ -2164,10 +2230,17 @@ __global__ void __launch_bounds__(NTHR) k_boltsolve(SolveArgs A) {
             }
         }
         if (T.tid == 0) {
-            A.status[md.out] = status;
+            const int si = A.serve ? mi : md.out;
+            A.status[si] = status;
             if (A.counters) {
-                long long* cc = A.counters + (size_t)md.out * 4;
+                long long* cc = A.counters + (size_t)si * 4;
                 cc[0] = nacc; cc[1] = nrej; cc[2] = nrhs; cc[3] = nsol;
+            }
+            if (A.serve) {
+                // the workgroup's outputs (ordered before this thread by the
+                // barrier above) reach memory, then the host's flag is raised
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                __hip_atomic_store(A.done + mi, A.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
         __syncthreads();
@@ -2348,13 +2421,14 @@ __global__ void k_rsa_post(const CosmoDev* cosmos, const CfgDev* cfgs, const Mod
 
 // plin's gauge transform and normalisation (spectra.jl:169-197), one thread per mode
 __global__ void k_plin_post(const CosmoDev* cosmos, const CfgDev* cfgs, const ModeDesc* modes, int B, double x,
-                            const double* states, double* pk) {
+                            const double* states, double* pk, int stride) {
+    // stride: doubles between consecutive modes' states (0: the state length)
     int mi = blockIdx.x * blockDim.x + threadIdx.x;
     if (mi >= B) return;
     ModeDesc md = modes[mi];
     const CosmoDev C = cosmos[md.cosmo];
     const CfgDev cf = cfgs[md.cfg];
-    const double* u = states + (size_t)md.out * cf.n;
+    const double* u = states + (size_t)md.out * (stride ? stride : cf.n);
     double k = md.k, a = exp(x);
     int ng = cf.ng, nn = cf.nn, nm = cf.nm, nq = cf.nq;
     int iM = 2 * (ng + 1) + nn + 1, iPhi = iM + (nm + 1) * nq;
@@ -3032,6 +3106,8 @@ struct CosmoSlot {
 
 struct Server;
 static void server_stop(Server* sv);
+struct QServer;
+static void qserver_stop(QServer* sv);
 
 // A device buffer a context keeps between its calls and only ever grows.
 // hipFree waits for the whole device, so a call that allocated and freed its
@@ -3071,7 +3147,8 @@ struct bolt_ctx {
     std::mutex mu;
     // launch descriptors and LOS work arrays of calls on `stream` (under mu)
     GBuf c_modes, c_cfgs, c_cosmos, c_counter, c_lidx, c_pmap, c_eta, c_dx, c_contrib, c_btab, c_cosmos_dot, c_etad;
-    Server* server = nullptr;  // request server (bolt_serve_start), if running
+    std::shared_ptr<Server> server;    // request server (bolt_serve_start), if running
+    std::shared_ptr<QServer> qserver;  // queue server (bolt_serve_start_queue), if running
     std::mutex server_mu;
 };
 
@@ -3176,10 +3253,9 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
         c->lds_max = lds_max - 1024;  // headroom for static __shared__
         for (const void* f : {(const void*)k_boltsolve<5, false>, (const void*)k_boltsolve<6, false>,
                               (const void*)k_boltsolve<7, false>, (const void*)k_boltsolve<8, false>,
-                              (const void*)k_boltsolve<SR_DEFAULT, true>})
+                              (const void*)k_boltsolve<SR_DEFAULT, true>, (const void*)k_boltsolve<SR_DEFAULT, false, true>,
+                              (const void*)k_rhs, (const void*)k_ics, (const void*)k_rhs_jvp, (const void*)k_ics_jvp})
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
-        (void)hipFuncSetAttribute((const void*)k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
-        (void)hipFuncSetAttribute((const void*)k_ics, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
         (void)hipGetLastError();  // an unsupported opt-in must not surface at the next launch check
     }
     {
@@ -3197,11 +3273,7 @@ int bolt_ctx_create(int device, bolt_ctx** out) {
 
 int bolt_ctx_destroy(bolt_ctx* ctx) {
     if (!ctx) return BOLT_OK;
-    {
-        std::lock_guard<std::mutex> lk(ctx->server_mu);
-        if (ctx->server) server_stop(ctx->server);
-        ctx->server = nullptr;
-    }
+    (void)bolt_serve_stop(ctx);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& s : ctx->cosmos) {
@@ -4155,7 +4227,7 @@ int bolt_plin(bolt_ctx* ctx, int cosmo_id, const bolt_hier_cfg* cfg, const doubl
     HIPCHK(hipMemcpyAsync(dm.p, md.data(), sizeof(ModeDesc) * nk, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dc.p, &cd, sizeof(CfgDev), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_plin_post, dim3((nk + 127) / 128), dim3(128), 0, ctx->stream, cs->d_self, dc.as<CfgDev>(),
-                       dm.as<ModeDesc>(), nk, x, dout.as<double>(), dpk.as<double>());
+                       dm.as<ModeDesc>(), nk, x, dout.as<double>(), dpk.as<double>(), 0);
     HIPCHK(hipGetLastError());
     std::vector<long long> ctr(4 * (size_t)nk);
     HIPCHK(hipMemcpyAsync(pk, dpk.p, sizeof(double) * nk, hipMemcpyDeviceToHost, ctx->stream));
@@ -4937,9 +5009,10 @@ static int validate_request(const bolt_request* r) {
 
 // one request on one slot; returns its status, *host_ms / *device_ms the
 // host-table and device times
-static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, double* host_ms, double* device_ms) {
-    const double t0 = now_ms();
-    std::vector<double> coef, xq, wq, scal(BOLT_NSCAL);
+// a request's Background + RECFAST tables on the host (x grid -20:0.01:0)
+static int request_tables(const bolt_request& rq, std::vector<double>& coef, std::vector<double>& scal,
+                          std::vector<double>& xq, std::vector<double>& wq) {
+    scal.assign(BOLT_NSCAL, 0.0);
     try {
         bolt::Background bg(rq.par, -20.0, 0.0, SV_NX, rq.cfg.nq);
         bg.ionization(-1.0);
@@ -4952,6 +5025,37 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
         return set_error(BOLT_E_NUMERIC, std::string("host tables: ") + e.what());
     }
     for (double v : coef) if (!std::isfinite(v)) return set_error(BOLT_E_NUMERIC, "host tables: non-finite entry");
+    return BOLT_OK;
+}
+
+// the request's k grid: given, or log-spaced as the full-CMB workload's
+static std::vector<double> request_k(const bolt_request& rq) {
+    std::vector<double> kh(rq.nk);
+    if (rq.k) {
+        std::memcpy(kh.data(), rq.k, sizeof(double) * rq.nk);
+    } else {
+        const double l0 = std::log10(rq.kmin), l1 = std::log10(rq.kmax);
+        for (int i = 0; i < rq.nk; ++i) kh[i] = std::pow(10.0, l0 + (l1 - l0) * i / (rq.nk - 1));
+    }
+    return kh;
+}
+
+// quadratic_k(0.01H₀, 1000H₀, nkd) (spectra.jl:60-63, 133)
+static std::vector<double> request_dense(double H0, int nkd) {
+    std::vector<double> dense(nkd);
+    const double kmin = 0.01 * H0, kmax = 1000.0 * H0;
+    for (int i = 1; i <= nkd; ++i) {
+        const double f = (double)i / nkd;
+        dense[i - 1] = kmin + (kmax - kmin) * (f * f);
+    }
+    return dense;
+}
+
+static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, double* host_ms, double* device_ms) {
+    const double t0 = now_ms();
+    std::vector<double> coef, xq, wq, scal;
+    int rt = request_tables(rq, coef, scal, xq, wq);
+    if (rt) return rt;
     bolt_tables t;
     t.nx = SV_NX; t.nq = rq.cfg.nq; t.x0 = -20.0; t.x1 = 0.0;
     for (int i = 0; i < BOLT_NSCAL; ++i) t.scal[i] = scal[i];
@@ -4967,13 +5071,7 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
     HIPCHK(hipMemcpyAsync(sl.d_q, qbuf.data(), sizeof(double) * qbuf.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(sl.d_self, &C, sizeof(CosmoDev), hipMemcpyHostToDevice, st));
     const int nk = rq.nk;
-    std::vector<double> kh(nk);  // given, or log-spaced as the k grids of the full-CMB workload
-    if (rq.k) {
-        std::memcpy(kh.data(), rq.k, sizeof(double) * nk);
-    } else {
-        const double l0 = std::log10(rq.kmin), l1 = std::log10(rq.kmax);
-        for (int i = 0; i < nk; ++i) kh[i] = std::pow(10.0, l0 + (l1 - l0) * i / (nk - 1));
-    }
+    const std::vector<double> kh = request_k(rq);
     HIPCHK(hipMemcpyAsync(sl.d_k, kh.data(), sizeof(double) * nk, hipMemcpyHostToDevice, st));
     std::vector<ModeDesc> md(nk);
     for (int i = 0; i < nk; ++i) { md[i].k = kh[i]; md[i].cosmo = 0; md[i].cfg = 0; md[i].out = i; md[i].pad = 0; }
@@ -4988,7 +5086,7 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
                               sl.d_ctr, cd.n, &ms)))
             return r;
         hipLaunchKernelGGL(k_plin_post, dim3((nk + 127) / 128), dim3(128), 0, st, sl.sb.cosmos, sl.sb.cfgs,
-                           sl.sb.modes, nk, x, sl.d_states, sl.d_pk);
+                           sl.sb.modes, nk, x, sl.d_states, sl.d_pk, 0);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(rq.out, sl.d_pk, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
     } else {
@@ -4996,12 +5094,7 @@ static int run_request(bolt_ctx* ctx, ServeSlot& sl, const bolt_request& rq, dou
                               sl.d_ctr, 0, &ms)))
             return r;
         const int nkd = rq.nkd;
-        std::vector<double> dense(nkd);  // quadratic_k(0.01H₀, 1000H₀, nkd) (spectra.jl:60-63, 133)
-        const double kmin = 0.01 * C.H0, kmax = 1000.0 * C.H0;
-        for (int i = 1; i <= nkd; ++i) {
-            const double f = (double)i / nkd;
-            dense[i - 1] = kmin + (kmax - kmin) * (f * f);
-        }
+        const std::vector<double> dense = request_dense(C.H0, nkd);
         HIPCHK(hipMemcpyAsync(sl.d_dense, dense.data(), sizeof(double) * nkd, hipMemcpyHostToDevice, st));
         std::vector<int> ells(rq.ell_max - 1);
         for (int l = 2; l <= rq.ell_max; ++l) ells[l - 2] = l;
@@ -5041,6 +5134,7 @@ static void worker_loop(Server* sv, int w) {
             bolt_result& r = sv->results[job.id];
             r.state = BOLT_REQ_DONE;
             r.status = st;
+            if (st) std::snprintf(r.error, sizeof(r.error), "%s", bolt_last_error());
             r.queue_ms = t0 - job.t_submit;
             r.host_ms = host_ms;
             r.device_ms = dev_ms;
@@ -5048,6 +5142,545 @@ static void worker_loop(Server* sv, int w) {
         }
         sv->cv_done.notify_all();
     }
+}
+
+// ------------------------------------------------------------ queue serving
+// bolt_serve_start_queue: requests join a running solver launch.  Each
+// request's device memory (tables, k grids, outputs) comes out of one HBM
+// arena, first fit, sized by the request itself; a pool of host threads builds
+// the requests' tables; a publisher uploads each request's tables, cosmology
+// and k-modes (the work queue, in device memory) and then raises the published
+// count in pinned host memory, which the workgroups of the running persistent
+// launch poll (serve_claim).  The launcher drains a launch once the server has
+// held no request for QS_DRAIN_MS and starts a new one whenever published
+// modes are left unclaimed.  A completion thread watches the per-mode done
+// flags (pinned host memory), runs the request's C_ℓ or P(k) step on its own
+// stream, on CUs the solver's stream is masked off, and copies the result out.
+namespace {
+constexpr int QS_QCAP = 1 << 19;     // queue positions before the queue is rewound
+constexpr int QS_RCAP = 256;         // requests published and not yet done
+constexpr double QS_IDLE_US = 2e6;    // ~2 s without work before a workgroup leaves on its own
+constexpr double QS_DRAIN_MS = 5.0;    // the launcher drains a launch after this long with no request in the server
+constexpr int QS_XCD = 8;           // MI355X: 8 XCDs, each with its own L2
+constexpr int QS_POLL_SLEEPS = 4;
+constexpr size_t QS_ALIGN = 256;
+}  // namespace
+
+struct QArena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    size_t used = 0, peak = 0;
+    void init(size_t bytes) { size = bytes; free_.clear(); free_[0] = bytes; used = peak = 0; }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = (bytes + QS_ALIGN - 1) / QS_ALIGN * QS_ALIGN;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes, o2 = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[o2] = rest;
+            used += bytes;
+            peak = std::max(peak, used);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + QS_ALIGN - 1) / QS_ALIGN * QS_ALIGN;
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+};
+
+struct QReq {
+    uint64_t id;
+    bolt_request rq;
+    double t_submit = 0, t_start = 0, host_ms = 0, t_pub = 0, t_solved = 0;
+    std::vector<double> coef, scal, xq, wq, kh;
+    size_t off = 0, bytes = 0;
+    int slot = -1, q0 = 0, epoch_pub = 0, status = BOLT_OK;
+    size_t o_coef = 0, o_q = 0, o_k = 0, o_out = 0, o_dense = 0, o_cl = 0, o_pk = 0, o_md = 0;
+    CosmoDev C;
+};
+
+struct QServer {
+    bolt_ctx* ctx = nullptr;
+    char* arena_d = nullptr;
+    QArena arena;
+    // pinned host memory: staging of the device copies below
+    ModeDesc* ring = nullptr;
+    CosmoDev* cosmos = nullptr;
+    CfgDev* cfgs = nullptr;
+    // pinned host memory the device polls and writes
+    int* ctl = nullptr;  // [0] published, [1] closed, [2] epoch of the launch to drain
+    int* done = nullptr;
+    // device memory: the work queue and the request slots' cosmologies and configurations
+    ModeDesc* d_ring = nullptr;
+    CosmoDev* d_cosmos = nullptr;
+    CfgDev* d_cfgs = nullptr;
+    int* d_counter = nullptr;
+    int* d_leave = nullptr;  // epoch of the last launch a workgroup has left
+    int* d_status = nullptr;
+    long long* d_ctr = nullptr;
+    double* d_xzero = nullptr;
+    LosBufs lb;
+    std::vector<void*> dallocs;
+    hipStream_t s_solve = nullptr, s_copy = nullptr, s_post = nullptr;
+    size_t lds = 0;
+    int grid = 0;
+    int poll_sleeps = QS_POLL_SLEEPS;
+    bool cu_mask = true;  // the solver and the post-step streams on disjoint CU masks
+    std::mutex mu;  // everything below
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<QReq>> pending;       // waiting for host tables
+    std::deque<std::shared_ptr<QReq>> ready;         // tables built, waiting for memory and publication
+    std::vector<std::shared_ptr<QReq>> inflight;     // published, not done
+    std::unordered_map<uint64_t, bolt_result> results;
+    std::vector<int> free_slots;
+    uint64_t next_id = 1;
+    int building = 0;       // requests whose host tables are being built
+    int unsettled = 0;      // requests holding queue positions and not yet done
+    bool failed = false;    // a device error: requests in flight end with it
+    int published = 0;
+    int epoch = 0;          // launches so far
+    bool running = false;   // a solver launch is in flight
+    bool stopping = false;
+    long long launches = 0, joined = 0, served = 0;
+    std::vector<std::thread> threads;
+};
+
+static size_t qreq_layout(QReq& q) {
+    const bolt_request& rq = q.rq;
+    size_t b = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = b;
+        b += (bytes + QS_ALIGN - 1) / QS_ALIGN * QS_ALIGN;
+        return o;
+    };
+    q.o_coef = take(sizeof(double) * (size_t)BOLT_NTAB * (SV_NX + 2));
+    q.o_q = take(sizeof(double) * 4 * (size_t)rq.cfg.nq);
+    q.o_k = take(sizeof(double) * (size_t)rq.nk);
+    if (rq.kind == BOLT_REQ_CLTT) {
+        q.o_out = take(sizeof(double) * (size_t)rq.nk * SV_NX);
+        q.o_dense = take(sizeof(double) * (size_t)rq.nkd);
+        q.o_cl = take(sizeof(double) * (size_t)(rq.ell_max - 1));
+    } else {
+        q.o_out = take(sizeof(double) * (size_t)rq.nk * bolt_state_length(&rq.cfg));
+        q.o_pk = take(sizeof(double) * (size_t)rq.nk);
+        q.o_md = take(sizeof(ModeDesc) * (size_t)rq.nk);
+    }
+    return b;
+}
+
+static void qs_finish(QServer* sv, const std::shared_ptr<QReq>& q, int status) {
+    bolt_result& r = sv->results[q->id];
+    r.state = BOLT_REQ_DONE;
+    r.status = status;
+    if (status) std::snprintf(r.error, sizeof(r.error), "%s", bolt_last_error());
+    r.queue_ms = q->t_start - q->t_submit;
+    r.host_ms = q->host_ms;
+    r.latency_ms = now_ms() - q->t_submit;
+}
+
+// host tables, one request at a time per thread
+static void qs_tables_loop(QServer* sv) {
+    for (;;) {
+        std::shared_ptr<QReq> q;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv.wait(lk, [&] { return sv->stopping || !sv->pending.empty(); });
+            if (sv->pending.empty()) return;
+            q = sv->pending.front();
+            sv->pending.pop_front();
+            ++sv->building;
+            q->t_start = now_ms();
+            sv->results[q->id].state = BOLT_REQ_RUNNING;
+        }
+        const double t0 = now_ms();
+        const int r = request_tables(q->rq, q->coef, q->scal, q->xq, q->wq);
+        q->kh = request_k(q->rq);
+        q->host_ms = now_ms() - t0;
+        std::lock_guard<std::mutex> lk(sv->mu);
+        --sv->building;
+        if (r) {
+            qs_finish(sv, q, r);
+        } else {
+            sv->ready.push_back(q);
+        }
+        sv->cv.notify_all();
+    }
+}
+
+// memory, upload and publication, in arrival order; then the launcher
+static int qs_publish(QServer* sv, QReq& q) {
+    const bolt_request& rq = q.rq;
+    char* base = sv->arena_d + q.off;
+    bolt_tables t;
+    t.nx = SV_NX; t.nq = rq.cfg.nq; t.x0 = -20.0; t.x1 = 0.0;
+    for (int i = 0; i < BOLT_NSCAL; ++i) t.scal[i] = q.scal[i];
+    t.xq = q.xq.data(); t.wq = q.wq.data(); t.coef = q.coef.data();
+    std::vector<double> qbuf;
+    make_cosmo_host(rq.par, t, q.C, qbuf);
+    bind_cosmo_device(q.C, (double*)(base + q.o_coef), (double*)(base + q.o_q));
+    hipStream_t st = sv->s_copy;
+    HIPCHK(hipMemcpyAsync(base + q.o_coef, q.coef.data(), sizeof(double) * q.coef.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + q.o_q, qbuf.data(), sizeof(double) * qbuf.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + q.o_k, q.kh.data(), sizeof(double) * rq.nk, hipMemcpyHostToDevice, st));
+    std::vector<double> dense;
+    if (rq.kind == BOLT_REQ_CLTT) {
+        dense = request_dense(q.C.H0, rq.nkd);
+        HIPCHK(hipMemcpyAsync(base + q.o_dense, dense.data(), sizeof(double) * rq.nkd, hipMemcpyHostToDevice, st));
+    } else {
+        std::vector<ModeDesc> pm(rq.nk);
+        for (int i = 0; i < rq.nk; ++i) { pm[i].k = q.kh[i]; pm[i].cosmo = q.slot; pm[i].cfg = q.slot; pm[i].out = i; pm[i].pad = 0; }
+        HIPCHK(hipMemcpyAsync(base + q.o_md, pm.data(), sizeof(ModeDesc) * rq.nk, hipMemcpyHostToDevice, st));
+    }
+    sv->cosmos[q.slot] = q.C;
+    sv->cfgs[q.slot] = to_cfgdev(&rq.cfg);
+    // the request's modes, largest k (most steps) first
+    const int n = bolt_state_length(&rq.cfg);
+    const size_t out0 = (q.off + q.o_out) / sizeof(double);
+    for (int j = 0; j < rq.nk; ++j) {
+        const int i = rq.nk - 1 - j;
+        ModeDesc& m = sv->ring[q.q0 + j];
+        m.k = q.kh[i];
+        m.cosmo = q.slot;
+        m.cfg = q.slot;
+        m.out = (int)(out0 + (size_t)i * (rq.kind == BOLT_REQ_CLTT ? SV_NX : n));
+        m.pad = rq.kind == BOLT_REQ_CLTT ? 1 : 0;
+        sv->done[q.q0 + j] = 0;
+    }
+    HIPCHK(hipMemcpyAsync(sv->d_cosmos + q.slot, sv->cosmos + q.slot, sizeof(CosmoDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv->d_cfgs + q.slot, sv->cfgs + q.slot, sizeof(CfgDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv->d_ring + q.q0, sv->ring + q.q0, sizeof(ModeDesc) * rq.nk, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    __atomic_store_n(&sv->ctl[0], q.q0 + rq.nk, __ATOMIC_RELEASE);
+    q.t_pub = now_ms();
+    return BOLT_OK;
+}
+
+static void qs_admit_loop(QServer* sv) {
+    (void)hipSetDevice(sv->ctx->device);
+    for (;;) {
+        std::shared_ptr<QReq> q;
+        int running_before = 0;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            for (;;) {
+                if (sv->stopping && sv->ready.empty() && sv->pending.empty() && !sv->building) return;
+                if (sv->failed) {
+                    for (auto& r : sv->ready) qs_finish(sv, r, set_error(BOLT_E_DEVICE, "serving: device error"));
+                    sv->ready.clear();
+                }
+                if (!sv->ready.empty()) {
+                    std::shared_ptr<QReq>& f = sv->ready.front();
+                    if (!f->bytes) f->bytes = qreq_layout(*f);
+                    const bool room = sv->published + f->rq.nk <= QS_QCAP && !sv->free_slots.empty();
+                    if (room && sv->arena.alloc(f->bytes, &f->off)) {
+                        q = f;
+                        sv->ready.pop_front();
+                        q->slot = sv->free_slots.back();
+                        sv->free_slots.pop_back();
+                        q->q0 = sv->published;
+                        sv->published += q->rq.nk;
+                        ++sv->unsettled;
+                        running_before = sv->running ? sv->epoch : 0;
+                        break;
+                    }
+                }
+                sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+            }
+        }
+        const int r = qs_publish(sv, *q);
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (r) {  // nothing was published for it: its positions hold no work (status stays 0, done 0)
+            sv->arena.release(q->off, q->bytes);
+            sv->free_slots.push_back(q->slot);
+            --sv->unsettled;
+            sv->failed = true;
+            qs_finish(sv, q, r);
+        } else {
+            // admitted into a running launch: the same launch was running before and after publication
+            q->epoch_pub = (running_before && sv->running && sv->epoch == running_before) ? running_before : 0;
+            sv->inflight.push_back(q);
+        }
+        sv->cv.notify_all();
+    }
+}
+
+// solver launches: whenever published modes are left unclaimed; the queue is
+// rewound when every published mode is done
+static void qs_launch_loop(QServer* sv) {
+    (void)hipSetDevice(sv->ctx->device);
+    int claimed = 0;
+    for (;;) {
+        int ep;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            for (;;) {
+                if (sv->failed) return;
+                if (__atomic_load_n(&sv->ctl[0], __ATOMIC_ACQUIRE) > claimed) break;
+                if (!sv->unsettled && sv->published > 0 && sv->published == claimed) {
+                    // rewind: nothing published is pending (ready requests publish afresh from 0)
+                    int zero = 0;
+                    if (hipMemcpyAsync(sv->d_counter, &zero, sizeof(int), hipMemcpyHostToDevice, sv->s_solve) ==
+                            hipSuccess &&
+                        hipStreamSynchronize(sv->s_solve) == hipSuccess) {
+                        __atomic_store_n(&sv->ctl[0], 0, __ATOMIC_RELEASE);
+                        sv->published = 0;
+                        claimed = 0;
+                    }
+                }
+                if (sv->stopping && sv->pending.empty() && sv->ready.empty() && !sv->building && !sv->unsettled)
+                    return;
+                sv->cv.wait_for(lk, std::chrono::milliseconds(1));
+            }
+            ep = ++sv->epoch;
+            sv->running = true;
+            ++sv->launches;
+        }
+        SolveArgs A;
+        A.cosmos = sv->d_cosmos; A.cfgs = sv->d_cfgs; A.modes = sv->d_ring; A.nmodes = QS_QCAP;
+        A.work_counter = sv->d_counter; A.out_mode = 0; A.x_save = sv->d_xzero; A.nsave = 1;
+        A.out = (double*)sv->arena_d; A.out2 = nullptr; A.status = sv->d_status; A.counters = sv->d_ctr;
+        A.n_state_stride = 0; A.lds_doubles = (int)(sv->lds / sizeof(double));
+        A.serve = 1; A.epoch = ep; A.poll_sleeps = sv->poll_sleeps;
+        A.idle_limit = (int)std::max(1000.0, QS_IDLE_US / (3.4 * sv->poll_sleeps + 1.0));
+        A.published = sv->ctl; A.closed = sv->ctl + 1; A.drain = sv->ctl + 2; A.leave = sv->d_leave;
+        A.done = sv->done;
+        launch_solver(SR_DEFAULT, false, sv->grid, sv->lds, sv->s_solve, A);
+        bool ok = hipGetLastError() == hipSuccess;
+        // wait for the launch; drain it once the server has held no request for QS_DRAIN_MS
+        double idle_since = -1.0;
+        while (ok) {
+            const hipError_t e = hipStreamQuery(sv->s_solve);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) { ok = false; break; }
+            std::unique_lock<std::mutex> lk(sv->mu);
+            const bool idle = !sv->unsettled && sv->ready.empty() && sv->pending.empty() && !sv->building;
+            if (!idle) {
+                idle_since = -1.0;
+            } else if (idle_since < 0) {
+                idle_since = now_ms();
+            } else if (now_ms() - idle_since > QS_DRAIN_MS || sv->stopping) {
+                __atomic_store_n(&sv->ctl[2], ep, __ATOMIC_RELEASE);
+            }
+            sv->cv.wait_for(lk, std::chrono::microseconds(200));
+        }
+        int c = claimed;
+        ok = ok && hipStreamSynchronize(sv->s_solve) == hipSuccess &&
+             hipMemcpy(&c, sv->d_counter, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->running = false;
+        if (ok) claimed = c;
+        else sv->failed = true;  // a device error: the requests in flight report it
+        sv->cv.notify_all();
+    }
+}
+
+// completion: every mode of a request done -> its C_ℓ or P(k), then the result
+static void qs_complete_loop(QServer* sv) {
+    (void)hipSetDevice(sv->ctx->device);
+    for (;;) {
+        std::shared_ptr<QReq> q;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            for (;;) {
+                for (auto it = sv->inflight.begin(); it != sv->inflight.end(); ++it) {
+                    QReq& r = **it;
+                    bool all = true;
+                    for (int j = 0; j < r.rq.nk && all; ++j) all = __atomic_load_n(&sv->done[r.q0 + j], __ATOMIC_ACQUIRE) != 0;
+                    if (all) { q = *it; q->t_solved = now_ms(); sv->inflight.erase(it); break; }
+                }
+                if (q) break;
+                if (sv->failed && !sv->running) {
+                    for (auto& r : sv->inflight) {
+                        qs_finish(sv, r, set_error(BOLT_E_DEVICE, "serving: device error"));
+                        sv->arena.release(r->off, r->bytes);
+                        sv->free_slots.push_back(r->slot);
+                        --sv->unsettled;
+                    }
+                    sv->inflight.clear();
+                }
+                if (sv->stopping && sv->pending.empty() && sv->ready.empty() && !sv->building && !sv->unsettled)
+                    return;
+                sv->cv.wait_for(lk, std::chrono::microseconds(200));
+            }
+        }
+        const bolt_request& rq = q->rq;
+        char* base = sv->arena_d + q->off;
+        bool joined = q->epoch_pub != 0;
+        for (int j = 0; j < rq.nk && joined; ++j) joined = sv->done[q->q0 + j] == q->epoch_pub;
+        std::vector<int> status(rq.nk);
+        int r = BOLT_OK;
+        hipStream_t st = sv->s_post;
+        const double tp0 = now_ms();
+        if (hipMemcpyAsync(status.data(), sv->d_status + q->q0, sizeof(int) * rq.nk, hipMemcpyDeviceToHost, st) !=
+            hipSuccess)
+            r = set_error(BOLT_E_DEVICE, "serving: status copy failed");
+        if (!r && rq.kind == BOLT_REQ_CLTT) {
+            std::vector<int> ells(rq.ell_max - 1);
+            for (int l = 2; l <= rq.ell_max; ++l) ells[l - 2] = l;
+            float ms = 0;
+            r = cl_core(sv->ctx, st, &sv->lb, q->C, sv->d_cosmos + q->slot, (double*)(base + q->o_out), nullptr,
+                        (double*)(base + q->o_k), rq.nk, (double*)(base + q->o_dense), rq.nkd, ells.data(),
+                        (int)ells.size(), 0, (double*)(base + q->o_cl), nullptr, nullptr, nullptr, &ms);
+            if (!r && hipMemcpyAsync(rq.out, base + q->o_cl, sizeof(double) * ells.size(), hipMemcpyDeviceToHost, st) !=
+                          hipSuccess)
+                r = set_error(BOLT_E_DEVICE, "serving: result copy failed");
+        } else if (!r) {
+            hipLaunchKernelGGL(k_plin_post, dim3((rq.nk + 127) / 128), dim3(128), 0, st, (const CosmoDev*)sv->d_cosmos,
+                               (const CfgDev*)sv->d_cfgs, (const ModeDesc*)(base + q->o_md), rq.nk, 0.0,
+                               (const double*)(base + q->o_out), (double*)(base + q->o_pk),
+                               bolt_state_length(&rq.cfg));
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(rq.out, base + q->o_pk, sizeof(double) * rq.nk, hipMemcpyDeviceToHost, st) != hipSuccess)
+                r = set_error(BOLT_E_DEVICE, "serving: plin post step failed");
+        }
+        if (hipStreamSynchronize(st) != hipSuccess && !r) r = set_error(BOLT_E_DEVICE, "serving: stream failed");
+        const double dev_ms = now_ms() - tp0;
+        for (int i = 0; i < rq.nk && !r; ++i)
+            if (status[i] != BOLT_MODE_OK) r = set_error(BOLT_E_NUMERIC, "a mode failed");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->arena.release(q->off, q->bytes);
+        sv->free_slots.push_back(q->slot);
+        --sv->unsettled;
+        qs_finish(sv, q, r);
+        bolt_result& res = sv->results[q->id];
+        res.device_ms = dev_ms;
+        res.solve_ms = q->t_solved - q->t_pub;
+        res.joined_running = joined ? 1 : 0;
+        sv->joined += joined ? 1 : 0;
+        ++sv->served;
+        sv->cv.notify_all();
+    }
+}
+
+// the server's device and pinned memory and streams (its threads have ended)
+static void qserver_free(QServer* sv) {
+    if (sv->s_solve) (void)hipStreamSynchronize(sv->s_solve);
+    if (sv->s_post) (void)hipStreamSynchronize(sv->s_post);
+    for (void* p : sv->dallocs) (void)hipFree(p);
+    sv->dallocs.clear();
+    for (void** p : {(void**)&sv->ring, (void**)&sv->cosmos, (void**)&sv->cfgs, (void**)&sv->ctl, (void**)&sv->done})
+        if (*p) { (void)hipHostFree(*p); *p = nullptr; }
+    for (hipStream_t* s : {&sv->s_solve, &sv->s_copy, &sv->s_post})
+        if (*s) { (void)hipStreamDestroy(*s); *s = nullptr; }
+}
+
+static void qserver_stop(QServer* sv) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stopping = true;
+    }
+    sv->cv.notify_all();
+    // the admitter and table threads finish the queued work first; then the queue closes
+    for (size_t i = 0; i < sv->threads.size(); ++i) sv->threads[i].join();
+    sv->threads.clear();
+    __atomic_store_n(&sv->ctl[1], 1, __ATOMIC_RELEASE);
+    qserver_free(sv);
+    sv->cv.notify_all();
+}
+
+template <class T>
+static hipError_t qs_dalloc(QServer* sv, T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) sv->dallocs.push_back((void*)*p);
+    return e;
+}
+
+static int qserver_create(bolt_ctx* ctx, double hbm_gib, int host_threads, std::shared_ptr<QServer>* out) {
+    auto owner = std::make_shared<QServer>();
+    QServer* sv = owner.get();
+    sv->ctx = ctx;
+    auto fail = [&](int code, const char* msg) {
+        qserver_free(sv);
+        return set_error(code, msg);
+    };
+    const size_t fixed = (sizeof(ModeDesc) + sizeof(int) + sizeof(long long) * 4) * QS_QCAP +
+                         (sizeof(CosmoDev) + sizeof(CfgDev)) * QS_RCAP +
+                         sizeof(double) * ((size_t)LOS_GRID * (SV_MAX_ELL + 1) + 2 * LOS_SLOTS * WAVE) +
+                         sizeof(int) * 2 * (SV_MAX_ELL + 1);
+    const double want = hbm_gib * 1073741824.0;
+    if (want < (double)fixed + 4.0 * 1048576.0) return fail(BOLT_E_NOMEM, "bolt_serve_start_queue: the HBM budget is too small");
+    // a mode's output offset is an int count of doubles: the arena stays below 2^31 doubles
+    const size_t arena = std::min((size_t)(want - (double)fixed), (size_t)15 << 30);
+    const unsigned hf = hipHostMallocMapped | hipHostMallocCoherent;
+    if (hipHostMalloc((void**)&sv->ring, sizeof(ModeDesc) * QS_QCAP, hf) != hipSuccess ||
+        hipHostMalloc((void**)&sv->cosmos, sizeof(CosmoDev) * QS_RCAP, hf) != hipSuccess ||
+        hipHostMalloc((void**)&sv->cfgs, sizeof(CfgDev) * QS_RCAP, hf) != hipSuccess ||
+        hipHostMalloc((void**)&sv->ctl, sizeof(int) * 3, hf) != hipSuccess ||
+        hipHostMalloc((void**)&sv->done, sizeof(int) * QS_QCAP, hf) != hipSuccess)
+        return fail(BOLT_E_NOMEM, "bolt_serve_start_queue: pinned host memory");
+    std::memset(sv->ctl, 0, sizeof(int) * 3);
+    std::memset(sv->done, 0, sizeof(int) * QS_QCAP);
+    if (hipMalloc((void**)&sv->arena_d, arena) != hipSuccess) return fail(BOLT_E_NOMEM, "bolt_serve_start_queue: arena");
+    sv->dallocs.push_back(sv->arena_d);
+    sv->arena.init(arena);
+    if (qs_dalloc(sv, &sv->d_ring, QS_QCAP) || qs_dalloc(sv, &sv->d_cosmos, QS_RCAP) ||
+        qs_dalloc(sv, &sv->d_cfgs, QS_RCAP) || qs_dalloc(sv, &sv->d_counter, 1) || qs_dalloc(sv, &sv->d_leave, 1) ||
+        qs_dalloc(sv, &sv->d_status, QS_QCAP) ||
+        qs_dalloc(sv, &sv->d_ctr, 4 * (size_t)QS_QCAP) || qs_dalloc(sv, &sv->d_xzero, 1) ||
+        qs_dalloc(sv, &sv->lb.lidx, SV_MAX_ELL + 1) || qs_dalloc(sv, &sv->lb.pmap, SV_MAX_ELL + 1) ||
+        qs_dalloc(sv, &sv->lb.etaj, LOS_SLOTS * WAVE) || qs_dalloc(sv, &sv->lb.dxj, LOS_SLOTS * WAVE) ||
+        qs_dalloc(sv, &sv->lb.partial, (size_t)LOS_GRID * (SV_MAX_ELL + 1)))
+        return fail(BOLT_E_NOMEM, "bolt_serve_start_queue: work arrays");
+    sv->lb.cap_lidx = SV_MAX_ELL + 1; sv->lb.cap_pmap = SV_MAX_ELL + 1; sv->lb.cap_nxi = LOS_SLOTS * WAVE;
+    sv->lb.cap_partial = (size_t)LOS_GRID * (SV_MAX_ELL + 1);
+    const double zero = 0.0;
+    const int izero = 0;
+    if (hipMemcpy(sv->d_xzero, &zero, sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(sv->d_counter, &izero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(sv->d_leave, &izero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(BOLT_E_DEVICE, "bolt_serve_start_queue: initialisation");
+    // The solver's stream is masked to n_cu - reserve CUs and the copy and
+    // post-step streams to the other ones: without the masks, the C_ℓ kernels of
+    // a finished request get no CU while the persistent launch is resident, even
+    // on CUs it leaves free, and wait for its workgroups to leave.  The reserve
+    // is whole XCDs (mask bits 32x .. 32x + 31 are one XCD's CUs): one XCD by
+    // default, whose L2 serves the C_ℓ kernels alone.  Splitting an XCD between
+    // the two (16 or 48 CUs) gave multi-second latency tails under load.
+    // Tunables: BOLT_QS_RESERVE_CU (rounded up to whole XCDs), BOLT_QS_POLL_SLEEPS,
+    // BOLT_QS_CU_MASK=0 (plain streams, 16 CUs left free)
+    const int per_xcd = std::max(1, ctx->n_cu / QS_XCD);
+    int reserve = per_xcd;
+    if (const char* e = std::getenv("BOLT_QS_RESERVE_CU"))
+        reserve = std::min(ctx->n_cu / 2, (std::max(1, std::atoi(e)) + per_xcd - 1) / per_xcd * per_xcd);
+    if (const char* e = std::getenv("BOLT_QS_CU_MASK")) sv->cu_mask = std::atoi(e) != 0;
+    if (!sv->cu_mask) reserve = std::min(reserve, 16);
+    if (const char* e = std::getenv("BOLT_QS_POLL_SLEEPS")) sv->poll_sleeps = std::max(1, std::min(4096, std::atoi(e)));
+    const int nsolve = std::max(1, ctx->n_cu - reserve);
+    if (sv->cu_mask) {
+        // mask bit i is the i-th CU of the device
+        const int words = (ctx->n_cu + 31) / 32;
+        std::vector<uint32_t> m_solve(words, 0u), m_post(words, 0u);
+        for (int i = 0; i < ctx->n_cu; ++i) (i < nsolve ? m_solve : m_post)[i / 32] |= 1u << (i % 32);
+        if (hipExtStreamCreateWithCUMask(&sv->s_solve, words, m_solve.data()) != hipSuccess ||
+            hipExtStreamCreateWithCUMask(&sv->s_copy, words, m_post.data()) != hipSuccess ||
+            hipExtStreamCreateWithCUMask(&sv->s_post, words, m_post.data()) != hipSuccess)
+            return fail(BOLT_E_DEVICE, "bolt_serve_start_queue: hipExtStreamCreateWithCUMask");
+    } else {
+        for (hipStream_t* s : {&sv->s_solve, &sv->s_copy, &sv->s_post})
+            if (hipStreamCreateWithFlags(s, hipStreamNonBlocking) != hipSuccess)
+                return fail(BOLT_E_DEVICE, "bolt_serve_start_queue: hipStreamCreate");
+    }
+    sv->lds = (size_t)ctx->lds_max;
+    sv->grid = nsolve;
+    for (int i = QS_RCAP - 1; i >= 0; --i) sv->free_slots.push_back(i);
+    const int nt = std::max(1, host_threads);
+    for (int i = 0; i < nt; ++i) sv->threads.emplace_back(qs_tables_loop, sv);
+    sv->threads.emplace_back(qs_admit_loop, sv);
+    sv->threads.emplace_back(qs_launch_loop, sv);
+    sv->threads.emplace_back(qs_complete_loop, sv);
+    *out = owner;
+    return BOLT_OK;
 }
 
 static void server_stop(Server* sv) {
@@ -5058,7 +5691,25 @@ static void server_stop(Server* sv) {
     sv->cv_job.notify_all();
     for (auto& t : sv->workers) t.join();
     for (auto& s : sv->slots) slot_free(s);
-    delete sv;
+    sv->cv_done.notify_all();
+}
+
+// a request's result from either server; wait: block until it is done, then
+// drop it from the server's table
+template <class S>
+static int result_of(const std::shared_ptr<S>& sv, std::condition_variable& cv, uint64_t id, bool wait,
+                     bolt_result* res) {
+    std::unique_lock<std::mutex> lq(sv->mu);
+    auto it = sv->results.find(id);
+    if (it == sv->results.end()) return set_error(BOLT_E_ARG, "unknown request id");
+    if (wait) {
+        cv.wait(lq, [&] { auto f = sv->results.find(id); return f == sv->results.end() || f->second.state == BOLT_REQ_DONE; });
+        it = sv->results.find(id);
+        if (it == sv->results.end()) return set_error(BOLT_E_ARG, "unknown request id");
+    }
+    *res = it->second;
+    if (wait) sv->results.erase(it);
+    return BOLT_OK;
 }
 
 extern "C" {
@@ -5067,24 +5718,50 @@ int bolt_serve_start(bolt_ctx* ctx, int slots, double hbm_gib) {
     if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
     if (slots < 1 || !(hbm_gib > 0)) return set_error(BOLT_E_ARG, "bolt_serve_start: need slots >= 1, hbm_gib > 0");
     std::lock_guard<std::mutex> lk(ctx->server_mu);
-    if (ctx->server) return set_error(BOLT_E_STATE, "bolt_serve_start: already serving");
+    if (ctx->server || ctx->qserver) return set_error(BOLT_E_STATE, "bolt_serve_start: already serving");
     const int by_hbm = (int)(hbm_gib * 1073741824.0 / (double)slot_bytes());
     const int n = std::min(slots, by_hbm);
     if (n < 1) return set_error(BOLT_E_NOMEM, "bolt_serve_start: the HBM budget holds no slot");
     HIPCHK(hipSetDevice(ctx->device));
-    Server* sv = new Server;
+    auto sv = std::make_shared<Server>();
     sv->ctx = ctx;
     sv->slots.resize(n);
     for (auto& s : sv->slots) {
         int r = slot_create(s);
         if (r) {
             for (auto& q : sv->slots) slot_free(q);
-            delete sv;
             return r;
         }
     }
-    for (int w = 0; w < n; ++w) sv->workers.emplace_back(worker_loop, sv, w);
+    for (int w = 0; w < n; ++w) sv->workers.emplace_back(worker_loop, sv.get(), w);
     ctx->server = sv;
+    return BOLT_OK;
+}
+
+int bolt_serve_start_queue(bolt_ctx* ctx, double hbm_gib, int host_threads) {
+    if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    if (!(hbm_gib > 0) || host_threads < 0) return set_error(BOLT_E_ARG, "bolt_serve_start_queue: bad argument");
+    std::lock_guard<std::mutex> lk(ctx->server_mu);
+    if (ctx->server || ctx->qserver) return set_error(BOLT_E_STATE, "bolt_serve_start_queue: already serving");
+    HIPCHK(hipSetDevice(ctx->device));
+    return qserver_create(ctx, hbm_gib, host_threads ? host_threads : 4, &ctx->qserver);
+}
+
+int bolt_serve_queue_info(bolt_ctx* ctx, bolt_queue_info* info) {
+    if (!ctx || !info) return set_error(BOLT_E_ARG, "bolt_serve_queue_info: NULL argument");
+    std::shared_ptr<QServer> sv;
+    {
+        std::lock_guard<std::mutex> lk(ctx->server_mu);
+        sv = ctx->qserver;
+    }
+    if (!sv) return set_error(BOLT_E_STATE, "bolt_serve_queue_info: no queue server");
+    std::lock_guard<std::mutex> lq(sv->mu);
+    info->launches = sv->launches;
+    info->served = sv->served;
+    info->joined_running = sv->joined;
+    info->arena_bytes = (long long)sv->arena.size;
+    info->arena_peak_bytes = (long long)sv->arena.peak;
+    info->solver_workgroups = sv->grid;
     return BOLT_OK;
 }
 
@@ -5099,17 +5776,37 @@ int bolt_submit(bolt_ctx* ctx, const bolt_request* req, uint64_t* id) {
     int r = validate_request(req);
     if (r) return r;
     std::lock_guard<std::mutex> lk(ctx->server_mu);
+    bolt_result res;
+    std::memset(&res, 0, sizeof(res));
+    res.state = BOLT_REQ_QUEUED;
+    if (ctx->qserver) {
+        QServer* sv = ctx->qserver.get();
+        const CfgDev cd = to_cfgdev(&req->cfg);
+        if (ws_bytes(cd.n, cd.nq) > (size_t)ctx->lds_max)
+            return set_error(BOLT_E_ARG, "bolt_submit: hierarchy too large for one workgroup's LDS");
+        auto q = std::make_shared<QReq>();
+        q->rq = *req;
+        q->t_submit = now_ms();
+        q->bytes = qreq_layout(*q);
+        if (q->bytes > sv->arena.size) return set_error(BOLT_E_NOMEM, "bolt_submit: the request exceeds the HBM arena");
+        {
+            std::lock_guard<std::mutex> lq(sv->mu);
+            q->id = sv->next_id++;
+            sv->results[q->id] = res;
+            sv->pending.push_back(q);
+            *id = q->id;
+        }
+        sv->cv.notify_all();
+        return BOLT_OK;
+    }
     if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_submit: call bolt_serve_start first");
-    Server* sv = ctx->server;
+    Server* sv = ctx->server.get();
     {
         std::lock_guard<std::mutex> lq(sv->mu);
         ServeJob j;
         j.id = sv->next_id++;
         j.req = *req;
         j.t_submit = now_ms();
-        bolt_result res;
-        std::memset(&res, 0, sizeof(res));
-        res.state = BOLT_REQ_QUEUED;
         sv->results[j.id] = res;
         sv->queue.push_back(j);
         *id = j.id;
@@ -5120,37 +5817,48 @@ int bolt_submit(bolt_ctx* ctx, const bolt_request* req, uint64_t* id) {
 
 int bolt_poll(bolt_ctx* ctx, uint64_t id, bolt_result* res) {
     if (!ctx || !res) return set_error(BOLT_E_ARG, "bolt_poll: NULL argument");
-    std::lock_guard<std::mutex> lk(ctx->server_mu);
-    if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_poll: not serving");
-    std::lock_guard<std::mutex> lq(ctx->server->mu);
-    auto it = ctx->server->results.find(id);
-    if (it == ctx->server->results.end()) return set_error(BOLT_E_ARG, "bolt_poll: unknown request id");
-    *res = it->second;
-    return BOLT_OK;
+    std::shared_ptr<Server> a;
+    std::shared_ptr<QServer> b;
+    {
+        std::lock_guard<std::mutex> lk(ctx->server_mu);
+        a = ctx->server;
+        b = ctx->qserver;
+    }
+    if (b) return result_of(b, b->cv, id, false, res);
+    if (a) return result_of(a, a->cv_done, id, false, res);
+    return set_error(BOLT_E_STATE, "bolt_poll: not serving");
 }
 
 int bolt_wait(bolt_ctx* ctx, uint64_t id, bolt_result* res) {
     if (!ctx || !res) return set_error(BOLT_E_ARG, "bolt_wait: NULL argument");
-    Server* sv;
+    std::shared_ptr<Server> a;
+    std::shared_ptr<QServer> b;
     {
         std::lock_guard<std::mutex> lk(ctx->server_mu);
-        if (!ctx->server) return set_error(BOLT_E_STATE, "bolt_wait: not serving");
-        sv = ctx->server;
+        a = ctx->server;
+        b = ctx->qserver;
     }
-    std::unique_lock<std::mutex> lq(sv->mu);
-    auto it = sv->results.find(id);
-    if (it == sv->results.end()) return set_error(BOLT_E_ARG, "bolt_wait: unknown request id");
-    sv->cv_done.wait(lq, [&] { return sv->results[id].state == BOLT_REQ_DONE; });
-    *res = sv->results[id];
-    return BOLT_OK;
+    // the shared pointers keep the server alive while this thread waits, also
+    // across a concurrent bolt_serve_stop (which finishes every request first)
+    if (b) return result_of(b, b->cv, id, true, res);
+    if (a) return result_of(a, a->cv_done, id, true, res);
+    return set_error(BOLT_E_STATE, "bolt_wait: not serving");
 }
 
 int bolt_serve_stop(bolt_ctx* ctx) {
     if (!ctx) return set_error(BOLT_E_STATE, "ctx is NULL");
+    std::shared_ptr<Server> a;
+    std::shared_ptr<QServer> b;
+    {
+        std::lock_guard<std::mutex> lk(ctx->server_mu);
+        a = ctx->server;
+        b = ctx->qserver;
+    }
+    if (a) server_stop(a.get());
+    if (b) qserver_stop(b.get());
     std::lock_guard<std::mutex> lk(ctx->server_mu);
-    if (!ctx->server) return BOLT_OK;
-    server_stop(ctx->server);
-    ctx->server = nullptr;
+    ctx->server.reset();
+    ctx->qserver.reset();
     return BOLT_OK;
 }
 

@@ -413,20 +413,53 @@ typedef struct {
     int status;          /* when DONE: BOLT_OK or the error code */
     double queue_ms;     /* submit -> start on a slot */
     double host_ms;      /* background + RECFAST tables on the host */
-    double device_ms;    /* solver + LOS kernel time */
+    double device_ms;    /* slots: solver + LOS kernel time; queue server: the C_l or P(k) step after the solve */
     double latency_ms;   /* submit -> done */
+    double solve_ms;     /* queue server: publication of the request's modes -> the last of them solved */
+    int joined_running;  /* queue server: 1 if the request's modes were published while a solver launch was
+                            running and that launch solved all of them */
+    char error[160];     /* when DONE with an error: its message */
 } bolt_result;
+
+typedef struct {
+    long long launches;          /* solver launches so far */
+    long long served;            /* requests completed */
+    long long joined_running;    /* of those, admitted into a launch that was already running */
+    long long arena_bytes;       /* HBM arena the requests' memory comes from */
+    long long arena_peak_bytes;  /* its high-water mark */
+    int solver_workgroups;       /* workgroups of each solver launch (CUs minus those left for C_l / P(k)) */
+} bolt_queue_info;
 
 /* Start serving with up to `slots` concurrent requests, bounded by hbm_gib of
  * device memory (each slot preallocates ~0.13 GiB). */
 int bolt_serve_start(bolt_ctx* ctx, int slots, double hbm_gib);
+/* Serving from a device work queue, for callers that submit requests while
+ * others run (no Bolt counterpart: Bolt's callers run source_grid/cltt/plin
+ * per request, spectra.jl:6-23, 84-96, 163-198).  One persistent solver
+ * launch takes every request's k-modes: the host uploads them into a work
+ * queue in device memory and raises a published count in pinned host memory
+ * while the launch runs, and its workgroups claim them as they free up; a
+ * launch is drained 5 ms after the server holds no request, and a new one
+ * starts when work arrives.  The solver's stream is masked off one XCD (32
+ * CUs), on which the C_l / P(k) steps of finished requests run
+ * (BOLT_QS_RESERVE_CU: more, in whole XCDs).  Each request's device memory
+ * (tables, k grids, outputs) is taken from one HBM arena (hbm_gib less ~30 MB
+ * of fixed work space, at most 15 GiB) by its own size and returned when it
+ * is done; a request larger than the arena is refused at submit
+ * (BOLT_E_NOMEM).  host_threads (0: 4) build the requests' host tables.
+ * Requests are the bolt_request of bolt_serve_start; bolt_submit / poll /
+ * wait / serve_stop serve both. */
+int bolt_serve_start_queue(bolt_ctx* ctx, double hbm_gib, int host_threads);
+/* Counters of the running queue server (BOLT_E_STATE if there is none). */
+int bolt_serve_queue_info(bolt_ctx* ctx, bolt_queue_info* info);
 /* Number of slots of the running server (0 if none). */
 int bolt_serve_slots(bolt_ctx* ctx);
 /* Queue a request; *id identifies it.  The request struct is copied. */
 int bolt_submit(bolt_ctx* ctx, const bolt_request* req, uint64_t* id);
 /* Non-blocking state of a request. */
 int bolt_poll(bolt_ctx* ctx, uint64_t id, bolt_result* res);
-/* Block until the request is done. */
+/* Block until the request is done; the result is then dropped from the
+ * server's table (a later poll or wait of the id reports an unknown id). */
 int bolt_wait(bolt_ctx* ctx, uint64_t id, bolt_result* res);
 /* Finish the queued and running requests, then free the slots. */
 int bolt_serve_stop(bolt_ctx* ctx);

@@ -90,3 +90,94 @@ def test_concurrent_contexts_match_one_context():
             assert np.array_equal(res[i], ref)
         for i in (1, 3):
             assert np.array_equal(res[i], ref_pk)
+
+
+def _pinned_ctx():
+    """a context whose solver runs six scan rows per lane, the shape of the
+    queue server's launches, so that standalone results are comparable bitwise"""
+    import os
+    old = os.environ.get("BOLT_SOLVER_SR")
+    os.environ["BOLT_SOLVER_SR"] = "6"
+    try:
+        return boltmi.Context(0)
+    finally:
+        if old is None:
+            del os.environ["BOLT_SOLVER_SR"]
+        else:
+            os.environ["BOLT_SOLVER_SR"] = old
+
+
+def test_queue_server_results_equal_standalone_and_join_a_running_launch():
+    """bolt_serve_start_queue: every request's k-modes go through one persistent
+    solver launch that the host feeds while it runs.  A large C_ℓ request is
+    submitted first; small P(k) and C_ℓ requests submitted while it is being
+    solved join that launch (their modes are published while it runs and it
+    solves them).  Every result is bitwise the same computation run alone."""
+    import time
+    ks_cl = boltmi.quadratic_k(0.1 * 2.33e-4, 1000 * 2.33e-4, 30)
+    ks_pk = np.logspace(-3, np.log10(0.3), 10)
+    big_k = np.logspace(-4, np.log10(0.5), 800)
+    plan = [("pk", "massive"), ("cltt", "nonu"), ("pk", "c1"), ("cltt", "c1")]
+    srv = boltmi.Server(hbm_gib=2.0, queue=True)
+    try:
+        par, bg, ih = cosmo.background("default")
+        big = srv.submit_cltt(par, k=big_k, ell_max=500, nkd=1000, ℓᵧ=50, ℓ_ν=50, ℓ_mν=10, reltol=1e-11)
+        time.sleep(0.15)  # its tables are built and its modes published; the launch is running
+        ids = []
+        for kind, name in plan:
+            p, b, i = cosmo.background(name)
+            if kind == "cltt":
+                ids.append(srv.submit_cltt(p, k=ks_cl, ell_max=300, nkd=600, ℓᵧ=8, ℓ_ν=8, ℓ_mν=10, reltol=1e-8))
+            else:
+                ids.append(srv.submit_pk(p, k=ks_pk))
+        got = [srv.wait(i) for i in ids]
+        got_big = srv.wait(big)
+        qi = srv.queue_info()
+    finally:
+        srv.close()
+    assert qi["served"] == len(plan) + 1
+    assert 0 < qi["arena_peak_bytes"] <= qi["arena_bytes"]
+    assert sum(info["joined_running"] for _, info in got) >= 1, [info for _, info in got]
+    assert qi["launches"] < len(plan) + 1
+    ctx = _pinned_ctx()
+    try:
+        for (kind, name), (vals, info) in zip(plan, got):
+            par, bg, ih = cosmo.background(name)
+            assert info["state"] == 2 and info["status"] == 0 and info["error"] == ""
+            if kind == "cltt":
+                sf = boltmi.source_grid(par, bg, ih, ks_cl, None, 8, 1e-8, ctx=ctx)
+                ref = boltmi.cltt_on_grid(np.arange(2, 301), sf,
+                                          boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 600), par, bg, ctx=ctx)
+            else:
+                ref = boltmi.plin(ks_pk, par, bg, ih, ctx=ctx)
+            assert np.array_equal(vals, ref), (kind, name, np.max(np.abs(vals / ref - 1)))
+        par, bg, ih = cosmo.background("default")
+        sf = boltmi.source_grid(par, bg, ih, big_k, None, 50, 1e-11, ℓ_ν=50, ℓ_mν=10, ctx=ctx)
+        ref = boltmi.cltt_on_grid(np.arange(2, 501), sf, boltmi.quadratic_k(0.01 * bg.H0, 1000 * bg.H0, 1000), par,
+                                  bg, ctx=ctx)
+        assert np.array_equal(got_big[0], ref)
+    finally:
+        ctx.close()
+
+
+def test_queue_server_admits_by_footprint():
+    """Requests whose memory together exceeds the arena wait for it and all
+    complete; one that exceeds the arena alone is refused at submit."""
+    probe = boltmi.Server(hbm_gib=1.0, queue=True)
+    fixed = 2**30 - probe.queue_info()["arena_bytes"]   # the server's own work space
+    probe.close()
+    srv = boltmi.Server(hbm_gib=(fixed + 11.6e6) / 2**30, queue=True)   # an arena of ~11.6 MB
+    try:
+        par, bg, ih = cosmo.background("c1")
+        ks = np.logspace(-3, np.log10(0.2), 300)   # 4.8 MB of source grid per request
+        ids = [srv.submit_cltt(par, k=ks, ell_max=200, nkd=400, ℓᵧ=8, ℓ_ν=8, ℓ_mν=10, reltol=1e-6)
+               for _ in range(6)]
+        outs = [srv.wait(i)[0] for i in ids]
+        qi = srv.queue_info()
+        with pytest.raises(boltmi._abi.BoltError):
+            srv.submit_cltt(par, k=np.logspace(-4, np.log10(0.5), 800), ell_max=2500, nkd=5000)
+    finally:
+        srv.close()
+    assert qi["arena_peak_bytes"] <= qi["arena_bytes"] < 6 * 300 * 2001 * 8
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
