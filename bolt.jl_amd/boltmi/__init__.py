@@ -7,5 +7,7 @@ from .api import (MASS_NATURAL, Background, autotune, backgrounds, BasicNewtonia
                   source_grid_P, source_grid_batch, source_grids_TP,
                   state_length)
 from .dual import PARAM_FIELDS, Dual, derivative, tables_jacobian
+from .bessel import (MomentTable, filon_sum, integrate_sph_bessel_filon, sph_bessel_interpolator,
+                     sph_j_moment_asymp, sph_j_moment_maclaurin_1F2, sph_j_moment_series)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

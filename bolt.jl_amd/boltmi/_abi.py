@@ -142,6 +142,13 @@ SIGNATURES = {
     "bolt_poll": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),
     "bolt_wait": (_I, [_VP, ctypes.c_uint64, ctypes.POINTER(ResultC)]),
     "bolt_serve_stop": (_I, [_VP]),
+    "bolt_moment_table_create": (_I, [_VP, _I, _I, _D, _D, _I, ctypes.POINTER(_VP)]),
+    "bolt_moment_table_destroy": (_I, [_VP]),
+    "bolt_moment_table_eval": (_I, [_VP, _PD, _I, _PD]),
+    "bolt_moment_table_coefficients": (_I, [_VP, _PD]),
+    "bolt_filon": (_I, [_VP, _PD, _PD, _PD, _PD, _PD, _PD, _I, _PD]),
+    "bolt_filon_sum": (_I, [_VP, _PD, _PD, _PD, _PD, _I, _PD, _I, _PD]),
+    "bolt_sph_moments": (_I, [_VP, _I, _PD, _PI, _PD, _I, _PD]),
 }
 
 _lib = None

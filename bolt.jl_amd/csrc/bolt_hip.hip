@@ -5863,3 +5863,534 @@ int bolt_serve_stop(bolt_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Moments of spherical Bessel functions, their tables and the Filon rule
+// (src/bessel/moments.jl, interpolator.jl, integrator.jl)
+//
+// I_m(x) = ∫_0^x t^m j_ν(t) dt.  Three device evaluations:
+// - the Lommel-function asymptotic (moments.jl:5-21, 72-86), FP64, with its
+//   series truncated as the reference truncates it (at most 21 terms, stop
+//   once a term is below 1e-16 of the sum);
+// - the Maclaurin series of ₁F₂ in FP64 (moments.jl:97-102, weniger.jl:237-249),
+//   used below a table's range;
+// - for the table nodes below x = 40, where the reference sums ₁F₂ by Weniger's
+//   transformation in Double64 (below 50, moments.jl:89-94):
+//   · below x = 20 the same series in double-double arithmetic,
+//     I_m = x^{ν+m+1}/(2ν+1)!! Σ_k (−x²/2)^k / (k! (2ν+3)…(2ν+2k+1) (2k+ν+m+1));
+//     its terms reach about e^x/√(2πx)·x^{ν+m+1}/(2ν+1)!! against the moment,
+//     ~1e12 of it at x = 20 but ~1e20 at 40 for m = 3, too much for 2⁻¹⁰⁶;
+//   · from 20 to 40 exact identities: ∫_0^x t^m j_ν = x^m j_{ν+1}(x) −
+//     (m−ν−2) ∫_0^x t^{m−1} j_{ν+1} (from d[t^{ν+2} j_{ν+1}] = t^{ν+2} j_ν dt)
+//     down to ∫_0^x j_L, and (L+1)∫j_{L+1} = L∫j_{L−1} − (2L+1) j_L(x) up from
+//     ∫j_0 = Si(x), ∫j_1 = 1 − j_0(x); Si by its power series in double-double,
+//     whose terms reach only ~e^x/x² against Si ≈ π/2.
+//   From x = 40 on the asymptotic holds to double precision for ν ≤ 3 (the
+//   tables' range of ν, like the reference's).
+// A table (interpolator.jl:85-111) holds the moments m = 0..order−1 on
+// LinRange(x_min, x_max, N) as cubic B-spline coefficients with Line(OnGrid())
+// ends; below x_min it evaluates the FP64 series, above x_max the asymptotic.
+// ============================================================================
+namespace {
+constexpr double MOM_CUT = 40.0;
+constexpr double MOM_SQRT_HALF_PI = 1.2533141373155002512078826424055;
+constexpr int MOM_CHUNK = 128;  // nodes per thread of the prefilter passes
+constexpr int MOM_WARM = 64;    // warm-up nodes: the recurrences contract by ~0.268 a node
+}  // namespace
+
+struct DD {
+    double hi, lo;
+};
+// error-free transformations: no contraction of their additions
+__device__ inline DD dd_fast2(double a, double b) {
+#pragma clang fp contract(off)
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ inline DD dd_two(double a, double b) {
+#pragma clang fp contract(off)
+    const double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline DD dd_add(DD a, DD b) {
+#pragma clang fp contract(off)
+    DD s = dd_two(a.hi, b.hi);
+    const DD t = dd_two(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_fast2(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_fast2(s.hi, s.lo);
+}
+__device__ inline DD dd_mul(DD a, DD b) {
+#pragma clang fp contract(off)
+    const double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e = fma(a.hi, b.lo, e);
+    e = fma(a.lo, b.hi, e);
+    return dd_fast2(p, e);
+}
+__device__ inline DD dd_div(DD a, double b) {
+#pragma clang fp contract(off)
+    const double q1 = a.hi / b;
+    const double p = q1 * b;
+    const double pe = fma(q1, b, -p);
+    const double r = ((a.hi - p) - pe) + a.lo;
+    return dd_fast2(q1, r / b);
+}
+
+// the series in double-double (see above); x ≥ 0
+__device__ double mom_series_dd(double x, int nu, double m) {
+#pragma clang fp contract(off)
+    if (x == 0.0) return 0.0;
+    const double p = x * x, pe = fma(x, x, -p);
+    const DD z = dd_fast2(-0.5 * p, -0.5 * pe);  // −x²/2
+    const double a = nu + m + 1.0;
+    DD t = {1.0, 0.0}, s = {0.0, 0.0};
+    for (int k = 0; k < 1000; ++k) {
+        const DD term = dd_div(t, a + 2.0 * k);
+        s = dd_add(s, term);
+        if (k > 2 && fabs(term.hi) < 1e-33 * fabs(s.hi)) break;
+        t = dd_div(dd_mul(t, z), (double)(k + 1) * (double)(2 * nu + 2 * k + 3));
+    }
+    double df = 1.0;
+    for (int i = 3; i <= 2 * nu + 1; i += 2) df *= i;
+    return pow(x, a) / df * (s.hi + s.lo);
+}
+
+// Si(x) = Σ_k (−1)^k x^{2k+1} / ((2k+1)(2k+1)!) in double-double (x ≤ 40)
+__device__ double mom_si_dd(double x) {
+#pragma clang fp contract(off)
+    const double p = x * x, pe = fma(x, x, -p);
+    const DD z = dd_fast2(-p, -pe);  // −x²
+    DD t = {x, 0.0}, s = {0.0, 0.0};
+    for (int k = 0; k < 1000; ++k) {
+        const DD term = dd_div(t, 2.0 * k + 1.0);
+        s = dd_add(s, term);
+        if (k > 2 && fabs(term.hi) < 1e-33 * fabs(s.hi)) break;
+        t = dd_div(dd_mul(t, z), (2.0 * k + 2.0) * (2.0 * k + 3.0));
+    }
+    return s.hi + s.lo;
+}
+
+// I_m(x) for integer 0 ≤ m ≤ 3, 1 ≤ ν ≤ 3 and 20 ≤ x < 40 by the identities above
+__device__ double mom_closed(double x, int nu, int m) {
+    double j[8];  // j_0 .. j_{ν+m+1} by upward recurrence (x > ν + m + 1)
+    j[0] = sin(x) / x;
+    j[1] = sin(x) / (x * x) - cos(x) / x;
+    for (int l = 1; l <= nu + m; ++l) j[l + 1] = (2 * l + 1) / x * j[l] - j[l - 1];
+    double a0 = mom_si_dd(x), a1 = 1.0 - j[0];  // ∫j_{L−1}, ∫j_L
+    for (int l = 1; l < nu + m; ++l) {
+        const double a2 = (l * a0 - (2 * l + 1) * j[l]) / (l + 1);
+        a0 = a1;
+        a1 = a2;
+    }
+    double I = nu + m == 0 ? a0 : a1;  // ∫_0^x j_{ν+m}
+    double xk = 1.0;
+    for (int k = 1; k <= m; ++k) {
+        const int mu = nu + m - k;
+        xk *= x;
+        I = xk * j[mu + 1] - (k - mu - 2) * I;
+    }
+    return I;
+}
+
+__device__ double mom_asymp(double x, int nu, double m, double pref);
+
+// a table node's value: the series below 20, the identities below 40, the asymptotic from 40 on
+__device__ inline double mom_node_value(double x, int nu, int m, double pref) {
+    return x < 20.0 ? mom_series_dd(x, nu, m) : (x < MOM_CUT ? mom_closed(x, nu, m) : mom_asymp(x, nu, m, pref));
+}
+
+// s⁽²⁾(t, α − ½, ν) (moments.jl:5-21)
+__device__ inline double mom_s2(double t, double amh, double nu) {
+    double s = 1.0, sk = 1.0;
+    const double ti = 1.0 / t, ti2 = ti * ti, nu2 = nu * nu, am1 = amh - 0.5;
+    for (int j = 0; j <= 20; ++j) {
+        const double d = am1 - 2.0 * j;
+        sk *= (nu2 - d * d) * ti2;
+        s += sk;
+        if (fabs(sk) < 1e-16 * fabs(s)) break;
+    }
+    return s * pow(t, amh) * sqrt(ti);
+}
+
+// sph_j_moment_asymp_prefactor (moments.jl:72-73): √(π/2) 2^{m−½} Γ((ν+m+1)/2) / Γ((ν−m+2)/2)
+__host__ __device__ inline double mom_asymp_prefactor(int nu, double m) {
+    const double nuh = nu + 0.5, al = m - 0.5;
+    return exp(log(2.0) * al + lgamma((nuh + al + 1.0) / 2.0) - lgamma((nuh - al + 1.0) / 2.0)) * MOM_SQRT_HALF_PI;
+}
+
+// sph_j_moment_asymp (moments.jl:76-86), ν ≥ 1, x > ν: J_{ν±½}(x) = √(2x/π) j_ν, j_{ν−1}
+// by upward recurrence from j_0, j_1
+__device__ double mom_asymp(double x, int nu, double m, double pref) {
+    double jm = sin(x) / x, jc = sin(x) / (x * x) - cos(x) / x;  // j_0, j_1
+    for (int l = 1; l < nu; ++l) {
+        const double jn = (2 * l + 1) / x * jc - jm;
+        jm = jc;
+        jc = jn;
+    }
+    const double c = sqrt(x) / MOM_SQRT_HALF_PI;
+    return pref + x * MOM_SQRT_HALF_PI *
+                      ((m + nu - 1.0) * (c * jc) * mom_s2(x, m - 2.0, nu - 0.5) - (c * jm) * mom_s2(x, m - 1.0, nu + 0.5));
+}
+
+// maclaurin_₁F₂ (weniger.jl:237-249) and sph_j_moment_maclaurin_₁F₂ (moments.jl:97-102), FP64
+__device__ double mom_maclaurin64(double x, int nu, double m) {
+    const double nup = nu + 1.5, a = (1.0 + m + nu) / 2.0, b1 = (3.0 + m + nu) / 2.0, b2 = nup, z = -x * x / 4.0;
+    double S0 = 1.0, S1 = 1.0 + a * z / (b1 * b2);
+    for (int j = 1; j < 100000; ++j) {
+        const bool more = isfinite(S0) && isfinite(S1) &&
+                          fabs(S0 - S1) > fmax(fabs(S0), fabs(S1)) * (10.0 * 2.220446049250313e-16);
+        if (!(more || j <= 1)) break;
+        double r = 1.0 / (j + 1.0);
+        r *= a + j;
+        r /= (b1 + j) * (b2 + j);
+        const double Sn = S1 + (S1 - S0) * r * z;
+        S0 = S1;
+        S1 = Sn;
+    }
+    return (MOM_SQRT_HALF_PI / (m + nu + 1.0)) * exp((m + nu + 1.0) * log(x) - (nup - 1.0) * log(2.0) - lgamma(nup)) *
+           S1;
+}
+
+struct MomTab {
+    int nu, order, n;
+    double xmin, xmax, dx;
+    double pref[4];
+    double* coef;  // [order][n + 2]
+};
+
+// node i of LinRange(x_min, x_max, n)
+__host__ __device__ inline double mom_node(const MomTab& T, int i) {
+    const double t = (double)i / (double)(T.n - 1);
+    return (1.0 - t) * T.xmin + t * T.xmax;
+}
+
+// node values into rows 1..n of each coefficient column
+__global__ void __launch_bounds__(256) k_moment_nodes(MomTab T) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n; i += gridDim.x * blockDim.x) {
+        const double x = mom_node(T, i);
+        for (int m = 0; m < T.order; ++m) T.coef[(size_t)m * (T.n + 2) + 1 + i] = mom_node_value(x, T.nu, m, T.pref[m]);
+    }
+}
+
+// Line(OnGrid()) prefilter, the Thomas algorithm of the host tables' prefilter
+// (d_q = (r_q − d_{q−1}) / (4 − cp_{q−1}), cp_q = 1/(4 − cp_{q−1}), cp_0 = 1/4,
+// then c = d − cp·c_next).  The unknowns are c[2..n−1] (q = 0..n−3); c[1] = y_0
+// and c[n] = y_{n−1}.  cp_q is bitwise constant from q ≈ 20 on, and is held at
+// cp_MOM_WARM beyond.  Each thread takes MOM_CHUNK unknowns and starts its
+// recurrence MOM_WARM unknowns early from zero (or at the true start): both
+// recurrences contract by cp ≈ 0.268 per unknown, so the dropped history is
+// ~1e-37 of the values.
+__device__ inline double mom_cp(int q) {
+    double cp = 0.25;
+    const int lim = q < MOM_WARM ? q : MOM_WARM;
+    for (int i = 1; i <= lim; ++i) cp = 1.0 / (4.0 - cp);
+    return cp;
+}
+// forward: d_q = (6 y_{q+1} − [q = 0] y_0 − [q = mm−1] y_{n−1} − d_{q−1}) / (4 − cp_{q−1}), into work
+__global__ void __launch_bounds__(256) k_moment_prefilter_fwd(MomTab T, double* work) {
+    const int mm = T.n - 2;
+    const int nchunk = (mm + MOM_CHUNK - 1) / MOM_CHUNK;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nchunk * T.order) return;
+    const int m = g / nchunk, ch = g % nchunk;
+    const double* y = T.coef + (size_t)m * (T.n + 2) + 1;  // node values
+    double* d = work + (size_t)m * mm;
+    const int q0 = ch * MOM_CHUNK, q1 = q0 + MOM_CHUNK < mm ? q0 + MOM_CHUNK : mm;
+    const int qs = q0 - MOM_WARM > 0 ? q0 - MOM_WARM : 0;
+    double dp = 0.0;
+    double cpm = qs > 0 ? mom_cp(qs - 1) : 0.25;  // cp_{q−1}
+    for (int q = qs; q < q1; ++q) {
+        double r = 6.0 * y[q + 1];
+        if (q == 0) r -= y[0];
+        if (q == mm - 1) r -= y[T.n - 1];
+        double v;
+        if (q == 0) {
+            v = r / 4.0;
+        } else {
+            const double den = 4.0 - cpm;
+            v = (r - dp) / den;
+            if (q <= MOM_WARM) cpm = 1.0 / den;
+        }
+        if (q >= q0) d[q] = v;
+        dp = v;
+    }
+}
+// backward: c_{q+2} = d_q − cp_q c_{q+3}, then the two padding coefficients
+__global__ void __launch_bounds__(256) k_moment_prefilter_bwd(MomTab T, const double* work) {
+    const int mm = T.n - 2;
+    const int nchunk = (mm + MOM_CHUNK - 1) / MOM_CHUNK;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nchunk * T.order) return;
+    const int m = g / nchunk, ch = g % nchunk;
+    double* c = T.coef + (size_t)m * (T.n + 2);
+    const double* d = work + (size_t)m * mm;
+    const int q0 = ch * MOM_CHUNK, q1 = q0 + MOM_CHUNK < mm ? q0 + MOM_CHUNK : mm;
+    const int qe = q1 + MOM_WARM < mm ? q1 + MOM_WARM : mm;
+    double xn = 0.0;  // c_{q+3}
+    const double cpw = mom_cp(MOM_WARM);
+    for (int q = qe - 1; q >= q0; --q) {
+        const double v = q == mm - 1 ? d[q] : d[q] - (q < MOM_WARM ? mom_cp(q) : cpw) * xn;
+        if (q < q1) c[q + 2] = v;
+        xn = v;
+    }
+}
+__global__ void k_moment_prefilter_ends(MomTab T) {
+    const int m = threadIdx.x;
+    if (m >= T.order) return;
+    double* c = T.coef + (size_t)m * (T.n + 2);
+    // c[1] = y_0 and c[n] = y_{n−1} are already in place
+    c[0] = 2.0 * c[1] - c[2];
+    c[T.n + 1] = 2.0 * c[T.n] - c[T.n - 1];
+}
+
+// MomentTable(x) (interpolator.jl:60-68)
+__device__ inline void mom_eval(const MomTab& T, double x, double* out) {
+    if (T.xmin <= x && x <= T.xmax) {
+        const double tt = (x - T.xmin) / T.dx;
+        int i = (int)floor(tt);
+        i = i < 0 ? 0 : (i > T.n - 2 ? T.n - 2 : i);
+        const double s = tt - i, r = 1.0 - s;
+        const double w0 = r * r * r / 6.0, w1 = 2.0 / 3.0 - s * s + 0.5 * s * s * s,
+                     w2 = 2.0 / 3.0 - r * r + 0.5 * r * r * r, w3 = s * s * s / 6.0;
+        for (int m = 0; m < T.order; ++m) {
+            const double* c = T.coef + (size_t)m * (T.n + 2) + i;
+            out[m] = w0 * c[0] + w1 * c[1] + w2 * c[2] + w3 * c[3];
+        }
+    } else if (x > T.xmax) {
+        for (int m = 0; m < T.order; ++m) out[m] = mom_asymp(x, T.nu, m, T.pref[m]);
+    } else {
+        for (int m = 0; m < T.order; ++m) out[m] = mom_maclaurin64(x, T.nu, m);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_moment_eval(MomTab T, const double* x, int nx, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx) return;
+    double v[4];
+    mom_eval(T, x[i], v);
+    for (int m = 0; m < T.order; ++m) out[(size_t)i * T.order + m] = v[m];
+}
+
+// integrate_sph_bessel_filon (integrator.jl:5-18): the quadratic through f, f′,
+// f″ at a against j_ν(kt) over (a, b); Ia = the moments at ka
+__device__ inline double mom_filon(double f, double fp, double fpp, double k, double a, const double* Ia,
+                                   const double* Ib) {
+    const double c2 = fpp / 2.0, afpp = a * fpp, c1 = fp - afpp, c0 = f - a * (fp - afpp / 2.0);
+    const double ki = 1.0 / k, ki2 = ki * ki, ki3 = ki2 * ki;
+    return (c0 * ki) * (Ib[0] - Ia[0]) + (c1 * ki2) * (Ib[1] - Ia[1]) + (c2 * ki3) * (Ib[2] - Ia[2]);
+}
+
+__global__ void __launch_bounds__(256) k_filon(MomTab T, const double* f, const double* fp, const double* fpp,
+                                              const double* k, const double* a, const double* b, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double Ia[4], Ib[4];
+    mom_eval(T, k[i] * b[i], Ib);
+    mom_eval(T, k[i] * a[i], Ia);
+    out[i] = mom_filon(f[i], fp[i], fpp[i], k[i], a[i], Ia, Ib);
+}
+
+// Σ_i Filon over (t_i, t_{i+1}) for each k (the loop form, integrator.jl:20-38):
+// one wavefront per k, lane l takes intervals l, l + 64, …, each reusing
+// nothing across lanes (I(k t_i) is evaluated at both ends); the lanes' sums
+// are added in a fixed order
+__global__ void __launch_bounds__(WAVE) k_filon_sum(MomTab T, const double* t, const double* f, const double* fp,
+                                                    const double* fpp, int nt, const double* ks, double* out) {
+    const double k = ks[blockIdx.x];
+    double s = 0.0;
+    for (int i = threadIdx.x; i + 1 < nt; i += WAVE) {
+        double Ia[4], Ib[4];
+        mom_eval(T, k * t[i + 1], Ib);
+        mom_eval(T, k * t[i], Ia);
+        s += mom_filon(f[i], fp[i], fpp[i], k, t[i], Ia, Ib);
+    }
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// the moment functions themselves, per point: method 0 asymptotic, 1 FP64
+// series, 2 a table node's method (x < 40: double-double series, identities)
+__global__ void __launch_bounds__(256) k_sph_moments(int method, const double* x, const int* nu, const double* m, int n,
+                                                    double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i], mi = m[i];
+    const int v = nu[i];
+    out[i] = method == 0 ? mom_asymp(xi, v, mi, mom_asymp_prefactor(v, mi))
+                         : (method == 1 ? mom_maclaurin64(xi, v, mi)
+                                        : (xi < 20.0 ? mom_series_dd(xi, v, mi) : mom_closed(xi, v, (int)mi)));
+}
+
+struct bolt_moment_table {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    MomTab T{};
+};
+
+namespace {
+// a host array of count doubles (or ints) on the device for one call
+template <class V>
+struct Scratch {
+    V* p = nullptr;
+    ~Scratch() { if (p) (void)hipFree(p); }
+    hipError_t up(const V* h, size_t count, hipStream_t st) {
+        hipError_t e = hipMalloc((void**)&p, sizeof(V) * (count ? count : 1));
+        if (e != hipSuccess) return e;
+        return count ? hipMemcpyAsync(p, h, sizeof(V) * count, hipMemcpyHostToDevice, st) : hipSuccess;
+    }
+    hipError_t alloc(size_t count) { return hipMalloc((void**)&p, sizeof(V) * (count ? count : 1)); }
+};
+}  // namespace
+
+extern "C" {
+
+int bolt_moment_table_create(bolt_ctx* ctx, int nu, int order, double x_min, double x_max, int n,
+                             bolt_moment_table** out) {
+    if (!ctx || !out) return set_error(BOLT_E_ARG, "bolt_moment_table_create: NULL argument");
+    if (nu < 1 || nu > 3) return set_error(BOLT_E_ARG, "bolt_moment_table_create: tables hold ν = 1, 2 or 3");
+    if (order < 3 || order > 4) return set_error(BOLT_E_ARG, "bolt_moment_table_create: order is 3 or 4");
+    if (n < 4 || n > (1 << 28)) return set_error(BOLT_E_ARG, "bolt_moment_table_create: need 4 <= n <= 2^28 nodes");
+    if (!(x_min >= 0.0) || !(x_max > x_min) || !std::isfinite(x_max))
+        return set_error(BOLT_E_ARG, "bolt_moment_table_create: need 0 <= x_min < x_max");
+    HIPCHK(hipSetDevice(ctx->device));
+    auto tab = std::make_unique<bolt_moment_table>();
+    tab->device = ctx->device;
+    MomTab& T = tab->T;
+    T.nu = nu; T.order = order; T.n = n; T.xmin = x_min; T.xmax = x_max;
+    T.dx = (x_max - x_min) / (n - 1);
+    for (int m = 0; m < 4; ++m) T.pref[m] = mom_asymp_prefactor(nu, m);
+    HIPCHK(hipStreamCreateWithFlags(&tab->stream, hipStreamNonBlocking));
+    if (hipMalloc((void**)&T.coef, sizeof(double) * (size_t)order * (n + 2)) != hipSuccess) {
+        (void)hipStreamDestroy(tab->stream);
+        return set_error(BOLT_E_NOMEM, "bolt_moment_table_create: coefficients");
+    }
+    Scratch<double> work;
+    if (work.alloc((size_t)order * (n - 2)) != hipSuccess) {
+        (void)hipFree(T.coef);
+        (void)hipStreamDestroy(tab->stream);
+        return set_error(BOLT_E_NOMEM, "bolt_moment_table_create: work space");
+    }
+    hipStream_t st = tab->stream;
+    const int nb = std::min((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_moment_nodes, dim3(nb), dim3(256), 0, st, T);
+    const int nth = ((n - 2 + MOM_CHUNK - 1) / MOM_CHUNK) * order;
+    hipLaunchKernelGGL(k_moment_prefilter_fwd, dim3((nth + 255) / 256), dim3(256), 0, st, T, work.p);
+    hipLaunchKernelGGL(k_moment_prefilter_bwd, dim3((nth + 255) / 256), dim3(256), 0, st, T, (const double*)work.p);
+    hipLaunchKernelGGL(k_moment_prefilter_ends, dim3(1), dim3(WAVE), 0, st, T);
+    const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(st);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)hipFree(T.coef);
+        (void)hipStreamDestroy(tab->stream);
+        return set_error(BOLT_E_DEVICE, "bolt_moment_table_create: kernel failed");
+    }
+    *out = tab.release();
+    return BOLT_OK;
+}
+
+int bolt_moment_table_destroy(bolt_moment_table* tab) {
+    if (!tab) return BOLT_OK;
+    (void)hipSetDevice(tab->device);
+    if (tab->stream) (void)hipStreamSynchronize(tab->stream);
+    if (tab->T.coef) (void)hipFree(tab->T.coef);
+    if (tab->stream) (void)hipStreamDestroy(tab->stream);
+    delete tab;
+    return BOLT_OK;
+}
+
+int bolt_moment_table_coefficients(bolt_moment_table* tab, double* out) {
+    if (!tab || !out) return set_error(BOLT_E_ARG, "bolt_moment_table_coefficients: NULL argument");
+    HIPCHK(hipSetDevice(tab->device));
+    HIPCHK(hipMemcpyAsync(out, tab->T.coef, sizeof(double) * (size_t)tab->T.order * (tab->T.n + 2),
+                          hipMemcpyDeviceToHost, tab->stream));
+    HIPCHK(hipStreamSynchronize(tab->stream));
+    return BOLT_OK;
+}
+
+int bolt_moment_table_eval(bolt_moment_table* tab, const double* x, int nx, double* out) {
+    if (!tab || (nx > 0 && (!x || !out)) || nx < 0) return set_error(BOLT_E_ARG, "bolt_moment_table_eval: bad argument");
+    if (nx == 0) return BOLT_OK;
+    HIPCHK(hipSetDevice(tab->device));
+    hipStream_t st = tab->stream;
+    Scratch<double> dx, dout;
+    HIPCHK(dx.up(x, nx, st));
+    HIPCHK(dout.alloc((size_t)nx * tab->T.order));
+    hipLaunchKernelGGL(k_moment_eval, dim3((nx + 255) / 256), dim3(256), 0, st, tab->T, (const double*)dx.p, nx, dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * (size_t)nx * tab->T.order, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return BOLT_OK;
+}
+
+int bolt_filon(bolt_moment_table* tab, const double* f, const double* fp, const double* fpp, const double* k,
+               const double* a, const double* b, int n, double* out) {
+    if (!tab || n < 0 || (n > 0 && (!f || !fp || !fpp || !k || !a || !b || !out)))
+        return set_error(BOLT_E_ARG, "bolt_filon: bad argument");
+    if (n == 0) return BOLT_OK;
+    for (int i = 0; i < n; ++i)
+        if (!(k[i] > 0.0)) return set_error(BOLT_E_ARG, "bolt_filon: k must be positive");
+    HIPCHK(hipSetDevice(tab->device));
+    hipStream_t st = tab->stream;
+    Scratch<double> df, dfp, dfpp, dk, da, db, dout;
+    HIPCHK(df.up(f, n, st)); HIPCHK(dfp.up(fp, n, st)); HIPCHK(dfpp.up(fpp, n, st));
+    HIPCHK(dk.up(k, n, st)); HIPCHK(da.up(a, n, st)); HIPCHK(db.up(b, n, st));
+    HIPCHK(dout.alloc(n));
+    hipLaunchKernelGGL(k_filon, dim3((n + 255) / 256), dim3(256), 0, st, tab->T, (const double*)df.p,
+                       (const double*)dfp.p, (const double*)dfpp.p, (const double*)dk.p, (const double*)da.p,
+                       (const double*)db.p, n, dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return BOLT_OK;
+}
+
+int bolt_filon_sum(bolt_moment_table* tab, const double* t, const double* f, const double* fp, const double* fpp,
+                   int nt, const double* k, int nk, double* out) {
+    if (!tab || nt < 2 || nk < 0 || !t || !f || !fp || !fpp || (nk > 0 && (!k || !out)))
+        return set_error(BOLT_E_ARG, "bolt_filon_sum: bad argument");
+    if (nk == 0) return BOLT_OK;
+    for (int i = 0; i < nk; ++i)
+        if (!(k[i] > 0.0)) return set_error(BOLT_E_ARG, "bolt_filon_sum: k must be positive");
+    HIPCHK(hipSetDevice(tab->device));
+    hipStream_t st = tab->stream;
+    Scratch<double> dt, df, dfp, dfpp, dk, dout;
+    HIPCHK(dt.up(t, nt, st)); HIPCHK(df.up(f, nt, st)); HIPCHK(dfp.up(fp, nt, st)); HIPCHK(dfpp.up(fpp, nt, st));
+    HIPCHK(dk.up(k, nk, st));
+    HIPCHK(dout.alloc(nk));
+    hipLaunchKernelGGL(k_filon_sum, dim3(nk), dim3(WAVE), 0, st, tab->T, (const double*)dt.p, (const double*)df.p,
+                       (const double*)dfp.p, (const double*)dfpp.p, nt, (const double*)dk.p, dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * nk, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return BOLT_OK;
+}
+
+int bolt_sph_moments(bolt_ctx* ctx, int method, const double* x, const int* nu, const double* m, int n, double* out) {
+    if (!ctx || n < 0 || (n > 0 && (!x || !nu || !m || !out)) || method < 0 || method > 2)
+        return set_error(BOLT_E_ARG, "bolt_sph_moments: bad argument");
+    for (int i = 0; i < n; ++i) {
+        if (nu[i] < 1 || nu[i] > 64) return set_error(BOLT_E_ARG, "bolt_sph_moments: need 1 <= ν <= 64");
+        if (method == 2 && !(x[i] < 20.0) &&
+            (x[i] >= MOM_CUT || nu[i] > 3 || m[i] != std::floor(m[i]) || m[i] < 0.0 || m[i] > 3.0))
+            return set_error(BOLT_E_ARG,
+                             "bolt_sph_moments: method 2 from x = 20 to 40 takes integer 0 <= m <= 3, ν <= 3");
+        if (!(x[i] >= 0.0) || !(m[i] > -nu[i] - 1.0)) return set_error(BOLT_E_ARG, "bolt_sph_moments: need x >= 0, m > -ν-1");
+        if (method == 0 && !(x[i] > nu[i])) return set_error(BOLT_E_ARG, "bolt_sph_moments: the asymptotic needs x > ν");
+    }
+    if (n == 0) return BOLT_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    Scratch<double> dx, dm, dout;
+    Scratch<int> dnu;
+    HIPCHK(dx.up(x, n, st)); HIPCHK(dm.up(m, n, st)); HIPCHK(dnu.up(nu, n, st));
+    HIPCHK(dout.alloc(n));
+    hipLaunchKernelGGL(k_sph_moments, dim3((n + 255) / 256), dim3(256), 0, st, method, (const double*)dx.p,
+                       (const int*)dnu.p, (const double*)dm.p, n, dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return BOLT_OK;
+}
+
+}  // extern "C"

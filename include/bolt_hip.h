@@ -464,6 +464,41 @@ int bolt_wait(bolt_ctx* ctx, uint64_t id, bolt_result* res);
 /* Finish the queued and running requests, then free the slots. */
 int bolt_serve_stop(bolt_ctx* ctx);
 
+/* ---- Moments of spherical Bessel functions and the Filon rule
+ * (src/bessel/moments.jl, interpolator.jl, integrator.jl).
+ * I_m(x) = ∫_0^x t^m j_ν(t) dt. */
+typedef struct bolt_moment_table bolt_moment_table;
+/* sph_bessel_interpolator(ν, order, x_min, x_max, N) (interpolator.jl:85-111):
+ * the moments m = 0..order-1 (order 3 or 4) on LinRange(x_min, x_max, N) as
+ * cubic B-splines with Line(OnGrid()) ends, built on the device: nodes below
+ * x = 20 by the ₁F₂ series in double-double, from 20 to 40 by exact
+ * recurrences down to Si(x) (the reference: Weniger's transformation of ₁F₂ in
+ * Double64 below 50), from 40 on by the Lommel asymptotic.
+ * ν = 1, 2 or 3 (the reference builds tables for 2 and 3); 4 <= N <= 2^28. */
+int bolt_moment_table_create(bolt_ctx* ctx, int nu, int order, double x_min, double x_max, int n,
+                             bolt_moment_table** out);
+int bolt_moment_table_destroy(bolt_moment_table* tab);
+/* MomentTable(x) (interpolator.jl:60-68): spline in [x_min, x_max], the FP64
+ * Maclaurin series below, the asymptotic above; out[i*order + m] (host arrays) */
+int bolt_moment_table_eval(bolt_moment_table* tab, const double* x, int nx, double* out);
+/* the spline coefficients, [order][N + 2] (host array) */
+int bolt_moment_table_coefficients(bolt_moment_table* tab, double* out);
+/* integrate_sph_bessel_filon(f, f′, f″, k, a, b, itp) (integrator.jl:5-18) for
+ * n intervals: ∫_a^b q(t) j_ν(kt) dt of the quadratic q through f, f′, f″ at a */
+int bolt_filon(bolt_moment_table* tab, const double* f, const double* fp, const double* fpp, const double* k,
+               const double* a, const double* b, int n, double* out);
+/* Σ_i of the Filon rule over (t_i, t_{i+1}), f, f′, f″ at t_i, for each of nk
+ * wavenumbers: a piecewise-quadratic integrand against j_ν(kt) (the loop form,
+ * integrator.jl:20-38) */
+int bolt_filon_sum(bolt_moment_table* tab, const double* t, const double* f, const double* fp, const double* fpp,
+                   int nt, const double* k, int nk, double* out);
+/* the moment functions at n points (x_i, ν_i, m_i): method 0 the Lommel
+ * asymptotic (sph_j_moment_asymp, moments.jl:76-86; x > ν), 1 the FP64
+ * Maclaurin series (sph_j_moment_maclaurin_₁F₂, moments.jl:97-102), 2 a table
+ * node's method: the series in double-double below x = 20, the recurrences from
+ * 20 to 40 (integer 0 <= m <= 3, ν <= 3) */
+int bolt_sph_moments(bolt_ctx* ctx, int method, const double* x, const int* nu, const double* m, int n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

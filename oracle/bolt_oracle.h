@@ -126,6 +126,26 @@ int orc_cltt(const orc_params* p, const orc_tables* t, const double* S, const do
              int nk_src, const double* kgrid, int nkd, const int* ells, int nell, int bessel_mode,
              double* cl, int nthreads, int kbeg, int kend);
 
+/* ---- bessel_moments.c: I_m(x) = ∫_0^x t^m j_ν(t) dt (src/bessel/moments.jl),
+ * their tables (interpolator.jl) and the Filon rule (integrator.jl) */
+double orc_sphj_moment_asymp_prefactor(int nu, double m);
+double orc_sphj_moment_asymp(double x, int nu, double m);
+double orc_sphj_moment_maclaurin(double x, int nu, double m);
+double orc_sphj_moment_maclaurin_q(double x, int nu, double m);
+double orc_J_moment_halfint(double x, int n, double alpha, int method);
+double orc_linrange(double xmin, double xmax, int n, int i);
+typedef struct {
+    int nu, order, n;
+    double xmin, xmax;
+    double* coef; /* [order][n + 2] */
+} orc_moment_table;
+int orc_moment_table_build(orc_moment_table* T, int nu, int order, double xmin, double xmax, int n, int nthreads);
+void orc_moment_table_free(orc_moment_table* T);
+void orc_moment_eval(const orc_moment_table* T, double x, double* out);
+double orc_filon(double f, double fp, double fpp, double k, double a, double b, const orc_moment_table* T);
+void orc_filon_sum(const orc_moment_table* T, const double* t, const double* f, const double* fp, const double* fpp,
+                   int nt, const double* k, int nk, double* out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,10 @@ class OrcCfg(ctypes.Structure):
                 ("abstol", _D), ("max_steps", _I), ("lu_refine", _I)]
 
 
+class OrcMomentTable(ctypes.Structure):
+    _fields_ = [("nu", _I), ("order", _I), ("n", _I), ("xmin", _D), ("xmax", _D), ("coef", _PD)]
+
+
 class OrcStats(ctypes.Structure):
     _fields_ = [("accepted", ctypes.c_int64), ("rejected", ctypes.c_int64),
                 ("rhs_evals", ctypes.c_int64), ("factorizations", ctypes.c_int64), ("status", _I)]
@@ -74,6 +78,19 @@ def lib():
         L.orc_bessel_table.argtypes = [_I, _D, _PD]
         L.orc_bessel_table_eval.argtypes = [_PD, _D, _D]
         L.orc_bessel_table_eval.restype = _D
+        for name in ("orc_sphj_moment_asymp", "orc_sphj_moment_maclaurin", "orc_sphj_moment_maclaurin_q"):
+            getattr(L, name).argtypes = [_D, _I, _D]
+            getattr(L, name).restype = _D
+        L.orc_linrange.argtypes = [_D, _D, _I, _I]
+        L.orc_linrange.restype = _D
+        L.orc_J_moment_halfint.argtypes = [_D, _I, _D, _I]
+        L.orc_J_moment_halfint.restype = _D
+        L.orc_moment_table_build.argtypes = [ctypes.POINTER(OrcMomentTable), _I, _I, _D, _D, _I, _I]
+        L.orc_moment_table_free.argtypes = [ctypes.POINTER(OrcMomentTable)]
+        L.orc_moment_eval.argtypes = [ctypes.POINTER(OrcMomentTable), _D, _PD]
+        L.orc_filon.argtypes = [_D, _D, _D, _D, _D, _D, ctypes.POINTER(OrcMomentTable)]
+        L.orc_filon.restype = _D
+        L.orc_filon_sum.argtypes = [ctypes.POINTER(OrcMomentTable), _PD, _PD, _PD, _PD, _I, _PD, _I, _PD, _I]
         L.orc_cltt.argtypes = [P, T, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _I, _I, _I]
         L.orc_cl.argtypes = [P, T, _PD, _PD, _PD, _I, _PD, _I, _PI, _I, _I, _PD, _PD, _PD, _I, _I, _I]
         L.orc_tableau.argtypes = [_PD, _PD, _PD, _PD, _PD]
@@ -281,3 +298,51 @@ def bessel_table(ell, zmax):
 
 def bessel_table_eval(c, zmax, z):
     return lib().orc_bessel_table_eval(_pd(c), zmax, z)
+
+
+# ---- moments of j_ν, their tables and the Filon rule (oracle/bessel_moments.c)
+def sphj_moment(x, nu, m, method="asymp"):
+    """∫_0^x t^m j_ν(t) dt: method "asymp" (Lommel), "maclaurin64" (FP64 ₁F₂
+    series), "series" (the same series in __float128, the tables' nodes below 40)"""
+    f = {"asymp": lib().orc_sphj_moment_asymp, "maclaurin64": lib().orc_sphj_moment_maclaurin,
+         "series": lib().orc_sphj_moment_maclaurin_q}[method]
+    return f(float(x), int(nu), float(m))
+
+
+def J_moment_halfint(x, n, alpha, method="asymp"):
+    """∫_0^x t^α J_{n+½}(t) dt"""
+    return lib().orc_J_moment_halfint(float(x), int(n), float(alpha), 0 if method == "asymp" else 1)
+
+
+class MomentTable:
+    """sph_bessel_interpolator(ν, order, x_min, x_max, N) (interpolator.jl:85-111)"""
+
+    def __init__(self, nu, order, xmin, xmax, n, nthreads=8):
+        self.t = OrcMomentTable()
+        r = lib().orc_moment_table_build(ctypes.byref(self.t), nu, order, float(xmin), float(xmax), int(n), nthreads)
+        assert r == 0
+        self.order = order
+
+    def __call__(self, x):
+        out = np.zeros(self.order)
+        lib().orc_moment_eval(ctypes.byref(self.t), float(x), _pd(out))
+        return out
+
+    def coefficients(self):
+        return np.ctypeslib.as_array(self.t.coef, shape=(self.order, self.t.n + 2)).copy()
+
+    def filon(self, f, fp, fpp, k, a, b):
+        return lib().orc_filon(float(f), float(fp), float(fpp), float(k), float(a), float(b), ctypes.byref(self.t))
+
+    def filon_sum(self, t, f, fp, fpp, ks, nthreads=8):
+        t, f, fp, fpp, ks = (np.ascontiguousarray(v, dtype=np.float64) for v in (t, f, fp, fpp, ks))
+        out = np.zeros(len(ks))
+        lib().orc_filon_sum(ctypes.byref(self.t), _pd(t), _pd(f), _pd(fp), _pd(fpp), len(t), _pd(ks), len(ks),
+                            _pd(out), nthreads)
+        return out
+
+    def __del__(self):
+        try:
+            lib().orc_moment_table_free(ctypes.byref(self.t))
+        except Exception:
+            pass
