@@ -523,6 +523,39 @@ def jvp(args):
     print(json.dumps(line), flush=True)
 
 
+def tables(args):
+    """Background + RECFAST tables for n cosmologies from the serving box (half
+    with a massive ν): bolt_device_tables_batch (one wavefront per cosmology) and
+    bolt_host_tables_batch on the host's threads, after one untimed device call."""
+    import boltmi
+    rng = np.random.default_rng(args.seed)
+
+    def box(n):
+        out = []
+        for c in range(n):
+            h = rng.uniform(0.6, 0.8)
+            out.append(boltmi.CosmoParams(h=h, Ω_b=rng.uniform(0.020, 0.024) / h ** 2,
+                                          Ω_c=rng.uniform(0.10, 0.14) / h ** 2, n=rng.uniform(0.92, 1.0),
+                                          A=math.exp(rng.uniform(2.9, 3.2)) * 1e-10, N_ν=3.046,
+                                          Σm_ν=(rng.uniform(0.06, 0.3) if c % 2 else 0.0) * boltmi.MASS_NATURAL))
+        return out
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    boltmi.backgrounds(box(1), device=True)
+    rows = {}
+    for n in (1, 64, 256, 1024):
+        p = box(n)
+        t0 = time.perf_counter()
+        boltmi.backgrounds(p, device=True)
+        t1 = time.perf_counter()
+        boltmi.backgrounds(p, threads=threads)
+        t2 = time.perf_counter()
+        rows[str(n)] = {"device_ms": (t1 - t0) * 1e3, "host_ms": (t2 - t1) * 1e3}
+        print(f"# {n} cosmologies: device {rows[str(n)]['device_ms']:.0f} ms, host ({threads} threads) "
+              f"{rows[str(n)]['host_ms']:.0f} ms", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "Background + RECFAST tables, wall time per batch", "unit": "ms",
+                      "host_threads": threads, "batches": rows}), flush=True)
+
+
 def serve(args):
     print(json.dumps(serve_run(args.rate, args.requests, args.slots, args.hbm_gib, args.seed, args.queue)), flush=True)
 
@@ -615,6 +648,8 @@ def main():
                     help="run one device-to-device copy of this many MiB before the workload (reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--c1", action="store_true", help="measure the plain-method workload (100 k, l=10, states + P(k))")
+    ap.add_argument("--tables", action="store_true",
+                    help="time the Background + RECFAST tables on the device and on the host for 1..1024 cosmologies")
     ap.add_argument("--jvp", action="store_true",
                     help="measure the forward-mode derivative of the full-CMB spectrum against the spectrum alone")
     ap.add_argument("--c4", action="store_true",
@@ -638,6 +673,9 @@ def main():
 
     if args.serve:
         serve(args)
+        return
+    if args.tables:
+        tables(args)
         return
     if args.batch:
         args.ncosmo = args.ncosmo or 64

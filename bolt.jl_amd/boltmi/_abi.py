@@ -149,6 +149,8 @@ SIGNATURES = {
     "bolt_filon": (_I, [_VP, _PD, _PD, _PD, _PD, _PD, _PD, _I, _PD]),
     "bolt_filon_sum": (_I, [_VP, _PD, _PD, _PD, _PD, _I, _PD, _I, _PD]),
     "bolt_sph_moments": (_I, [_VP, _I, _PD, _PI, _PD, _I, _PD]),
+    "bolt_device_tables_batch": (_I, [_VP, ctypes.POINTER(CosmoParamsC), _I, _D, _D, _I, _I, _PD, _PD, _PD, _PD,
+                                      _PI]),
 }
 
 _lib = None

@@ -188,11 +188,13 @@ def _release_uploads(uploads):
     uploads.clear()
 
 
-def backgrounds(pars, x_grid=(-20.0, 0.0, 2001), nq=15, threads=0):
+def backgrounds(pars, x_grid=(-20.0, 0.0, 2001), nq=15, threads=0, device=False, ctx=None):
     """[(par, Background(par), IonizationHistory(RECFAST(bg), par, bg)) for par
     in pars], the tables built on `threads` host threads (0: all hardware
     threads) by bolt_host_tables_batch; each bitwise what Background(par) builds.
-    This is the table stage of a many-cosmology batch (source_grid_batch)."""
+    This is the table stage of a many-cosmology batch (source_grid_batch).
+    device=True builds them on the GPU instead (bolt_device_tables_batch, one
+    workgroup per cosmology), equal to the host's to RECFAST's ODE tolerance."""
     pars = list(pars)
     x0, x1, nx = float(x_grid[0]), float(x_grid[1]), int(x_grid[2])
     nc = len(pars)
@@ -201,8 +203,12 @@ def backgrounds(pars, x_grid=(-20.0, 0.0, 2001), nq=15, threads=0):
     xq, wq = np.zeros(nq), np.zeros(nq)
     status = np.zeros(max(nc, 1), dtype=np.int32)
     cp = (_abi.CosmoParamsC * max(nc, 1))(*[p.c_struct() for p in pars])
-    check(lib().bolt_host_tables_batch(cp, nc, x0, x1, nx, int(nq), int(threads), pd(coef), pd(scal), pd(xq),
-                                       pd(wq), pi(status)))
+    if device:
+        check(lib().bolt_device_tables_batch((ctx or default_context()).handle, cp, nc, x0, x1, nx, int(nq), pd(coef),
+                                             pd(scal), pd(xq), pd(wq), pi(status)))
+    else:
+        check(lib().bolt_host_tables_batch(cp, nc, x0, x1, nx, int(nq), int(threads), pd(coef), pd(scal), pd(xq),
+                                           pd(wq), pi(status)))
     out = []
     for c, par in enumerate(pars):
         bg = Background(par, (x0, x1, nx), nq, _tables=(coef[c], scal[c], xq.copy(), wq.copy()))

@@ -6394,3 +6394,761 @@ int bolt_sph_moments(bolt_ctx* ctx, int method, const double* x, const int* nu, 
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Background, RECFAST and ionization tables on the device: bolt_host_tables'
+// algorithm (src/background.jl:32-128, src/ionization/recfast.jl:22-536,
+// 674-726, ionization.jl:95-137) with one wavefront per cosmology, so a batch
+// of cosmologies builds its tables side by side.  The node-wise work (ρ₀ℳ, ℋ,
+// η quadratures, τ′, g̃, c_s²) and the spline derivatives run across the
+// wavefront's lanes; the spline prefilters run one table per lane; the
+// RECFAST solve (two root finds, the He stage, the H + He stage, then the
+// reionized matter temperature from z = 50) is one Dormand–Prince 5(4)
+// integration by the wavefront, which evaluates each rate function's
+// transcendental terms on separate lanes; the same steps and tolerances as
+// the host's.  The
+// arithmetic is the host's, uncontracted; only the transcendental functions
+// differ (the device's math library), which moves the adaptive steps and
+// leaves differences at the level of the ODE tolerance (1e-11).
+// ============================================================================
+namespace gbg {
+
+// RECFAST's constants of one cosmology (the host RECFAST's constructor)
+struct RfC {
+    double Yp, OmegaB, HO, Tnow, mu_H, mu_T, fHe, Nnow;
+};
+struct Par {  // one cosmology: its parameters and Background's and RECFAST's constants (from the host)
+    double h, Omega_r, Omega_b, Omega_c, N_nu, Sum_m_nu, Y_p;
+    double H0, rho_crit, Omega_L, rad_fac, Tnu, logqmin, logqmax;
+    RfC rc;
+};
+struct Args {
+    const Par* par;
+    int nx, nq;
+    double x0, x1, dx;
+    const double* xq;  // [nq] Gauss–Legendre nodes and weights
+    const double* wq;
+    double H0_conv, K_conv, m_H, sigma_T;
+    double* coef;   // [ncosmo][BOLT_NTAB][nx + 2]
+    double* scal;   // [ncosmo][BOLT_NSCAL]
+    double* work;   // [ncosmo][WK][nx]
+    int* status;    // [ncosmo]: 0, or 1 if the RECFAST integration ran out of steps
+};
+constexpr int WK = 12;
+
+__device__ inline double node(const Args& A, int j) { return (A.x0 * (A.nx - 1 - j) + A.x1 * j) / (A.nx - 1); }
+
+// CosmoT (host_tables.cpp; background.jl:21-77)
+__device__ inline double f0(const Par& P, double q) {
+#pragma clang fp contract(off)
+    return 2.0 / pow(2.0 * M_PI, 3) / (exp(q / P.Tnu) + 1.0);
+}
+__device__ inline double from_ui(double x, double lo, double hi) {
+#pragma clang fp contract(off)
+    return lo + (hi - lo) / 2.0 * (x + 1.0);
+}
+__device__ inline double to_ui(double lq, double lo, double hi) {
+#pragma clang fp contract(off)
+    return -1.0 + 2.0 / (hi - lo) * (lq - lo);
+}
+__device__ inline double dxdq(double q, double lo, double hi) {
+#pragma clang fp contract(off)
+    return (1.0 + to_ui(1.0 + lo, lo, hi)) / (q * log(10.0));
+}
+__device__ inline double xq2q(double x, double lo, double hi) { return pow(10.0, from_ui(x, lo, hi)); }
+__device__ double rho0_massive(const Args& A, const Par& P, double a) {
+#pragma clang fp contract(off)
+    const double m = P.Sum_m_nu;
+    double s = 0.0;
+    for (int i = 0; i < A.nq; ++i) {
+        const double q = xq2q(A.xq[i], P.logqmin, P.logqmax);
+        const double eps = sqrt(q * q + (a * m) * (a * m));
+        const double I = q * q * eps * f0(P, q) / dxdq(q, P.logqmin, P.logqmax);
+        s += I * A.wq[i];
+    }
+    return 4.0 * M_PI * pow(a, -4) * s;
+}
+__device__ double H_a(const Args& A, const Par& P, double a) {
+#pragma clang fp contract(off)
+    const double rho_nu = rho0_massive(A, P, a);
+    return P.H0 * sqrt((P.Omega_c + P.Omega_b) * pow(a, -3) + rho_nu / P.rho_crit +
+                       P.Omega_r * pow(a, -4) * P.rad_fac + P.Omega_L);
+}
+__device__ inline double Hc_a(const Args& A, const Par& P, double a) { return a * H_a(A, P, a); }
+__device__ double eta(const Args& A, const Par& P, double x) {
+#pragma clang fp contract(off)
+    const double lo = -13.75, hi = log10(exp(x));
+    double s = 0.0;
+    for (int i = 0; i < A.nq; ++i) {
+        const double ay = xq2q(A.xq[i], lo, hi);
+        s += 1.0 / (ay * Hc_a(A, P, ay)) / dxdq(ay, lo, hi) * A.wq[i];
+    }
+    return s;
+}
+
+// uniform cubic B-spline with Line(OnGrid()) ends (util.jl:11-13): the host's
+// prefilter_t, eval_t and node_derivatives
+__device__ void prefilter(const double* y, int n, double* c) {
+#pragma clang fp contract(off)
+    c[1] = y[0];
+    c[n] = y[n - 1];
+    const int m = n - 2;
+    // forward elimination into c[2 .. n-1]; cp_i = 1/(4 − cp_{i−1}) is bitwise
+    // constant from i ≈ 20 on, so the backward pass recomputes only its first terms
+    double cp = 0.25, dprev = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double d = 6.0 * y[i + 1];
+        if (i == 0) d -= c[1];
+        if (i == m - 1) d -= c[n];
+        if (i == 0) {
+            d = d / 4.0;
+        } else {
+            const double den = 4.0 - cp;
+            cp = 1.0 / den;
+            d = (d - dprev) / den;
+        }
+        c[i + 2] = d;
+        dprev = d;
+    }
+    double cpc = 0.25;
+    for (int i = 1; i <= 64; ++i) cpc = 1.0 / (4.0 - cpc);  // the converged multiplier
+    for (int i = m - 2; i >= 0; --i) {
+        double cpi = cpc;
+        if (i < 64) {
+            cpi = 0.25;
+            for (int k = 1; k <= i; ++k) cpi = 1.0 / (4.0 - cpi);
+        }
+        c[i + 2] -= cpi * c[i + 3];
+    }
+    c[0] = 2.0 * c[1] - c[2];
+    c[n + 1] = 2.0 * c[n] - c[n - 1];
+}
+__device__ double spl(const double* c, int n, double x0, double dx, double x, int deriv) {
+#pragma clang fp contract(off)
+    const double t = (x - x0) / dx;
+    int i = (int)floor(t);
+    if (i < 0) i = 0;
+    if (i > n - 2) i = n - 2;
+    const double s = t - i, r = 1.0 - s;
+    const double* p = c + i;
+    if (deriv == 0) {
+        const double w0 = r * r * r / 6.0, w1 = 2.0 / 3.0 - s * s + 0.5 * s * s * s,
+                     w2 = 2.0 / 3.0 - r * r + 0.5 * r * r * r, w3 = s * s * s / 6.0;
+        return w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3];
+    } else if (deriv == 1) {
+        const double w0 = -0.5 * r * r, w1 = -2.0 * s + 1.5 * s * s, w2 = 2.0 * r - 1.5 * r * r, w3 = 0.5 * s * s;
+        return (w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3]) / dx;
+    }
+    const double w0 = r, w1 = -2.0 + 3.0 * s, w2 = -2.0 + 3.0 * r, w3 = s;
+    return (w0 * p[0] + w1 * p[1] + w2 * p[2] + w3 * p[3]) / (dx * dx);
+}
+__device__ inline double node_deriv(const double* c, int j, double dx, int deriv) {
+#pragma clang fp contract(off)
+    const double* p = c + j;
+    return deriv == 1 ? (p[2] - p[0]) / (2.0 * dx) : (p[0] - 2.0 * p[1] + p[2]) / (dx * dx);
+}
+
+// RECFAST (recfast.jl:22-121 constants, 156-310 rates, 363-427): the literal
+// constants at compile time, the derived ones in constant memory (computed on
+// the host with the host RECFAST's formulas), the cosmology's own in RfC
+namespace rfc {
+constexpr double C = 2.99792458e8, k_B = 1.380658e-23, h_P = 6.6260755e-34, m_e = 9.1093897e-31,
+                 m_H = 1.673575e-27, not4 = 3.9715e0, sigma = 6.6524616e-29, a = 7.565914e-16, G = 6.6742e-11;
+constexpr double Lambda = 8.2245809e0, Lambda_He = 51.3e0, L_H_ion = 1.096787737e7, L_H_alpha = 8.225916453e6,
+                 L_He1_ion = 1.98310772e7, L_He2_ion = 4.389088863e7, L_He_2s = 1.66277434e7, L_He_2p = 1.71134891e7;
+constexpr double A2P_s = 1.798287e9, A2P_t = 177.58e0, L_He_2Pt = 1.690871466e7, L_He_2St = 1.5985597526e7,
+                 L_He2St_ion = 3.8454693845e6, sigma_He_2Ps = 1.436289e-22, sigma_He_2Pt = 1.484872e-22;
+constexpr double AGauss1 = -0.14e0, AGauss2 = 0.079e0, zGauss1 = 7.28e0, zGauss2 = 6.73e0, wGauss1 = 0.18e0,
+                 wGauss2 = 0.33e0;
+constexpr double a_PPB = 4.309, b_PPB = -0.6166, c_PPB = 0.6703, d_PPB = 0.5300;
+constexpr double b_VF = 0.711, b_trip = 0.761, H_frac = 1e-3, fu = 1.125, b_He = 0.86;
+constexpr int Heswitch = 6;
+}  // namespace rfc
+struct RfK {
+    double a_VF, T_0, T_1, a_trip, Lalpha, Lalpha_He, CDB, CDB_He, CB1, CB1_He1, CB1_He2, CR, CK, CK_He, CL, CL_He,
+        CT, Bfact;
+    // folded products for the rates: 1/T_0, 1/T_1, 2k_B/(m_H·not4·C²) (the Doppler width's),
+    // 1/(8π L_He_2Pt³), and γ_2P's constant factors 3A2P·C²/(√π σ 8π (C L)²)
+    double iT_0, iT_1, dop, itau_t, g2Ps, g2Pt;
+};
+__constant__ RfK c_rfk;
+
+// one spline table's view for the rates: ℋ and ℋ′ coefficients
+struct HcTab {
+    const double* hc;
+    const double* hcp;
+    int nx;
+    double x0, dx, iH0_conv;
+};
+
+struct R3 {
+    double f1, f2, f3;
+};
+
+// ion_recfast (recfast.jl:156-310), evaluated by a whole wavefront (every
+// lane calls it with the same arguments and receives the same result): its
+// independent transcendental terms are one log, one sqrt, one pow and one
+// exp across the lanes, lane i holding the i-th term's argument, then one
+// exp and one pow more for the four terms that need ℋ and n_He; every lane
+// combines them.  The formulas are the host's; divisions by T_mat, ℋ, 1+z and
+// x are multiplications by their reciprocals, and products of constants are
+// folded (constant memory, computed on the host), which moves results by
+// ulps.  One out-of-line copy serves every stage of the integration.
+__device__ __attribute__((noinline)) R3 rf_rates(double x_H, double x_He, double Tmat, double z, RfC P, HcTab H) {
+    using namespace rfc;
+    const RfK& K0 = c_rfk;
+    const double fHe = P.fHe, Nnow = P.Nnow, Tnow = P.Tnow;
+    const int L = threadIdx.x & (WAVE - 1);
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    const double x = x_H + fHe * x_He;
+    const double zp1 = 1.0 + z, i1z = 1.0 / zp1, iT = 1.0 / Tmat;
+    const double Trad = Tnow * zp1;
+    // lane 0: x_a = log a; lane 1: log(1 + z)
+    const double lg = log(L == 0 ? i1z : zp1);
+    const double x_a = readlane64(lg, 0), lz = readlane64(lg, 1);
+    const double sq = sqrt(Tmat * (L == 0 ? K0.iT_0 : (L == 1 ? K0.iT_1 : K0.dop)));
+    const double sq_0 = readlane64(sq, 0), sq_1 = readlane64(sq, 1), dsq = readlane64(sq, 2);
+    double pb = 1.0, pe = 1.0;
+    switch (L) {
+        case 0: pb = zp1; pe = 3; break;
+        case 1: pb = Tmat * 1e-4; pe = b_PPB; break;
+        case 2: pb = Tmat * 1e-4; pe = d_PPB; break;
+        case 3: pb = K0.CR * Tmat; pe = 1.5; break;
+        case 4: pb = 1.0 + sq_0; pe = 1 - b_VF; break;
+        case 5: pb = 1.0 + sq_1; pe = 1 + b_VF; break;
+        case 6: pb = 1.0 + sq_0; pe = 1 - b_trip; break;
+        case 7: pb = 1.0 + sq_1; pe = 1 + b_trip; break;
+        case 8: pb = Trad; pe = 4; break;
+        case 9: pb = zp1; pe = 1.5; break;
+        case 10: pb = Trad; pe = 3; break;
+        case 11: pb = (lz - zGauss1) * (1.0 / wGauss1); pe = 2; break;
+        case 12: pb = (lz - zGauss2) * (1.0 / wGauss2); pe = 2; break;
+        default: break;
+    }
+    const double pw = pow(pb, pe);
+    const double p13 = readlane64(pw, 0), pT_b = readlane64(pw, 1), pT_d = readlane64(pw, 2);
+    const double pCR = readlane64(pw, 3), pVF0 = readlane64(pw, 4), pVF1 = readlane64(pw, 5);
+    const double pTr0 = readlane64(pw, 6), pTr1 = readlane64(pw, 7), pTrad4 = readlane64(pw, 8);
+    const double pz15 = readlane64(pw, 9), pTrad3 = readlane64(pw, 10);
+    const double pG1 = readlane64(pw, 11), pG2 = readlane64(pw, 12);
+    // exp arguments: −E/T for the Boltzmann factors (E/k_B folded), the two Gaussians
+    constexpr double kTrip = h_P * C * L_He2St_ion / k_B, kPSt = h_P * C * (L_He_2Pt - L_He_2St) / k_B,
+                     k2St = h_P * C * L_He_2St / k_B;
+    double ea = 0.0;
+    switch (L) {
+        case 0: ea = -K0.CDB * iT; break;
+        case 1: ea = -K0.CDB_He * iT; break;
+        case 2: ea = fmin(K0.Bfact * iT, 680.); break;
+        case 3: ea = -pG1; break;
+        case 4: ea = -pG2; break;
+        case 5: ea = -kTrip * iT; break;
+        case 6: ea = -kPSt * iT; break;
+        case 7: ea = -K0.CL * iT; break;
+        case 8: ea = -K0.CL_He * iT; break;
+        case 9: ea = -k2St * iT; break;
+        default: break;
+    }
+    const double ex = exp(ea);
+    const double eCDB = readlane64(ex, 0), eCDBHe = readlane64(ex, 1), He_Boltz = readlane64(ex, 2);
+    const double eG1 = readlane64(ex, 3), eG2 = readlane64(ex, 4), eTrip = readlane64(ex, 5);
+    const double ePSt = readlane64(ex, 6), eCL = readlane64(ex, 7), eCLHe = readlane64(ex, 8);
+    const double e2St = readlane64(ex, 9);
+    const double hc = spl(H.hc, H.nx, H.x0, H.dx, x_a, 0), hcp = spl(H.hcp, H.nx, H.x0, H.dx, x_a, 0);
+    const double n = Nnow * p13;
+    const double n_He = fHe * Nnow * p13;
+    const double Hz = hc * zp1 * H.iH0_conv, iHz = 1.0 / Hz;
+    const double dHdz = (-hcp + hc) * H.iH0_conv;
+    const double Rdown = 1e-19 * a_PPB * pT_b / (1. + c_PPB * pT_d);
+    const double Rup = Rdown * pCR * eCDB;
+    const double Rdown_He = K0.a_VF / (sq_0 * pVF0 * pVF1);
+    const double Rup_He = 4. * (Rdown_He * pCR * eCDBHe);
+    const double K = K0.CK * iHz * (1.0 + AGauss1 * eG1 + AGauss2 * eG2);
+    const double Rdown_trip = K0.a_trip / (sq_0 * pTr0 * pTr1);
+    const double Rup_trip = Rdown_trip * eTrip * pCR * (4.0 / 3.0);
+    const int Heflag = ((x_He < 5.e-9) || (x_He > 0.980)) ? 0 : Heswitch;
+    // second round: the terms that need n_He and ℋ
+    const double tauHe_s = A2P_s * K0.CK_He * 3 * n_He * (1.0 - x_He) * iHz;
+    const double tauHe_t = A2P_t * n_He * (1. - x_He) * 3 * iHz * K0.itau_t;
+    const double gamma_2Ps = K0.g2Ps * fHe * (1.0 - x_He) / (dsq * (1.0 - x_H));
+    const double gamma_2Pt = K0.g2Pt * fHe * (1.0 - x_He) / (dsq * (1.0 - x_H));
+    // lanes 0, 1: exp(−τ_s), exp(−τ_t); lanes 2, 3: γ_2Ps^b_He, γ_2Pt^0.9
+    const double e2 = exp(L == 0 ? -tauHe_s : -tauHe_t);
+    const double p2 = pow(L == 2 ? gamma_2Ps : gamma_2Pt, L == 2 ? b_He : 0.9);
+    const double etHs = readlane64(e2, 0), etHt = readlane64(e2, 1), pgPs = readlane64(p2, 2), pgPt = readlane64(p2, 3);
+    double K_He, CfHe_t = 0.0;
+    const double omHe = 1.0 - x_He, omH = 1.0 - x_H;
+    if (Heflag == 0) {
+        K_He = K0.CK_He * iHz;
+    } else {
+        const double pHe_s = (1.0 - etHs) / tauHe_s;
+        K_He = 1.0 / (A2P_s * pHe_s * 3 * n_He * omHe);
+        if (((Heflag == 2) || (Heflag >= 5)) && (x_H < 0.9999999)) {
+            const double AHcon = A2P_s / (1.0 + 0.36 * pgPs);
+            K_He = 1.0 / ((A2P_s * pHe_s + AHcon) * 3 * n_He * omHe);
+        }
+        if (Heflag >= 3) {
+            const double pHe_t = (1.0 - etHt) / tauHe_t;
+            if ((Heflag == 3) || (Heflag == 5) || (x_H > 0.99999)) {
+                CfHe_t = A2P_t * pHe_t * ePSt;
+            } else {
+                const double AHcon = A2P_t / (1.0 + 0.66 * pgPt) * (1.0 / 3);
+                CfHe_t = (A2P_t * pHe_t + AHcon) * ePSt;
+            }
+            CfHe_t = CfHe_t / (Rup_trip + CfHe_t);
+        }
+    }
+    const double ix = 1.0 / x;
+    const double timeTh = (1.0 + x + fHe) * ix / (K0.CT * pTrad4);
+    const double timeH = 2.0 / (3 * P.HO * pz15);
+    const double iHz1z = iHz * i1z;  // 1 / (ℋ(1 + z)), in the rates' units
+    if (x_H > 0.99) {
+        f1 = 0.;
+    } else if (x_H > 0.985) {
+        f1 = (x * x_H * n * Rdown - Rup * omH * eCL) * iHz1z;
+    } else {
+        const double KLn = K * Lambda * n * omH;
+        f1 = ((x * x_H * n * Rdown - Rup * omH * eCL) * (1.0 + KLn)) * iHz1z /
+             ((1.0 / fu) * (1.0 + KLn) + K * Rup * n * omH);
+    }
+    if (x_He < 1e-15) {
+        f2 = 0.;
+    } else {
+        const double KB = K_He * n_He * omHe * He_Boltz;
+        f2 = ((x * x_He * n * Rdown_He - Rup_He * omHe * eCLHe) * (1.0 + KB * Lambda_He)) * iHz1z /
+             (1.0 + KB * (Lambda_He + Rup_He));
+        if (Heflag >= 3) f2 = f2 + (x * x_He * n * Rdown_trip - omHe * 3 * Rup_trip * e2St) * CfHe_t * iHz1z;
+    }
+    if (timeTh < H_frac * timeH) {
+        const double epsilon = Hz * (1.0 + x + fHe) * ix / (K0.CT * pTrad3);
+        f3 = Tnow + epsilon * ((1.0 + fHe) / (1.0 + fHe + x)) * ((f1 + fHe * f2) * ix) - epsilon * dHdz * iHz +
+             3 * epsilon * i1z;
+    } else {
+        f3 = K0.CT * pTrad4 * x / (1.0 + x + fHe) * (Tmat - Trad) * iHz1z + 2 * Tmat * i1z;
+    }
+    return {f1, f2, f3};
+}
+__device__ inline double saha_rhs(double z, double cb, const RfC& P) {
+#pragma clang fp contract(off)
+    return exp(1.5 * log(c_rfk.CR * P.Tnow / (1.0 + z)) - cb / (P.Tnow * (1.0 + z))) / P.Nnow;
+}
+__device__ inline double he_saha_x0(double z, const RfC& P) {
+#pragma clang fp contract(off)
+    const double rhs = saha_rhs(z, c_rfk.CB1_He1, P) * 4;
+    return 0.5 * (sqrt((rhs - 1.0) * (rhs - 1.0) + 4 * (1.0 + P.fHe) * rhs) - (rhs - 1.0));
+}
+__device__ inline double x_H0_saha(double z, const RfC& P) {
+#pragma clang fp contract(off)
+    const double rhs = saha_rhs(z, c_rfk.CB1, P);
+    return 0.5 * (sqrt(rhs * rhs + 4 * rhs) - rhs);
+}
+template <class F>
+__device__ double rf_bisect(F&& g, double lo, double hi) {
+    double flo = g(lo);
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double fm = g(mid);
+        if ((fm > 0) == (flo > 0)) { lo = mid; flo = fm; } else { hi = mid; }
+        if (fabs(hi - lo) <= 1e-13 * fmax(1.0, fabs(mid))) break;
+    }
+    return 0.5 * (lo + hi);
+}
+__device__ inline double xe_saha_early(double z, const RfC& P) {
+#pragma clang fp contract(off)
+    const double fHe = P.fHe;
+    if (z > 8000.) return 1.0 + 2 * fHe;
+    if (z > 5000.) {
+        const double rhs = saha_rhs(z, c_rfk.CB1_He2, P);
+        return 0.5 * (sqrt((rhs - 1.0 - fHe) * (rhs - 1.0 - fHe) + 4 * (1.0 + 2 * fHe) * rhs) - (rhs - 1.0 - fHe));
+    }
+    if (z > 3500.) return 1.0 + fHe;
+    return he_saha_x0(z, P);
+}
+
+// dopri_outputs (host_tables.cpp): explicit Dormand–Prince 5(4) with step
+// clipping onto the requested outputs zout(0..nout-1); out(i, y) receives them.
+// Returns false if 500,000 steps ran out (the solves here take ~1e4) or the
+// step size fell below 1e-12 of z.
+template <int N, class F, class Z, class O>
+__device__ bool dopri(F&& f, double z0, double* y, int nout, Z&& zout, O&& out, double rtol, double atol) {
+#pragma clang fp contract(off)
+    const double c2 = 1.0 / 5, c3 = 3.0 / 10, c4 = 4.0 / 5, c5 = 8.0 / 9;
+    const double a21 = 1.0 / 5;
+    const double a31 = 3.0 / 40, a32 = 9.0 / 40;
+    const double a41 = 44.0 / 45, a42 = -56.0 / 15, a43 = 32.0 / 9;
+    const double a51 = 19372.0 / 6561, a52 = -25360.0 / 2187, a53 = 64448.0 / 6561, a54 = -212.0 / 729;
+    const double a61 = 9017.0 / 3168, a62 = -355.0 / 33, a63 = 46732.0 / 5247, a64 = 49.0 / 176, a65 = -5103.0 / 18656;
+    const double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
+    const double e1 = 71.0 / 57600, e3 = -71.0 / 16695, e4 = 71.0 / 1920, e5 = -17253.0 / 339200, e6 = 22.0 / 525,
+                 e7 = -1.0 / 40;
+    double z = z0;
+    double h = (nout == 0 ? 0.0 : (zout(nout - 1) < z0 ? -1e-3 : 1e-3)) * fmax(1.0, fabs(z0));
+    double k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], yt[N], yn[N];
+    f(z, y, k1);
+    long long guard = 0;
+    for (int io = 0; io < nout; ++io) {
+        const double zt = zout(io);
+        while (z != zt) {
+            if (++guard > 500000 || !(fabs(h) > 1e-12 * fmax(1.0, fabs(z)))) return false;
+            const double dir = (zt < z) ? -1.0 : 1.0;
+            if (h * dir <= 0) h = -h;
+            bool last = false;
+            if (fabs(h) >= fabs(zt - z)) { h = zt - z; last = true; }
+            for (int i = 0; i < N; ++i) yt[i] = y[i] + h * a21 * k1[i];
+            f(z + c2 * h, yt, k2);
+            for (int i = 0; i < N; ++i) yt[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+            f(z + c3 * h, yt, k3);
+            for (int i = 0; i < N; ++i) yt[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
+            f(z + c4 * h, yt, k4);
+            for (int i = 0; i < N; ++i) yt[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
+            f(z + c5 * h, yt, k5);
+            for (int i = 0; i < N; ++i)
+                yt[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
+            f(z + h, yt, k6);
+            for (int i = 0; i < N; ++i) yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
+            f(z + h, yn, k7);
+            double err = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double e = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] + e7 * k7[i]);
+                const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+                err += (e / sc) * (e / sc);
+            }
+            err = sqrt(err / N);
+            if (!(err == err)) err = 1e10;
+            if (err <= 1.0) {
+                z = last ? zt : z + h;
+                for (int i = 0; i < N; ++i) { y[i] = yn[i]; k1[i] = k7[i]; }
+                const double fac = (err == 0.0) ? 5.0 : fmin(5.0, fmax(0.2, 0.9 * pow(err, -0.2)));
+                if (!last) h *= fac;
+            } else {
+                h *= fmax(0.1, 0.9 * pow(err, -0.2));
+            }
+        }
+        out(io, y);
+    }
+    return true;
+}
+
+// IonizationHistory(RECFAST) on the grid (recfast.jl:446-536, 674-726): X_e and
+// T_mat at every node into xe[], tm[], by one whole wavefront (the rates are
+// evaluated across its lanes; every lane carries the same integration)
+__device__ bool recfast_grid(const Args& A, const Par& Pc, const double* coef, double* xe, double* tm) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & (WAVE - 1);
+    const RfC P = Pc.rc;
+    HcTab H;
+    H.hc = coef + (size_t)BOLT_T_HC * (A.nx + 2);
+    H.hcp = coef + (size_t)BOLT_T_HC_P * (A.nx + 2);
+    H.nx = A.nx; H.x0 = A.x0; H.dx = A.dx; H.iH0_conv = 1.0 / A.H0_conv;
+    const double fHe = P.fHe, Tnow = P.Tnow;
+    const int nx = A.nx;
+    const double zinitial = 10000.0;
+    const double x_init = log(1.0 / (zinitial + 1.0));
+    const double zre_ini = 50.0;
+    int j_init = nx;
+    for (int j = 0; j < nx; ++j)
+        if (!(node(A, j) < x_init)) { j_init = j; break; }
+    auto zj = [&](int j) { return 1.0 / exp(node(A, j)) - 1.0; };
+    // the stage switches (recfast.jl:373-380, 408, 451-459)
+    const double zHeS = rf_bisect([&](double z) { return (he_saha_x0(z, P) - 1.0) / fHe - 0.99; }, 3500.0, 0.0);
+    const double zHHS = rf_bisect([&](double z) { return x_H0_saha(z, P) - 0.985; }, zHeS, 0.0);
+    // node ranges by stage, in descending z (ascending j)
+    int jHe = j_init;  // first node with z <= zHeS
+    while (jHe < nx && zj(jHe) > zHeS) ++jHe;
+    int jHH = jHe;     // first node with z <= zHHS
+    while (jHH < nx && zj(jHH) > zHHS) ++jHH;
+    int jre = jHH;     // first node with z <= 50
+    while (jre < nx && zj(jre) > zre_ini) ++jre;
+    for (int j = lane; j < jHe; j += WAVE) {
+        const double Trad = Tnow * (1 + (1.0 / exp(node(A, j)) - 1.0));
+        if (j < j_init) { xe[j] = 1.0 + 2 * fHe; tm[j] = Trad; }
+        else { const double z = zj(j); xe[j] = xe_saha_early(z, P); tm[j] = Tnow * (1.0 + z); }
+    }
+    // stage 2: He + Tmat with H in Saha (recfast.jl:461-465, 383-399), outputs
+    // at the nodes, then at zHHS
+    double y2[2];
+    {
+        const double x0 = he_saha_x0(zHeS, P);
+        y2[0] = (x0 - 1.0) / fHe;
+        y2[1] = Tnow * (1.0 + zHeS);
+    }
+    auto fHe_stage = [&](double z, const double* y, double* dy) {
+        const R3 f = rf_rates(x_H0_saha(z, P), y[0], y[1], z, P, H);
+        dy[0] = f.f2;
+        dy[1] = f.f3;
+    };
+    const int n2 = jHH - jHe;
+    bool ok = dopri<2>(fHe_stage, zHeS, y2, n2 + 1, [&](int i) { return i < n2 ? zj(jHe + i) : zHHS; },
+                       [&](int i, const double* y) {
+                           if (i < n2 && lane == 0) {
+                               const double z = zj(jHe + i);
+                               xe[jHe + i] = x_H0_saha(z, P) + fHe * y[0];
+                               tm[jHe + i] = y[1];
+                           }
+                       }, 1e-11, 1e-14);
+    // stage 3: H + He + Tmat down to z = 50 (recfast.jl:467-471)
+    double y3[3] = {x_H0_saha(zHHS, P), y2[0], y2[1]};
+    auto fHH = [&](double z, const double* y, double* dy) {
+        const R3 f = rf_rates(y[0], y[1], y[2], z, P, H);
+        dy[0] = f.f1; dy[1] = f.f2; dy[2] = f.f3;
+    };
+    const int n3 = jre - jHH;
+    ok = ok && dopri<3>(fHH, zHHS, y3, n3 + 1, [&](int i) { return i < n3 ? zj(jHH + i) : zre_ini; },
+                        [&](int i, const double* y) {
+                            if (i < n3 && lane == 0) {
+                                xe[jHH + i] = y[0] + fHe * y[1];
+                                tm[jHH + i] = y[2];
+                            }
+                        }, 1e-11, 1e-14);
+    // stage 4: with the reionized matter temperature from z = 50 (recfast.jl:478-536)
+    const double X_fin = 1.0 + P.Yp / (rfc::not4 * (1.0 - P.Yp));
+    const double zre = 7.6711, alpha = 1.5, dH = 0.5, zHe = 3.5, dHe = 0.5;
+    const double fHe_r = X_fin - 1.0;
+    auto reio_xe = [&](double z, double x_orig) {
+        const double xH = (X_fin - x_orig) / 2 *
+                              (1.0 + tanh((pow(1 + zre, alpha) - pow(1.0 + z, alpha)) / (alpha * pow(1 + zre, alpha - 1)) / dH)) +
+                          x_orig;
+        const double xHe = fHe_r / 2 * (1.0 + tanh((zHe - z) / dHe));
+        return xH + xHe;
+    };
+    double y4[4] = {y3[0], y3[1], y3[2], y3[2]};
+    auto f4 = [&](double z, const double* y, double* dy) {
+        const R3 f = rf_rates(y[0], y[1], y[2], z, P, H);
+        dy[0] = f.f1; dy[1] = f.f2; dy[2] = f.f3;
+        const double xr = reio_xe(z, y[0] + fHe * y[1]);
+        const double aa = 1.0 / (1.0 + z);
+        const double Hz = spl(H.hc, H.nx, H.x0, H.dx, log(aa), 0) / aa / A.H0_conv;
+        const double Trad = Tnow * (1.0 + z);
+        dy[3] = c_rfk.CT * pow(Trad, 4) * xr / (1.0 + xr + fHe) * (y[3] - Trad) / (Hz * (1.0 + z)) + 2 * y[3] / (1.0 + z);
+    };
+    const int n4 = nx - jre;
+    ok = ok && dopri<4>(f4, zre_ini, y4, n4, [&](int i) { return zj(jre + i); },
+                        [&](int i, const double* y) {
+                            if (lane != 0) return;
+                            const double z = zj(jre + i);
+                            xe[jre + i] = reio_xe(z, y[0] + fHe * y[1]);
+                            tm[jre + i] = y[3];
+                        }, 1e-11, 1e-14);
+    return ok;
+}
+
+// one wavefront per cosmology: the RECFAST integration needs exactly one, and
+// the node-wise phases are short, so more cosmologies fit on a CU at once
+__global__ void __launch_bounds__(WAVE) k_tables(Args A) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+    const Par P = A.par[c];
+    const int nx = A.nx;
+    double* coef = A.coef + (size_t)c * BOLT_NTAB * (nx + 2);
+    double* W = A.work + (size_t)c * WK * nx;
+    auto tab = [&](int k) { return coef + (size_t)k * (nx + 2); };
+    auto w = [&](int k) { return W + (size_t)k * nx; };
+    // ρ₀ℳ, ℋ, η at the nodes (background.jl:106-108)
+    for (int j = t; j < nx; j += nt) {
+        const double x = node(A, j);
+        w(0)[j] = rho0_massive(A, P, exp(x));
+        w(1)[j] = Hc_a(A, P, exp(x));
+        w(2)[j] = eta(A, P, x);
+    }
+    __syncthreads();
+    if (t == 0) prefilter(w(0), nx, tab(BOLT_T_RHO0M));
+    if (t == 1) prefilter(w(1), nx, tab(BOLT_T_HC));
+    if (t == 2) prefilter(w(2), nx, tab(BOLT_T_ETA));
+    __syncthreads();
+    // ℋ′, ℋ″, η′, η″: splines of the node derivatives (util.jl:12-13)
+    for (int j = t; j < nx; j += nt) {
+        w(3)[j] = node_deriv(tab(BOLT_T_HC), j, A.dx, 1);
+        w(4)[j] = node_deriv(tab(BOLT_T_HC), j, A.dx, 2);
+        w(5)[j] = node_deriv(tab(BOLT_T_ETA), j, A.dx, 1);
+        w(6)[j] = node_deriv(tab(BOLT_T_ETA), j, A.dx, 2);
+    }
+    __syncthreads();
+    if (t == 0) prefilter(w(3), nx, tab(BOLT_T_HC_P));
+    if (t == 1) prefilter(w(4), nx, tab(BOLT_T_HC_PP));
+    if (t == 2) prefilter(w(5), nx, tab(BOLT_T_ETA_P));
+    if (t == 3) prefilter(w(6), nx, tab(BOLT_T_ETA_PP));
+    if (t == 4) A.scal[(size_t)c * BOLT_NSCAL + BOLT_S_ETA0] = eta(A, P, 0.0);  // background.jl:112
+    __syncthreads();
+    // RECFAST on the first wavefront: X_e into w(7), T_mat into w(8)
+    if (t < WAVE) {
+        const bool ok = recfast_grid(A, P, coef, w(7), w(8));
+        if (t == 0) A.status[c] = ok ? 0 : 1;
+    }
+    __syncthreads();
+    // τ′ at the nodes, its reverse cumulative trapezoid, g̃ (ionization.jl:107-129)
+    const double rho_crit = P.rho_crit;
+    for (int j = t; j < nx; j += nt) {
+        const double x = node(A, j), a = exp(x);
+        const double nH = P.Omega_b * rho_crit / (A.m_H * a * a * a) * (1.0 - P.Y_p);
+        w(9)[j] = -w(7)[j] * nH * a * A.sigma_T / spl(tab(BOLT_T_HC), nx, A.x0, A.dx, x, 0);
+    }
+    __syncthreads();
+    if (t == 0) {
+        double* tau = w(10);
+        const double* tp = w(9);
+        tau[nx - 1] = 0.0;
+        for (int j = nx - 2; j >= 0; --j) tau[j] = tau[j + 1] + (node(A, j) - node(A, j + 1)) * (tp[j] + tp[j + 1]) * 0.5;
+        A.scal[(size_t)c * BOLT_NSCAL + BOLT_S_TAU0] = tau[nx - 1];
+    }
+    __syncthreads();
+    for (int j = t; j < nx; j += nt) w(11)[j] = -w(9)[j] * exp(-w(10)[j]);  // g̃_function
+    __syncthreads();
+    if (t == 0) prefilter(w(7), nx, tab(BOLT_T_XE));
+    if (t == 1) prefilter(w(10), nx, tab(BOLT_T_TAU));
+    if (t == 2) prefilter(w(11), nx, tab(BOLT_T_G));
+    if (t == 3) prefilter(w(8), nx, tab(BOLT_T_TMAT));
+    __syncthreads();
+    for (int j = t; j < nx; j += nt) {
+        w(0)[j] = node_deriv(tab(BOLT_T_TAU), j, A.dx, 1);
+        w(1)[j] = node_deriv(tab(BOLT_T_TAU), j, A.dx, 2);
+        w(2)[j] = node_deriv(tab(BOLT_T_G), j, A.dx, 1);
+        w(3)[j] = node_deriv(tab(BOLT_T_G), j, A.dx, 2);
+        w(4)[j] = node_deriv(tab(BOLT_T_TMAT), j, A.dx, 1);
+    }
+    __syncthreads();
+    if (t == 0) prefilter(w(0), nx, tab(BOLT_T_TAU_P));
+    if (t == 1) prefilter(w(1), nx, tab(BOLT_T_TAU_PP));
+    if (t == 2) prefilter(w(2), nx, tab(BOLT_T_G_P));
+    if (t == 3) prefilter(w(3), nx, tab(BOLT_T_G_PP));
+    // sound speed (recfast.jl:710-712), with RECFAST's SI constants
+    {
+        const double C = 2.99792458e8, k_B = 1.380658e-23, m_H = 1.673575e-27, not4 = 3.9715e0;
+        const double Yp = P.Y_p, mu_T = not4 / (not4 - (not4 - 1) * Yp);
+        for (int j = t; j < nx; j += nt) {
+            const double x = node(A, j);
+            const double pre = pow(C, -2) * k_B / m_H * (1.0 / mu_T + (1.0 - Yp) * spl(tab(BOLT_T_XE), nx, A.x0, A.dx, x, 0));
+            w(5)[j] = pre * (spl(tab(BOLT_T_TMAT), nx, A.x0, A.dx, x, 0) - 1.0 / 3.0 * w(4)[j]);
+        }
+    }
+    __syncthreads();
+    if (t == 0) prefilter(w(5), nx, tab(BOLT_T_CSB2));
+    if (t == 1) {
+        double* s = A.scal + (size_t)c * BOLT_NSCAL;
+        s[BOLT_S_H0] = P.H0;
+        s[BOLT_S_RHO_CRIT] = P.rho_crit;
+        s[BOLT_S_OMEGA_L] = P.Omega_L;
+    }
+}
+
+}  // namespace gbg
+
+namespace gbg {
+// RECFAST's derived constants and one cosmology's, as the host RECFAST's
+// constructor computes them (host_tables.cpp, RecfastT; recfast.jl:22-121)
+static RfK rf_constants() {
+#pragma clang fp contract(off)
+    using namespace rfc;
+    RfK k;
+    k.a_VF = std::pow(10.0, -16.744);
+    k.T_0 = std::pow(10.0, 0.477121);
+    k.T_1 = std::pow(10.0, 5.114);
+    k.a_trip = std::pow(10.0, -16.306);
+    k.Lalpha = 1 / L_H_alpha;
+    k.Lalpha_He = 1 / L_He_2p;
+    const double DeltaB = h_P * C * (L_H_ion - L_H_alpha);
+    k.CDB = DeltaB / k_B;
+    const double DeltaB_He = h_P * C * (L_He1_ion - L_He_2s);
+    k.CDB_He = DeltaB_He / k_B;
+    k.CB1 = h_P * C * L_H_ion / k_B;
+    k.CB1_He1 = h_P * C * L_He1_ion / k_B;
+    k.CB1_He2 = h_P * C * L_He2_ion / k_B;
+    k.CR = 2 * M_PI * (m_e / h_P) * (k_B / h_P);
+    k.CK = std::pow(k.Lalpha, 3) / (8 * M_PI);
+    k.CK_He = std::pow(k.Lalpha_He, 3) / (8 * M_PI);
+    k.CL = C * h_P / (k_B * k.Lalpha);
+    k.CL_He = C * h_P / (k_B / L_He_2s);
+    k.CT = (8.0 / 3.0) * (sigma / (m_e * C)) * a;
+    k.Bfact = h_P * C * (L_He_2p - L_He_2s) / k_B;
+    k.iT_0 = 1.0 / k.T_0;
+    k.iT_1 = 1.0 / k.T_1;
+    k.dop = 2 * k_B / (m_H * not4 * C * C);
+    k.itau_t = 1.0 / (8 * M_PI * std::pow(L_He_2Pt, 3));
+    k.g2Ps = 3 * A2P_s * C * C / (std::sqrt(M_PI) * sigma_He_2Ps * 8 * M_PI * C * L_He_2p) / std::pow(C * L_He_2p, 2);
+    k.g2Pt = 3 * A2P_t * C * C / (std::sqrt(M_PI) * sigma_He_2Pt * 8 * M_PI * C * L_He_2Pt) / std::pow(C * L_He_2Pt, 2);
+    return k;
+}
+static RfC rf_cosmology(const bolt_cosmo_params& p, double H0, double rho_crit) {
+#pragma clang fp contract(off)
+    using namespace rfc;
+    const bolt::NaturalUnits& u = bolt::natural_units();
+    RfC r;
+    r.Yp = p.Y_p;
+    r.OmegaB = p.Omega_b;
+    r.HO = H0 / u.H0_conv;
+    const double OmegaG = p.Omega_r;
+    r.Tnow = std::pow(15.0 / (M_PI * M_PI) * rho_crit * OmegaG, 0.25) * u.K_conv;
+    r.mu_H = 1.0 / (1.0 - r.Yp);
+    r.mu_T = not4 / (not4 - (not4 - 1) * r.Yp);
+    r.fHe = r.Yp / (not4 * (1.0 - r.Yp));
+    r.Nnow = 3.0 * r.HO * r.HO * r.OmegaB / (8 * M_PI * G * r.mu_H * m_H);
+    return r;
+}
+}  // namespace gbg
+
+extern "C" {
+
+int bolt_device_tables_batch(bolt_ctx* ctx, const bolt_cosmo_params* pars, int ncosmo, double x0, double x1, int nx,
+                             int nq, double* coef_out, double* scal_out, double* xq_out, double* wq_out, int* status) {
+    if (!ctx || !pars || ncosmo < 0 || !coef_out || !scal_out || !xq_out || !wq_out || !status)
+        return set_error(BOLT_E_ARG, "bolt_device_tables_batch: NULL argument or negative count");
+    if (nx < 4 || nx > (1 << 20) || nq < 1 || nq > SV_MAX_NQ || !(x1 > x0))
+        return set_error(BOLT_E_ARG, "bolt_device_tables_batch: bad grid");
+    for (int c = 0; c < ncosmo; ++c)
+        if (!(pars[c].Omega_r > 0) || !(pars[c].N_nu > 0) || !(pars[c].h > 0))
+            return set_error(BOLT_E_ARG, "bolt_device_tables_batch: Omega_r, N_nu and h must be positive");
+    if (ncosmo == 0) return BOLT_OK;
+    // Background's constants per cosmology, from the host's own CosmoT
+    std::vector<gbg::Par> hp(ncosmo);
+    std::vector<double> xq, wq;
+    for (int c = 0; c < ncosmo; ++c) {
+        const bolt::CosmoT<double> cm(bolt::ParamsT<double>(pars[c]), nq);
+        gbg::Par& P = hp[c];
+        P.h = pars[c].h; P.Omega_r = pars[c].Omega_r; P.Omega_b = pars[c].Omega_b; P.Omega_c = pars[c].Omega_c;
+        P.N_nu = pars[c].N_nu; P.Sum_m_nu = pars[c].Sum_m_nu; P.Y_p = pars[c].Y_p;
+        P.H0 = cm.H0; P.rho_crit = cm.rho_crit; P.Omega_L = cm.Omega_L; P.rad_fac = cm.rad_fac; P.Tnu = cm.Tnu;
+        P.logqmin = cm.logqmin; P.logqmax = cm.logqmax;
+        P.rc = gbg::rf_cosmology(pars[c], cm.H0, cm.rho_crit);
+        if (c == 0) { xq = cm.xq; wq = cm.wq; }
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const gbg::RfK rk = gbg::rf_constants();
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(gbg::c_rfk), &rk, sizeof(rk), 0, hipMemcpyHostToDevice, st));
+    const size_t ntab = (size_t)BOLT_NTAB * (nx + 2);
+    Scratch<gbg::Par> dpar;
+    Scratch<double> dxq, dwq, dcoef, dscal, dwork;
+    Scratch<int> dst;
+    HIPCHK(dpar.up(hp.data(), ncosmo, st));
+    HIPCHK(dxq.up(xq.data(), nq, st));
+    HIPCHK(dwq.up(wq.data(), nq, st));
+    if (dcoef.alloc(ntab * ncosmo) != hipSuccess || dscal.alloc((size_t)BOLT_NSCAL * ncosmo) != hipSuccess ||
+        dwork.alloc((size_t)gbg::WK * nx * ncosmo) != hipSuccess || dst.alloc(ncosmo) != hipSuccess)
+        return set_error(BOLT_E_NOMEM, "bolt_device_tables_batch: device memory");
+    const bolt::NaturalUnits& u = bolt::natural_units();
+    gbg::Args A;
+    A.par = dpar.p; A.nx = nx; A.nq = nq; A.x0 = x0; A.x1 = x1; A.dx = (x1 - x0) / (nx - 1);
+    A.xq = dxq.p; A.wq = dwq.p;
+    A.H0_conv = u.H0_conv; A.K_conv = u.K_conv; A.m_H = u.m_H; A.sigma_T = u.sigma_T;
+    A.coef = dcoef.p; A.scal = dscal.p; A.work = dwork.p; A.status = dst.p;
+    hipLaunchKernelGGL(gbg::k_tables, dim3(ncosmo), dim3(WAVE), 0, st, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(coef_out, dcoef.p, sizeof(double) * ntab * ncosmo, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(scal_out, dscal.p, sizeof(double) * BOLT_NSCAL * ncosmo, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(status, dst.p, sizeof(int) * ncosmo, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < nq; ++i) { xq_out[i] = xq[i]; wq_out[i] = wq[i]; }
+    int bad = 0;
+    for (int c = 0; c < ncosmo; ++c) {
+        if (!status[c]) {
+            const double* cf = coef_out + ntab * c;
+            for (size_t i = 0; i < ntab; ++i)
+                if (!std::isfinite(cf[i])) { status[c] = BOLT_E_NUMERIC; break; }
+        } else {
+            status[c] = BOLT_E_NUMERIC;
+        }
+        bad += status[c] != 0;
+    }
+    if (bad) return set_error(BOLT_E_NUMERIC, "bolt_device_tables_batch: a cosmology's tables failed (status)");
+    return BOLT_OK;
+}
+
+}  // extern "C"

@@ -171,6 +171,16 @@ int bolt_ctx_create(int device, bolt_ctx** out);
 int bolt_ctx_destroy(bolt_ctx* ctx);
 int bolt_ctx_sync(bolt_ctx* ctx);
 
+/* bolt_host_tables_batch computed on the GPU of ctx: Background and
+ * IonizationHistory(RECFAST) (src/background.jl:85-128, src/ionization/
+ * recfast.jl:446-536, 674-726) with one workgroup per cosmology, the host's
+ * algorithm and arithmetic; RECFAST's adaptive steps follow the device's math
+ * library, so the tables agree with the host's to the ODE tolerance, not
+ * bitwise.  Same layouts as bolt_host_tables_batch; status[c] is 0 or
+ * BOLT_E_NUMERIC (RECFAST out of steps, or a non-finite entry). */
+int bolt_device_tables_batch(bolt_ctx* ctx, const bolt_cosmo_params* pars, int ncosmo, double x0, double x1, int nx,
+                             int nq, double* coef_out, double* scal_out, double* xq_out, double* wq_out, int* status);
+
 /* Layout introspection (no reference counterpart; host only, no GPU needed):
  * the solver's per-k-mode LDS workspace for state length n and nq momentum
  * bins (fp32 != 0: as the FP32 solve mode uses it), as `*nregions` [begin, end) byte ranges written to regions[2*i..] (at
