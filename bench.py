@@ -7,12 +7,15 @@ cltt for ℓ = 2..2500 on quadratic_k(0.01H₀, 1000H₀, 5000), flat ΛCDM with
 h = 0.7, Ω_b = 0.046, Ω_c = 0.224, n_s = 0.96, A_s = 2.1e-9, N_eff = 3.046,
 massless ν.  Tables, k grids and the source grid stay resident in HBM.
 
-Steps are independent spectra.  With --inflight N (default 3) they run on N
-contexts, each with its own HIP stream, result buffers and host thread, so a
-spectrum's solve can use the CUs that the previous one's longest modes leave
-idle; value counts completed spectra per second of wall time, and latency_ms
-gives the time of one spectrum as its caller sees it.  --inflight 1 runs them
-one after another.
+Steps are independent spectra.  --per-launch B (default 2) solves B spectra in
+one persistent solver launch (their 800·B modes share the work queue, largest k
+first), followed by B LOS passes.  With --inflight N (default 2) such batches
+run on N contexts, each with its own HIP stream, result buffers and host
+thread, so a batch's solve can use the CUs that the previous one's longest
+modes leave idle; value counts completed spectra per second of wall time, and
+latency_ms gives the time of one batch as its caller sees it (every spectrum of
+a batch completes with it).  --per-launch 1 --inflight 1 runs the spectra one
+after another.
 
 --batch measures the many-cosmology workload instead (k-mode solves/s of one
 persistent launch over 64 cosmologies × 800 k), and --serve a Poisson request
@@ -55,14 +58,16 @@ def workload():
     return par, bg, ih, ks, cfg, ells, dense
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size), or None."""
+def measured_traffic(kernel, per_launch=1):
+    """HBM bytes per launch of `kernel` (solving per_launch spectra) from the
+    committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE, scaled by a plain
+    copy of known size), or None where that shape was not measured."""
     import re
     path = os.path.join(ROOT, "profiles", "k_boltsolve_cmb_full.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+([0-9.eE+-]+)", open(path).read())
+    key = kernel if per_launch == 1 else f"{kernel}_x{per_launch}"
+    m = re.search(r"traffic_bytes_per_launch\s+" + re.escape(key) + r"\s+([0-9.eE+-]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -634,10 +639,12 @@ def serve_run(rate, nreq, slots, hbm_gib, seed, queue=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=9)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--inflight", type=int, default=3,
-                    help="independent spectra in flight, each on its own context (HIP stream) and host thread")
+    ap.add_argument("--steps", type=int, default=12,
+                    help="spectra timed (rounded up to a whole number of launches of --per-launch spectra)")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="independent batches in flight, each on its own context (HIP stream) and host thread")
+    ap.add_argument("--per-launch", type=int, default=2, help="spectra solved by one persistent solver launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-mode-per-launch GPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default solver and LOS shapes)")
@@ -698,6 +705,7 @@ def main():
 
     par, bg, ih, ks, cfg, ells, dense = workload()
     nlanes = max(1, args.inflight)
+    B = max(1, args.per_launch)
     L = _abi.lib()
     dev = torch.device("cuda:0")
     d_k = torch.tensor(ks, dtype=torch.float64, device=dev)
@@ -705,21 +713,26 @@ def main():
     ells_c = np.ascontiguousarray(ells)
 
     class Lane:
-        """one spectrum in flight: a context (its own HIP stream) and its own result buffers"""
+        """one batch of B spectra in flight: a context (its own HIP stream) and its own result buffers"""
 
         def __init__(self, ctx):
             self.ctx = ctx
             self.cid = bg.cosmo_id(ctx)
-            self.d_S = torch.empty((NK, bg.nx), dtype=torch.float64, device=dev)
-            self.d_status = torch.zeros(NK, dtype=torch.int32, device=dev)
-            self.d_ctr = torch.zeros((NK, 4), dtype=torch.int64, device=dev)
-            self.d_cl = torch.empty(len(ells), dtype=torch.float64, device=dev)
+            self.ids = (ctypes.c_int * B)(*([self.cid] * B))
+            self.cfgs = (_abi.HierCfgC * B)(*([cfg] * B))
+            self.d_kB = d_k.repeat(B).contiguous()
+            self.d_SB = torch.empty((B, NK, bg.nx), dtype=torch.float64, device=dev)
+            self.d_statusB = torch.zeros((B, NK), dtype=torch.int32, device=dev)
+            self.d_ctrB = torch.zeros((B, NK, 4), dtype=torch.int64, device=dev)
+            self.d_clB = torch.empty((B, len(ells)), dtype=torch.float64, device=dev)
+            # the batch's first spectrum (the baselines and the error check use it)
+            self.d_S, self.d_status, self.d_ctr, self.d_cl = self.d_SB[0], self.d_statusB[0], self.d_ctrB[0], self.d_clB[0]
             self.d_steps = torch.zeros(NKD - 1, dtype=torch.int64, device=dev)
             self.d_dense = d_dense
             self.ells = ells_c
 
     lanes = [Lane(boltmi.default_context())] + [Lane(boltmi.Context(0)) for _ in range(nlanes - 1)]
-    d_status, d_ctr, d_steps = lanes[0].d_status, lanes[0].d_ctr, lanes[0].d_steps
+    d_ctr, d_steps = lanes[0].d_ctrB, lanes[0].d_steps
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
@@ -730,16 +743,26 @@ def main():
     torch.cuda.synchronize()
 
     def step(ln):
-        """one whole spectrum on lane ln: source_grid, then cltt from that grid"""
+        """B whole spectra on lane ln: one source_grid launch for all of them,
+        then cltt from each grid; returns the solver launch's ms and the mean
+        LOS ms per spectrum"""
         ms1 = ctypes.c_float()
-        _abi.check(L.bolt_source_grid_dev(ln.ctx.handle, ln.cid, ctypes.byref(cfg), d_k.data_ptr(), NK,
-                                          ln.d_S.data_ptr(), ln.d_status.data_ptr(), ln.d_ctr.data_ptr(),
-                                          ctypes.byref(ms1)))
-        ms2 = ctypes.c_float()
-        _abi.check(L.bolt_cltt_dev(ln.ctx.handle, ln.cid, ln.d_S.data_ptr(), d_k.data_ptr(), NK, d_dense.data_ptr(),
-                                   NKD, _abi.pi(ells_c), len(ells_c), ln.d_cl.data_ptr(), ln.d_steps.data_ptr(),
-                                   ctypes.byref(ms2)))
-        return ms1.value, ms2.value
+        if B == 1:
+            _abi.check(L.bolt_source_grid_dev(ln.ctx.handle, ln.cid, ctypes.byref(cfg), d_k.data_ptr(), NK,
+                                              ln.d_S.data_ptr(), ln.d_status.data_ptr(), ln.d_ctr.data_ptr(),
+                                              ctypes.byref(ms1)))
+        else:
+            _abi.check(L.bolt_source_grid_batch_dev(ln.ctx.handle, B, ln.ids, ln.cfgs, ln.d_kB.data_ptr(), NK,
+                                                    ln.d_SB.data_ptr(), None, ln.d_statusB.data_ptr(),
+                                                    ln.d_ctrB.data_ptr(), ctypes.byref(ms1)))
+        los = 0.0
+        for b in range(B):
+            ms2 = ctypes.c_float()
+            _abi.check(L.bolt_cltt_dev(ln.ctx.handle, ln.cid, ln.d_SB[b].data_ptr(), d_k.data_ptr(), NK,
+                                       d_dense.data_ptr(), NKD, _abi.pi(ells_c), len(ells_c), ln.d_clB[b].data_ptr(),
+                                       ln.d_steps.data_ptr(), ctypes.byref(ms2)))
+            los += ms2.value
+        return ms1.value, los / B
 
     def run(counts, label):
         """counts[j] steps on lane j, the lanes concurrently (the C calls release the GIL)"""
@@ -772,7 +795,9 @@ def main():
         return [r for rr in res for r in rr]
 
     def split(k):
-        return [k // len(lanes) + (1 if j < k % len(lanes) else 0) for j in range(len(lanes))]
+        """k spectra as launches of B per lane"""
+        nl = -(-k // B)
+        return [nl // len(lanes) + (1 if j < nl % len(lanes) else 0) for j in range(len(lanes))]
 
     # the project's autotuner picks the solver and LOS shapes for this workload
     # (untimed; the choice holds for every context on the device)
@@ -785,20 +810,23 @@ def main():
     run([max(c, 1) for c in split(args.warmup)], "warmup")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    timed = run(split(args.steps), "step")
+    counts = split(args.steps)
+    nspec = sum(counts) * B
+    timed = run(counts, "step")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kms = [(a, b) for a, b, _ in timed]
     lat = [w for _, _, w in timed]
-    dt = (t1 - t0) / args.steps
+    dt = (t1 - t0) / nspec
     for ln in lanes:
-        if int(ln.d_status.max().item()) != 0:
+        if int(ln.d_statusB.max().item()) != 0:
             raise RuntimeError("a k-mode failed in the benchmark run")
-    ctr = d_ctr.cpu().numpy()
+    ctr = d_ctr.cpu().numpy().reshape(-1, 4)
     steps_total = int(ctr[:, 0].sum() + ctr[:, 1].sum())
     acc_total = int(ctr[:, 0].sum())
     n = boltmi.state_length(cfg)
-    f_ode = ode_flops(n, NQ, steps_total, acc_total, NK, bg.nx)
+    # per solver launch (B spectra)
+    f_ode = ode_flops(n, NQ, steps_total, acc_total, NK * B, bg.nx)
     los_exec = int(d_steps.sum().item())
     nxi = int(np.sum(bg.x_grid > -8.0)) - 1
     f_los_nominal = 4.0 * len(ells) * (NKD - 1) * nxi
@@ -809,26 +837,29 @@ def main():
     los_tf_exec = 4.0 * los_exec / (ms_los * 1e-3) / 1e12
     if ms_ode >= ms_los:
         roof = {"kernel": "k_boltsolve", "bound": "fp64-vector", "achieved": ode_tf, "peak": PEAK_FP64,
-                "unit": "TFLOP/s", "frac": ode_tf / PEAK_FP64, "traffic": measured_traffic("k_boltsolve"),
-                "chip_level_tflops": f_ode / dt / 1e12,
+                "unit": "TFLOP/s", "frac": ode_tf / PEAK_FP64, "traffic": measured_traffic("k_boltsolve", B),
+                "chip_level_tflops": f_ode / B / dt / 1e12,
                 "note": "FP64 vector peak (no MFMA instruction is used); algorithmic flops of the structured "
-                        "ESDIRK solve per step/RHS/node, from the kernel's own step counters, over the kernel's "
-                        "average HIP-event duration (which with spectra in flight includes waiting for CUs); "
-                        "chip_level_tflops divides the same flops by the wall time per spectrum"}
+                        "ESDIRK solve per step/RHS/node, from the kernel's own step counters, for one launch "
+                        "(per_launch spectra) over the kernel's average HIP-event duration (which with batches in "
+                        "flight includes waiting for CUs); chip_level_tflops divides one spectrum's flops by the "
+                        "wall time per spectrum"}
     else:
         roof = {"kernel": "k_los", "bound": "fp64-vector", "achieved": los_tf, "peak": PEAK_FP64, "unit": "TFLOP/s",
                 "frac": los_tf / PEAK_FP64, "traffic": measured_traffic("k_los")}
     line = {
-        "metric": METRIC, "value": 1.0 / dt, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": 1.0 / dt, "unit": "spectra/s", "n_gpus": 1, "steps": nspec,
         "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "full CMB path: source_grid 800 log k in [1e-4,0.5]/Mpc (l_g=l_nu=50, l_mnu=10, "
                                "nq=15, reltol 1e-11) + C_l^TT l=2..2500 on quadratic_k(0.01H0,1000H0,5000)",
                    "n_kmodes": NK, "state_length": n, "n_ell": len(ells), "n_kdense": NKD,
-                   "inflight": len(lanes)},
+                   "inflight": len(lanes), "per_launch": B},
         "kmode_solves_per_s": NK / dt,
         "inflight": len(lanes),
-        "latency_ms": {"mean": float(np.mean(lat)), "max": float(np.max(lat))},
+        "per_launch": B,
+        "latency_ms": {"mean": float(np.mean(lat)), "max": float(np.max(lat)),
+                       "note": "wall time of one batch of per_launch spectra on its lane"},
         "kernels_ms": {"k_boltsolve": ms_ode, "k_los": ms_los},
         "ode_steps_total": steps_total,
         "los": {"nominal_tflops": los_tf, "executed_tflops": los_tf_exec, "executed_steps": los_exec,
