@@ -7111,6 +7111,8 @@ int bolt_device_tables_batch(bolt_ctx* ctx, const bolt_cosmo_params* pars, int n
     }
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
+    // RECFAST's constants do not depend on the cosmology: calls from several
+    // contexts at once write the same bytes into the device's constant block
     const gbg::RfK rk = gbg::rf_constants();
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(gbg::c_rfk), &rk, sizeof(rk), 0, hipMemcpyHostToDevice, st));
     const size_t ntab = (size_t)BOLT_NTAB * (nx + 2);
