@@ -61,6 +61,16 @@ def test_filon_known_answers(e):
     assert err(v, e["ref"], e["kind"]) < e["tol"]
 
 
+def test_filon_loop_known_answer():
+    """testbessel.jl:289-291: the piecewise form (the moment at k·a carried over
+    from the previous interval) gives the same known answer on (1, 2); here the
+    oracle's loop over the nodes t = 1, 2"""
+    e = next(e for e in G["filon"] if e["line"] == 274 and e["a"] == 1.0 and e["b"] == 2.0)
+    T = table(e["table"])
+    v = T.filon_sum([1.0, 2.0], [e["f"], 0.0], [e["fp"], 0.0], [e["fpp"], 0.0], [e["k"]])
+    assert err(float(v[0]), e["ref"], e["kind"]) < e["tol"]
+
+
 def _mp_moment(x, nu, m):
     mp = pytest.importorskip("mpmath")
     mp.mp.dps = 30

@@ -74,6 +74,14 @@ def test_known_answers_of_the_filon_rule():
         assert boltmi.integrate_sph_bessel_filon(e0["f"], e0["fp"], e0["fpp"], e0["k"], e0["a"], e0["b"], T) == v[0]
 
 
+def test_known_answer_of_the_filon_loop():
+    """testbessel.jl:289-291: the piecewise loop over the nodes t = 1, 2
+    (bolt_filon_sum) gives the same known answer as the single interval"""
+    e = next(e for e in G["filon"] if e["line"] == 274 and e["a"] == 1.0 and e["b"] == 2.0)
+    v = boltmi.filon_sum(dev_table(e["table"]), [1.0, 2.0], [e["f"], 0.0], [e["fp"], 0.0], [e["fpp"], 0.0], [e["k"]])
+    assert err(float(v[0]), e["ref"], e["kind"]) < e["tol"]
+
+
 def envelope(x, m):
     """the scale of I_m's terms at x, ~x^{m−1}: FP64 evaluations carry ε of it"""
     return np.maximum(1.0, np.maximum(np.abs(x), 1.0) ** (np.asarray(m) - 1.0))
