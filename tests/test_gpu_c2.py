@@ -53,6 +53,16 @@ def test_c2_source_columns_match_oracle(c2):
     assert refs[-1][1].accepted > 5000   # k = 0.5: the longest mode of the launch
 
 
+def test_c2_two_spectra_per_launch_bitwise(c2):
+    """bench.py's default shape: two spectra of the cosmology in one persistent
+    launch (1600 modes, largest k first) give bitwise the single-spectrum grid"""
+    par, bg, ih, ks, sf = c2
+    St, _ = boltmi.source_grid_batch([(par, bg, ih)] * 2, ks, ℓᵧ=50, reltol=1e-11, ℓ_ν=50, ℓ_mν=10, nq=15,
+                                     abstol=1e-6)
+    for g in St:
+        assert np.array_equal(g.grid, sf.grid)
+
+
 def test_c2_cltt_matches_oracle_every_ell(c2):
     par, bg, ih, ks, sf = c2
     o = cosmo.oracle("c1")
