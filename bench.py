@@ -186,8 +186,10 @@ def c1(args):
     Output: states at 1000 x from -20 to 0 (δ_c and Φ among them), then P(k)
     for the same 100 k (plin, reltol 1e-5).
     One step is both; value is k-mode solves/s of wall time, states copied to
-    the host included.  Beside it: the oracle on a sample of the modes (its
-    time and the largest scaled difference of δ_c and Φ)."""
+    the host included.  With --inflight N (default 3) the steps run on N contexts
+    (each its own HIP stream and host thread), as the default line does.  Beside it: the
+    oracle on a sample of the modes (its time and the largest scaled difference
+    of δ_c and Φ)."""
     import torch
     import boltmi
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -199,17 +201,40 @@ def c1(args):
     cfg = boltmi.hier_cfg(10, 10, 10, NQ, 1e-6, 1e-6)
     n = boltmi.state_length(cfg)
 
-    def step():
-        out, status, st = boltmi.boltsolve_batch(par, bg, cfg, ks, xs)
-        pk = boltmi.plin(ks, par, bg, ih, n_q=NQ, ℓᵧ=10, ℓ_ν=10, ℓ_mν=10)
+    nin = args.inflight or 3
+    ctxs = [boltmi.default_context()] + [boltmi.Context(0) for _ in range(max(1, nin) - 1)]
+
+    def step(ctx):
+        out, status, st = boltmi.boltsolve_batch(par, bg, cfg, ks, xs, ctx=ctx)
+        pk = boltmi.plin(ks, par, bg, ih, n_q=NQ, ℓᵧ=10, ℓ_ν=10, ℓ_mν=10, ctx=ctx)
         return out, st, pk
 
-    for _ in range(max(args.warmup, 1)):
-        step()
+    def run(nsteps):
+        """nsteps steps spread over the contexts, each context on its own thread"""
+        import threading
+        counts = [nsteps // len(ctxs) + (1 if j < nsteps % len(ctxs) else 0) for j in range(len(ctxs))]
+        res, err = [None] * len(ctxs), []
+
+        def work(j):
+            try:
+                for _ in range(counts[j]):
+                    res[j] = step(ctxs[j])
+            except Exception as e:  # reported after the join
+                err.append(e)
+
+        th = [threading.Thread(target=work, args=(j,)) for j in range(len(ctxs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if err:
+            raise err[0]
+        return res[0]
+
+    run(max(args.warmup, len(ctxs)))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out, st, pk = step()
+    out, st, pk = run(args.steps)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     line = {"metric": "k-mode solves/s, plain-method workload (solve with 1000 saved states + P(k))",
@@ -217,8 +242,8 @@ def c1(args):
             "ms_per_step": dt * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "flat LCDM, l_g=l_nu=l_mnu=10, nq=15 (n=203), 100 log k in [1e-4,1] h/Mpc, "
                                    "reltol=abstol=1e-6, states at 1000 x, then P(k) at the same k",
-                       "n_kmodes": len(ks), "state_length": n},
-            "solver_kernel_ms": st["kernel_ms"], "steps_accepted": st["accepted"]}
+                       "n_kmodes": len(ks), "state_length": n, "inflight": len(ctxs)},
+            "inflight": len(ctxs), "solver_kernel_ms": st["kernel_ms"], "steps_accepted": st["accepted"]}
     if not args.no_cpu:
         import pyoracle
         from concurrent.futures import ThreadPoolExecutor
@@ -642,8 +667,9 @@ def main():
     ap.add_argument("--steps", type=int, default=12,
                     help="spectra timed (rounded up to a whole number of launches of --per-launch spectra)")
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--inflight", type=int, default=2,
-                    help="independent batches in flight, each on its own context (HIP stream) and host thread")
+    ap.add_argument("--inflight", type=int, default=None,
+                    help="independent batches in flight, each on its own context (HIP stream) and host thread "
+                         "(default 2; 3 for --c1)")
     ap.add_argument("--per-launch", type=int, default=2, help="spectra solved by one persistent solver launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-mode-per-launch GPU baseline")
@@ -704,7 +730,7 @@ def main():
     from boltmi import _abi
 
     par, bg, ih, ks, cfg, ells, dense = workload()
-    nlanes = max(1, args.inflight)
+    nlanes = max(1, args.inflight or 2)
     B = max(1, args.per_launch)
     L = _abi.lib()
     dev = torch.device("cuda:0")
